@@ -1,0 +1,221 @@
+/*
+ * pathtracer.h — C ABI of the MI355X (gfx950) spectral path tracer.
+ *
+ * The reference renders through one Vulkan compute dispatch of
+ * src/shader.comp. Its "operator API" is:
+ *   - a 16,388-byte uniform buffer `UniformBufferObject`
+ *     (src/pathtracer.cpp:187-195, packed by UpdateUniformBuffer :3642-3811),
+ *   - 88 bytes of push constants `PushConstantValues`
+ *     (src/pathtracer.cpp:197-218, filled by UpdatePushConstant :3813-3834),
+ *   - an RGBA32F CIE-XYZ texel buffer of W*H*16 bytes that the kernel
+ *     reads, blends and writes (src/pathtracer.cpp:2250-2255,
+ *     src/shader.comp:1525-1533),
+ *   - SDF programs given as GLSL text in the scene JSON and spliced into the
+ *     shader (src/pathtracer.cpp:2004-2054),
+ *   - vkCmdDispatch(ceil(W/16), ceil(H/16), 1) per frame
+ *     (src/pathtracer.cpp:3603), after the caller advanced `frame` and
+ *     `currentSamples` (src/pathtracer.cpp:4042-4048).
+ * The entry points below replace exactly that contract. Plain C types only;
+ * no exception crosses this boundary. Every call returns a pt_status; the
+ * message for the last failure on a context is pt_last_error(ctx).
+ *
+ * Threading: a context is externally synchronised (one host thread at a
+ * time), as the reference's single render thread is. Work is asynchronous on
+ * the context's HIP stream until pt_sync() or a read-back call.
+ */
+#ifndef PATHTRACER_H
+#define PATHTRACER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int pt_status;
+#define PT_OK 0
+#define PT_ERR_INVALID_ARG (-1)
+#define PT_ERR_JSON (-2)
+#define PT_ERR_CAPACITY (-3)
+#define PT_ERR_UNSUPPORTED_SDF (-4)
+#define PT_ERR_HIP (-5)
+#define PT_ERR_NO_DEVICE (-6)
+#define PT_ERR_BUDGET (-7)
+#define PT_ERR_STATE (-8)
+
+/* Capacities of the packed scene; same values as src/shader.comp:11-15 and
+ * src/pathtracer.cpp:39-43. */
+#define PT_MAX_OBJECTS_SIZE 1024
+#define PT_MAX_SDFS_SIZE 768
+#define PT_MAX_MATERIALS_SIZE 783
+#define PT_MAX_LIGHTS_SIZE 128
+#define PT_MAX_LIGHTIDS_SIZE 64
+#define PT_CIE_SIZE 1323
+/* SDF slots addressable by the reference's 32-bit set mask (src/shader.comp:753-770). */
+#define PT_MAX_SDF_OBJECTS 32
+
+/* Binary-identical to UniformBufferObject (src/pathtracer.cpp:187-195). */
+typedef struct pt_packed_scene {
+    float numObjects[7]; /* spheres, planes, boxes, lenses, cyclides, sdfs, lights */
+    float objects[PT_MAX_OBJECTS_SIZE];
+    float sdfs[PT_MAX_SDFS_SIZE];
+    float materials[PT_MAX_MATERIALS_SIZE];
+    float lights[PT_MAX_LIGHTS_SIZE];
+    float lightIDs[PT_MAX_LIGHTIDS_SIZE];
+    float cie[PT_CIE_SIZE];
+} pt_packed_scene;
+
+/* Binary-identical to PushConstantValues (src/pathtracer.cpp:197-218). */
+typedef struct pt_push_constants {
+    int32_t resolution[2];
+    int32_t frame;           /* samples dispatched so far, including this dispatch */
+    int32_t currentSamples;  /* samples in the running mean after this dispatch */
+    int32_t samplesPerFrame;
+    float FPS;
+    float persistence;
+    int32_t pathLength;      /* maximum segments per path */
+    float cameraAngle[2];    /* (-pitch, yaw) degrees, src/pathtracer.cpp:3821 */
+    float cameraPosX, cameraPosY, cameraPosZ;
+    int32_t ISO;
+    float cameraSize;
+    float apertureSize;
+    float apertureDist;
+    float lensRadius;
+    float lensFocalLength;
+    float lensThickness;
+    float lensDistance;
+    int32_t tonemap;
+} pt_push_constants;
+
+/* Built-in SDF programs. The reference splices arbitrary GLSL; a HIP build
+ * cannot compile GLSL, so pt_sdf_recognize maps the shipped programs
+ * (sdfs/{blob,mandelbulb,menger,terrain}.glsl and the `glsl` strings of scenes/scene{3..10}.json) to these kinds. */
+#define PT_SDF_NONE 0
+#define PT_SDF_MANDELBULB 1 /* power-8 Chebyshev form, sdfs/mandelbulb.glsl */
+#define PT_SDF_MENGER 2     /* sdfs/menger.glsl */
+#define PT_SDF_BLOB 3       /* four smooth-min spheres, sdfs/blob.glsl */
+#define PT_SDF_TERRAIN 4    /* sum of sines height field, sdfs/terrain.glsl */
+
+#define PT_SDFMAT_CONSTANT 0 /* material = m[0] */
+#define PT_SDFMAT_RADIAL 1   /* mix(m[0], m[1], |p|^2 / (m[2] + |p|^2)) */
+#define PT_SDFMAT_BLOB 2     /* smooth-min of the four blob spheres carrying m[0..3] */
+
+typedef struct pt_sdf_desc {
+    int32_t kind;          /* PT_SDF_* */
+    int32_t iterations;    /* mandelbulb iterations / menger levels */
+    float bailout;         /* mandelbulb escape radius */
+    int32_t material_kind; /* PT_SDFMAT_* */
+    float material[4];
+    float pos[3];          /* bounding-box centre; the SDF is evaluated at p - pos */
+    float size[3];         /* bounding-box size */
+} pt_sdf_desc;
+
+/* Tracing semantics.
+ * PT_MODE_REF: the reference's method bit for bit (4 rotated wavelengths in
+ *   [390, 720) nm, sequential PCG chain keyed on (frame, pixel), brute-force
+ *   closest hit, its MIS/RR rules and quirks).
+ * PT_MODE_EXT: the extended physically based mode (quads, multi-Gaussian
+ *   reflectance, dispersive dielectrics, conductors, environment light, BVH,
+ *   hero wavelengths over [380, 780) nm, counter-based PCG keyed on
+ *   (seed, pixel, sample, bounce, dimension)). */
+typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
+
+/* Launch/tuning knobs (0 = library default). */
+typedef struct pt_config {
+    int32_t wave_paths;    /* paths in flight per wave (queue capacity) */
+    int32_t block_size;    /* threads per block of the queue kernels (multiple of 64) */
+    int32_t bvh_width;     /* 2, 4 or 8 */
+    int32_t sdf_max_steps; /* sphere-tracing budget of the extended mode */
+    int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
+} pt_config;
+
+typedef struct pt_counters {
+    uint64_t segments;        /* extension rays traced (one per path segment) */
+    uint64_t shadow_segments; /* NEE visibility rays traced */
+    uint64_t sdf_steps;       /* sphere-tracing SDF evaluations */
+    uint64_t paths;           /* camera paths started */
+    uint64_t nan_samples;     /* samples discarded as NaN (src/shader.comp:1479-1487) */
+    uint64_t dispatches;
+} pt_counters;
+
+typedef struct pt_ctx pt_ctx;
+typedef struct pt_scene pt_scene;
+typedef struct pt_accum pt_accum;
+
+/* Context: one HIP device and one stream (the reference's device + compute queue,
+ * src/pathtracer.cpp:1534-1536). */
+pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg);
+void pt_destroy(pt_ctx* ctx);
+const char* pt_last_error(const pt_ctx* ctx);
+/* Message of the last failure of a call that had no context (create, recognise). */
+const char* pt_last_error_global(void);
+
+/* Scene from the reference's JSON format (ReadJSON + UpdateFromJSON +
+ * UpdateUniformBuffer + InsertSDF, src/pathtracer.cpp:893-908, 2576-2722,
+ * 3642-3811, 2004-2054). Unknown SDF text fails with PT_ERR_UNSUPPORTED_SDF.
+ * The extended keys (quad, material types, environment, render) are accepted
+ * and used by PT_MODE_EXT. */
+pt_status pt_scene_from_json(pt_ctx* ctx, const char* json, size_t len, pt_scene** out);
+/* Scene from an already packed uniform buffer plus SDF descriptors
+ * (UpdateUniformBuffer's output, src/pathtracer.cpp:3642-3811). */
+pt_status pt_scene_from_packed(pt_ctx* ctx, const pt_packed_scene* ubo, const pt_sdf_desc* sdfs,
+                               int n_sdfs, pt_scene** out);
+/* Copies the packed uniform buffer of a scene (the bytes the reference uploads). */
+pt_status pt_scene_get_packed(const pt_scene* scene, pt_packed_scene* out);
+/* Copies up to `cap` SDF descriptors; *n_out receives the scene's count. */
+pt_status pt_scene_get_sdfs(const pt_scene* scene, pt_sdf_desc* out, int cap, int* n_out);
+/* Fills the camera fields of `pc` from camera shot `shot` (1-based) as
+ * UpdateFromJSON + UpdatePushConstant do (src/pathtracer.cpp:2577-2606, 3813-3834). */
+pt_status pt_scene_camera(const pt_scene* scene, int shot, pt_push_constants* pc);
+/* Device bytes the scene holds (for admission by HBM footprint). */
+pt_status pt_scene_device_bytes(const pt_scene* scene, int64_t* bytes);
+void pt_scene_release(pt_scene* scene);
+
+/* GLSL SDF text -> built-in kind and parameters. */
+pt_status pt_sdf_recognize(const char* glsl, size_t len, pt_sdf_desc* out);
+
+/* Host-only packing, no device needed: the scene JSON -> the uniform buffer
+ * the reference uploads (UpdateFromJSON + UpdateUniformBuffer,
+ * src/pathtracer.cpp:2576-2722, 3642-3811), the camera fields of the push
+ * constants for camera shot `shot` (1-based, UpdatePushConstant :3813-3834),
+ * and up to `cap` SDF descriptors (*n_sdfs receives the count). Any output
+ * pointer may be NULL. */
+pt_status pt_pack_json(const char* json, size_t len, int shot, pt_packed_scene* ubo, pt_push_constants* pc,
+                       pt_sdf_desc* sdfs, int cap, int* n_sdfs);
+
+/* Accumulator: device RGBA32F, W*H texels, zeroed (the reference's texel buffer). */
+pt_status pt_accum_create(pt_ctx* ctx, int width, int height, pt_accum** out);
+pt_status pt_accum_reset(pt_accum* acc);
+/* Host copies of the texels (4 floats per pixel, row gid.y = 0 first). */
+pt_status pt_accum_read_xyz(pt_accum* acc, float* host_rgba, size_t n_floats);
+pt_status pt_accum_write_xyz(pt_accum* acc, const float* host_rgba, size_t n_floats);
+void pt_accum_release(pt_accum* acc);
+
+/* == vkCmdPushConstants + vkCmdDispatch + vkQueueSubmit: trace
+ * pc->samplesPerFrame samples per pixel and fold them into `acc`
+ * (src/shader.comp:1509-1533). Asynchronous on the context's stream. */
+pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* scene, pt_accum* acc, const pt_push_constants* pc,
+                      pt_mode mode);
+pt_status pt_sync(pt_ctx* ctx);
+
+/* Counters summed over dispatches since creation or the last reset (synchronises). */
+pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out);
+pt_status pt_counters_reset(pt_ctx* ctx);
+
+/* Display resolve of SaveRender / shader.frag (src/pathtracer.cpp:3491-3518,
+ * src/shader.frag:81-99): XYZ -> Bradford E->D65 -> linear sRGB -> tonemap
+ * (0 none, 1 Reinhard, 2 ACES, 3 DEUCES) -> sRGB OETF -> 8 bit. */
+pt_status pt_resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb, size_t n_bytes);
+
+/* Diagnostics: evaluates the tracer's FP32 math functions (fn: 0 sin, 1 cos,
+ * 2 exp, 3 log, 4 pow, 5 acos, 6 sqrt, 7 rsqrt) on the host build or on the
+ * device build, so the two can be compared bit for bit. */
+pt_status pt_math_eval_host(int fn, const float* x, const float* y, float* out, size_t n);
+pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PATHTRACER_H */

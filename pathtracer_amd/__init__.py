@@ -1,0 +1,421 @@
+"""pathtracer_amd — MI355X (gfx950) spectral path tracer behind a C ABI.
+
+This module is a thin ctypes binding of ``include/pathtracer.h`` (the
+library ``libpathtracer.so`` built in this directory) plus the reference
+application's frame bookkeeping (``Renderer``), so Python tests and benches
+drive exactly the C entry points a C++ host would.
+
+The compute path is only the HIP library: importing this module fails loudly
+when ``libpathtracer.so`` has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libpathtracer.so"
+
+PT_OK = 0
+PT_ERR_INVALID_ARG = -1
+PT_ERR_JSON = -2
+PT_ERR_CAPACITY = -3
+PT_ERR_UNSUPPORTED_SDF = -4
+PT_ERR_HIP = -5
+PT_ERR_NO_DEVICE = -6
+PT_ERR_BUDGET = -7
+PT_ERR_STATE = -8
+
+MODE_REF = 0
+MODE_EXT = 1
+
+SDF_MANDELBULB, SDF_MENGER, SDF_BLOB, SDF_TERRAIN = 1, 2, 3, 4
+SDFMAT_CONSTANT, SDFMAT_RADIAL, SDFMAT_BLOB = 0, 1, 2
+
+
+class PackedScene(C.Structure):
+    """UniformBufferObject (src/pathtracer.cpp:187-195), 16,388 bytes."""
+
+    _fields_ = [
+        ("numObjects", C.c_float * 7),
+        ("objects", C.c_float * 1024),
+        ("sdfs", C.c_float * 768),
+        ("materials", C.c_float * 783),
+        ("lights", C.c_float * 128),
+        ("lightIDs", C.c_float * 64),
+        ("cie", C.c_float * 1323),
+    ]
+
+
+class PushConstants(C.Structure):
+    """PushConstantValues (src/pathtracer.cpp:197-218), 88 bytes."""
+
+    _fields_ = [
+        ("resolution", C.c_int32 * 2),
+        ("frame", C.c_int32),
+        ("currentSamples", C.c_int32),
+        ("samplesPerFrame", C.c_int32),
+        ("FPS", C.c_float),
+        ("persistence", C.c_float),
+        ("pathLength", C.c_int32),
+        ("cameraAngle", C.c_float * 2),
+        ("cameraPosX", C.c_float),
+        ("cameraPosY", C.c_float),
+        ("cameraPosZ", C.c_float),
+        ("ISO", C.c_int32),
+        ("cameraSize", C.c_float),
+        ("apertureSize", C.c_float),
+        ("apertureDist", C.c_float),
+        ("lensRadius", C.c_float),
+        ("lensFocalLength", C.c_float),
+        ("lensThickness", C.c_float),
+        ("lensDistance", C.c_float),
+        ("tonemap", C.c_int32),
+    ]
+
+
+class SdfDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("iterations", C.c_int32),
+        ("bailout", C.c_float),
+        ("material_kind", C.c_int32),
+        ("material", C.c_float * 4),
+        ("pos", C.c_float * 3),
+        ("size", C.c_float * 3),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("wave_paths", C.c_int32),
+        ("block_size", C.c_int32),
+        ("bvh_width", C.c_int32),
+        ("sdf_max_steps", C.c_int32),
+        ("hbm_budget", C.c_int64),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("segments", C.c_uint64),
+        ("shadow_segments", C.c_uint64),
+        ("sdf_steps", C.c_uint64),
+        ("paths", C.c_uint64),
+        ("nan_samples", C.c_uint64),
+        ("dispatches", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+assert C.sizeof(PackedScene) == 16388
+assert C.sizeof(PushConstants) == 88
+
+# Every symbol include/pathtracer.h declares, with (restype, argtypes).
+_P = C.c_void_p
+SIGNATURES = {
+    "pt_create": (C.c_int, [C.POINTER(_P), C.c_int, C.POINTER(Config)]),
+    "pt_destroy": (None, [_P]),
+    "pt_last_error": (C.c_char_p, [_P]),
+    "pt_last_error_global": (C.c_char_p, []),
+    "pt_scene_from_json": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(_P)]),
+    "pt_scene_from_packed": (C.c_int, [_P, C.POINTER(PackedScene), C.POINTER(SdfDesc), C.c_int, C.POINTER(_P)]),
+    "pt_scene_get_packed": (C.c_int, [_P, C.POINTER(PackedScene)]),
+    "pt_scene_get_sdfs": (C.c_int, [_P, C.POINTER(SdfDesc), C.c_int, C.POINTER(C.c_int)]),
+    "pt_scene_camera": (C.c_int, [_P, C.c_int, C.POINTER(PushConstants)]),
+    "pt_scene_device_bytes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "pt_scene_release": (None, [_P]),
+    "pt_sdf_recognize": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(SdfDesc)]),
+    "pt_pack_json": (
+        C.c_int,
+        [C.c_char_p, C.c_size_t, C.c_int, C.POINTER(PackedScene), C.POINTER(PushConstants), C.POINTER(SdfDesc),
+         C.c_int, C.POINTER(C.c_int)],
+    ),
+    "pt_accum_create": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(_P)]),
+    "pt_accum_reset": (C.c_int, [_P]),
+    "pt_accum_read_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "pt_accum_write_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "pt_accum_release": (None, [_P]),
+    "pt_dispatch": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants), C.c_int]),
+    "pt_sync": (C.c_int, [_P]),
+    "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
+    "pt_counters_reset": (C.c_int, [_P]),
+    "pt_resolve_srgb8": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint8), C.c_size_t]),
+    "pt_math_eval_host": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "pt_math_eval_device": (
+        C.c_int,
+        [_P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t],
+    ),
+}
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `make -C {_HERE}` (or __graft_entry__.build()) first; "
+            "there is no CPU fallback"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class PTError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+def _check(st: int, ctx=None) -> None:
+    if st != PT_OK:
+        if ctx is not None:
+            msg = lib.pt_last_error(ctx)
+        else:
+            msg = lib.pt_last_error_global()
+        raise PTError(st, (msg or b"").decode(errors="replace"))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def recognize_sdf(glsl: str) -> SdfDesc:
+    d = SdfDesc()
+    b = glsl.encode()
+    _check(lib.pt_sdf_recognize(b, len(b), C.byref(d)))
+    return d
+
+
+def pack_json(text: str, shot: int = 1, pc: PushConstants | None = None):
+    """Host-only UpdateFromJSON + UpdateUniformBuffer: returns (PackedScene,
+    PushConstants with the camera of `shot`, [SdfDesc])."""
+    b = text.encode()
+    ubo = PackedScene()
+    pc = pc if pc is not None else PushConstants()
+    n = C.c_int()
+    _check(lib.pt_pack_json(b, len(b), shot, C.byref(ubo), C.byref(pc), None, 0, C.byref(n)))
+    arr = (SdfDesc * max(1, n.value))()
+    _check(lib.pt_pack_json(b, len(b), shot, None, None, arr, n.value, C.byref(n)))
+    return ubo, pc, [arr[i] for i in range(n.value)]
+
+
+def math_eval_host(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    yy = np.ascontiguousarray(y, dtype=np.float32) if y is not None else None
+    out = np.empty_like(x)
+    _check(lib.pt_math_eval_host(fn, _fptr(x), _fptr(yy) if yy is not None else None, _fptr(out), x.size))
+    return out
+
+
+class Context:
+    """pt_ctx: one device, one stream."""
+
+    def __init__(self, device: int = 0, wave_paths: int = 0, block_size: int = 0, bvh_width: int = 0,
+                 sdf_max_steps: int = 0, hbm_budget: int = 0):
+        cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget)
+        h = C.c_void_p()
+        _check(lib.pt_create(C.byref(h), device, C.byref(cfg)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            lib.pt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int) -> None:
+        _check(st, self.h)
+
+    def scene_from_json(self, text: str) -> "Scene":
+        b = text.encode()
+        h = C.c_void_p()
+        self.check(lib.pt_scene_from_json(self.h, b, len(b), C.byref(h)))
+        return Scene(self, h)
+
+    def scene_from_packed(self, ubo: PackedScene, sdfs: list[SdfDesc]) -> "Scene":
+        arr = (SdfDesc * max(1, len(sdfs)))(*sdfs)
+        h = C.c_void_p()
+        self.check(lib.pt_scene_from_packed(self.h, C.byref(ubo), arr, len(sdfs), C.byref(h)))
+        return Scene(self, h)
+
+    def accum(self, w: int, h: int) -> "Accum":
+        a = C.c_void_p()
+        self.check(lib.pt_accum_create(self.h, w, h, C.byref(a)))
+        return Accum(self, a, w, h)
+
+    def dispatch(self, scene: "Scene", acc: "Accum", pc: PushConstants, mode: int = MODE_REF) -> None:
+        self.check(lib.pt_dispatch(self.h, scene.h, acc.h, C.byref(pc), mode))
+
+    def sync(self) -> None:
+        self.check(lib.pt_sync(self.h))
+
+    def counters(self) -> dict:
+        c = Counters()
+        self.check(lib.pt_counters_get(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self) -> None:
+        self.check(lib.pt_counters_reset(self.h))
+
+    def math_eval(self, fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        yy = np.ascontiguousarray(y, dtype=np.float32) if y is not None else None
+        out = np.empty_like(x)
+        self.check(lib.pt_math_eval_device(self.h, fn, _fptr(x), _fptr(yy) if yy is not None else None,
+                                           _fptr(out), x.size))
+        return out
+
+
+class Scene:
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.h = h
+
+    def release(self) -> None:
+        if self.h:
+            lib.pt_scene_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h:
+                self.release()
+        except Exception:
+            pass
+
+    def packed(self) -> PackedScene:
+        u = PackedScene()
+        _check(lib.pt_scene_get_packed(self.h, C.byref(u)))
+        return u
+
+    def sdfs(self) -> list[SdfDesc]:
+        n = C.c_int()
+        _check(lib.pt_scene_get_sdfs(self.h, None, 0, C.byref(n)))
+        arr = (SdfDesc * max(1, n.value))()
+        _check(lib.pt_scene_get_sdfs(self.h, arr, n.value, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def camera(self, shot: int, pc: PushConstants) -> PushConstants:
+        _check(lib.pt_scene_camera(self.h, shot, C.byref(pc)))
+        return pc
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        _check(lib.pt_scene_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+
+class Accum:
+    def __init__(self, ctx: Context, h, w: int, hh: int):
+        self.ctx, self.h, self.w, self.hgt = ctx, h, w, hh
+
+    def release(self) -> None:
+        if self.h:
+            lib.pt_accum_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h:
+                self.release()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        self.ctx.check(lib.pt_accum_reset(self.h))
+
+    def read(self) -> np.ndarray:
+        out = np.empty((self.hgt, self.w, 4), dtype=np.float32)
+        self.ctx.check(lib.pt_accum_read_xyz(self.h, _fptr(out), out.size))
+        return out
+
+    def write(self, texels: np.ndarray) -> None:
+        t = np.ascontiguousarray(texels, dtype=np.float32)
+        self.ctx.check(lib.pt_accum_write_xyz(self.h, _fptr(t), t.size))
+
+    def resolve_srgb8(self, tonemap: int = 3) -> np.ndarray:
+        out = np.empty((self.hgt, self.w, 3), dtype=np.uint8)
+        self.ctx.check(lib.pt_resolve_srgb8(self.ctx.h, self.h, tonemap,
+                                            out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size))
+        return out
+
+
+def push_constants(width: int, height: int, spf: int = 1, path_length: int = 5, persistence: float = 0.0625,
+                   fps: float = 1.0 / 0.0166, tonemap: int = 3) -> PushConstants:
+    """Push constants with the reference application's defaults
+    (src/pathtracer.cpp:1163-1174); camera fields come from Scene.camera()."""
+    pc = PushConstants()
+    pc.resolution[0] = width
+    pc.resolution[1] = height
+    pc.samplesPerFrame = spf
+    pc.pathLength = path_length
+    pc.persistence = persistence
+    pc.FPS = fps
+    pc.tonemap = tonemap
+    return pc
+
+
+class Renderer:
+    """Frame bookkeeping of the reference's MainLoop (src/pathtracer.cpp:4005-4050):
+    `frame` counts every sample dispatched, `currentSamples` restarts at
+    samplesPerFrame after a reset. Those two counters key the RNG and the
+    running mean, so a sequence of render_frame() calls reproduces the
+    reference's sequence of dispatches."""
+
+    def __init__(self, ctx: Context, scene: Scene, width: int, height: int, spf: int = 1, path_length: int = 5,
+                 shot: int = 1, mode: int = MODE_REF, tonemap: int = 3):
+        self.ctx, self.scene, self.mode = ctx, scene, mode
+        self.pc = push_constants(width, height, spf, path_length, tonemap=tonemap)
+        scene.camera(shot, self.pc)
+        self.acc = ctx.accum(width, height)
+        self.frame = 0
+        self.current = 0
+        self.reset_pending = False
+
+    def reset(self) -> None:
+        self.reset_pending = True
+
+    def render_frame(self) -> None:
+        spf = self.pc.samplesPerFrame
+        self.frame += spf
+        if self.reset_pending:
+            self.current = spf
+            self.reset_pending = False
+        else:
+            self.current += spf
+        self.pc.frame = self.frame
+        self.pc.currentSamples = self.current
+        self.ctx.dispatch(self.scene, self.acc, self.pc, self.mode)
+
+    def render(self, total_samples: int) -> np.ndarray:
+        while self.current < total_samples:
+            self.render_frame()
+        return self.acc.read()
+
+    def save_ppm(self, path: str | os.PathLike, tonemap: int | None = None) -> None:
+        rgb = self.acc.resolve_srgb8(self.pc.tonemap if tonemap is None else tonemap)
+        with open(path, "wb") as f:
+            f.write(b"P6\n%d\n%d\n255\n" % (self.acc.w, self.acc.hgt))
+            f.write(rgb.tobytes())

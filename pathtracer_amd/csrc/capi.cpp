@@ -1,0 +1,405 @@
+// capi.cpp — the C ABI declared in include/pathtracer.h.
+//
+// Errors never cross the boundary as exceptions: every entry point catches,
+// stores a message (per context, or thread-local for context-free calls) and
+// returns a PT_ERR_* status, as the reference's VkResult checks turn into one
+// failure per call (src/pathtracer.cpp:3878-3880).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/pathtracer.h"
+#include "scene.hpp"
+#include "tracer.hpp"
+
+namespace {
+thread_local std::string g_err;
+
+int set_global(int st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+int set_err(pt_ctx* ctx, int st, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return st;
+}
+}  // namespace
+
+namespace ptk {
+
+int hip_fail(pt_ctx* ctx, hipError_t e, const char* what) {
+    ctx->err = std::string(what) + ": " + hipGetErrorString(e);
+    return PT_ERR_HIP;
+}
+
+int dev_alloc(pt_ctx* ctx, DevAlloc& a, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (ctx->cfg.hbm_budget > 0 && ctx->allocated + (int64_t)bytes > ctx->cfg.hbm_budget) {
+        ctx->err = "HBM budget exceeded: " + std::to_string(ctx->allocated + (int64_t)bytes) + " > " +
+                   std::to_string(ctx->cfg.hbm_budget) + " bytes";
+        return PT_ERR_BUDGET;
+    }
+    hipError_t e = hipMalloc(&a.p, bytes);
+    if (e != hipSuccess) {
+        a.p = nullptr;
+        return hip_fail(ctx, e, "hipMalloc");
+    }
+    a.bytes = bytes;
+    ctx->allocated += (int64_t)bytes;
+    return PT_OK;
+}
+
+void dev_free(pt_ctx* ctx, DevAlloc& a) {
+    if (a.p) {
+        (void)hipFree(a.p);
+        ctx->allocated -= (int64_t)a.bytes;
+    }
+    a.p = nullptr;
+    a.bytes = 0;
+}
+
+}  // namespace ptk
+
+extern "C" {
+
+const char* pt_last_error(const pt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+const char* pt_last_error_global(void) { return g_err.c_str(); }
+
+pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
+    if (!out) return set_global(PT_ERR_INVALID_ARG, "pt_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_global(PT_ERR_NO_DEVICE, "pt_create: no HIP device is visible");
+    if (device_ordinal < 0 || device_ordinal >= ndev)
+        return set_global(PT_ERR_INVALID_ARG, "pt_create: device ordinal out of range");
+    pt_ctx* ctx = new (std::nothrow) pt_ctx();
+    if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_create: out of host memory");
+    ctx->device = device_ordinal;
+    if (cfg) ctx->cfg = *cfg;
+    if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && (ctx->cfg.block_size % 64) != 0) ||
+        (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8)) {
+        delete ctx;
+        return set_global(PT_ERR_INVALID_ARG, "pt_create: invalid pt_config");
+    }
+    hipError_t e = hipSetDevice(device_ordinal);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(ctx->d_stats, 0, 8 * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+        std::string msg = std::string("pt_create: ") + hipGetErrorString(e);
+        if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_global(PT_ERR_HIP, msg);
+    }
+    *out = ctx;
+    return PT_OK;
+}
+
+void pt_destroy(pt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ptk::ref_wave_release(ctx);
+    ptk::ext_wave_release(ctx);
+    if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+static pt_status finish_scene(pt_ctx* ctx, pt_scene* s, pt_scene** out) {
+    if (s->ref_ok) {
+        int st = ptk::ref_scene_upload(ctx, s);
+        if (st != PT_OK) {
+            pt_scene_release(s);
+            return st;
+        }
+    }
+    *out = s;
+    return PT_OK;
+}
+
+pt_status pt_scene_from_json(pt_ctx* ctx, const char* json, size_t len, pt_scene** out) {
+    if (!ctx || !json || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_scene_from_json: NULL argument");
+    *out = nullptr;
+    try {
+        (void)hipSetDevice(ctx->device);
+        pt_scene* s = new pt_scene();
+        s->ctx = ctx;
+        std::string err;
+        int st = pt::load_scene_json(json, len, s->model, err);
+        if (st != PT_OK) {
+            delete s;
+            return set_err(ctx, st, "scene JSON: " + err);
+        }
+        s->has_model = true;
+        for (const pt::RSdf& d : s->model.sdfs) s->sdfs.push_back(d.desc);
+        std::string perr;
+        s->ref_ok = pt::pack_reference_ubo(s->model, s->ubo, perr) == PT_OK;
+        if (!s->ref_ok) s->ref_err = perr;
+        return finish_scene(ctx, s, out);
+    } catch (const std::exception& e) {
+        return set_err(ctx, PT_ERR_JSON, std::string("pt_scene_from_json: ") + e.what());
+    }
+}
+
+pt_status pt_scene_from_packed(pt_ctx* ctx, const pt_packed_scene* ubo, const pt_sdf_desc* sdfs, int n_sdfs,
+                               pt_scene** out) {
+    if (!ctx || !ubo || !out || n_sdfs < 0 || (n_sdfs > 0 && !sdfs))
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_scene_from_packed: bad argument");
+    *out = nullptr;
+    try {
+        (void)hipSetDevice(ctx->device);
+        pt_scene* s = new pt_scene();
+        s->ctx = ctx;
+        s->ubo = *ubo;
+        s->sdfs.assign(sdfs, sdfs + n_sdfs);
+        s->ref_ok = true;
+        return finish_scene(ctx, s, out);
+    } catch (const std::exception& e) {
+        return set_err(ctx, PT_ERR_INVALID_ARG, std::string("pt_scene_from_packed: ") + e.what());
+    }
+}
+
+pt_status pt_scene_get_packed(const pt_scene* s, pt_packed_scene* out) {
+    if (!s || !out) return set_global(PT_ERR_INVALID_ARG, "pt_scene_get_packed: NULL argument");
+    if (!s->ref_ok) return set_global(PT_ERR_CAPACITY, "scene has no reference packing: " + s->ref_err);
+    *out = s->ubo;
+    return PT_OK;
+}
+
+pt_status pt_scene_get_sdfs(const pt_scene* s, pt_sdf_desc* out, int cap, int* n_out) {
+    if (!s || !n_out || cap < 0 || (cap > 0 && !out)) return set_global(PT_ERR_INVALID_ARG, "pt_scene_get_sdfs: bad argument");
+    *n_out = (int)s->sdfs.size();
+    for (int i = 0; i < cap && i < (int)s->sdfs.size(); ++i) out[i] = s->sdfs[(size_t)i];
+    return PT_OK;
+}
+
+pt_status pt_scene_camera(const pt_scene* s, int shot, pt_push_constants* pc) {
+    if (!s || !pc) return set_global(PT_ERR_INVALID_ARG, "pt_scene_camera: NULL argument");
+    if (!s->has_model) return set_global(PT_ERR_STATE, "pt_scene_camera: scene was not loaded from JSON");
+    std::string err;
+    int st = pt::camera_to_push(s->model, shot, *pc, err);
+    if (st != PT_OK) return set_global(st, err);
+    return PT_OK;
+}
+
+pt_status pt_scene_device_bytes(const pt_scene* s, int64_t* bytes) {
+    if (!s || !bytes) return set_global(PT_ERR_INVALID_ARG, "pt_scene_device_bytes: NULL argument");
+    *bytes = s->device_bytes;
+    return PT_OK;
+}
+
+void pt_scene_release(pt_scene* s) {
+    if (!s) return;
+    pt_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ptk::dev_free(ctx, s->ref_mem);
+    ptk::ext_scene_release(ctx, s);
+    delete s;
+}
+
+pt_status pt_sdf_recognize(const char* glsl, size_t len, pt_sdf_desc* out) {
+    if (!glsl || !out) return set_global(PT_ERR_INVALID_ARG, "pt_sdf_recognize: NULL argument");
+    try {
+        std::string err;
+        int st = pt::recognize_sdf(std::string(glsl, len), *out, err);
+        if (st != PT_OK) return set_global(st, err);
+        return PT_OK;
+    } catch (const std::exception& e) {
+        return set_global(PT_ERR_UNSUPPORTED_SDF, e.what());
+    }
+}
+
+pt_status pt_pack_json(const char* json, size_t len, int shot, pt_packed_scene* ubo, pt_push_constants* pc,
+                       pt_sdf_desc* sdfs, int cap, int* n_sdfs) {
+    if (!json || cap < 0 || (cap > 0 && !sdfs)) return set_global(PT_ERR_INVALID_ARG, "pt_pack_json: bad argument");
+    try {
+        pt::SceneModel m;
+        std::string err;
+        int st = pt::load_scene_json(json, len, m, err);
+        if (st != PT_OK) return set_global(st, "scene JSON: " + err);
+        if (ubo) {
+            st = pt::pack_reference_ubo(m, *ubo, err);
+            if (st != PT_OK) return set_global(st, err);
+        }
+        if (pc) {
+            st = pt::camera_to_push(m, shot, *pc, err);
+            if (st != PT_OK) return set_global(st, err);
+        }
+        if (n_sdfs) *n_sdfs = (int)m.sdfs.size();
+        for (int i = 0; i < cap && i < (int)m.sdfs.size(); ++i) sdfs[i] = m.sdfs[(size_t)i].desc;
+        return PT_OK;
+    } catch (const std::exception& e) {
+        return set_global(PT_ERR_JSON, std::string("pt_pack_json: ") + e.what());
+    }
+}
+
+pt_status pt_accum_create(pt_ctx* ctx, int width, int height, pt_accum** out) {
+    if (!ctx || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create: NULL argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 28)
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create: resolution out of range");
+    (void)hipSetDevice(ctx->device);
+    pt_accum* a = new (std::nothrow) pt_accum();
+    if (!a) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create: out of host memory");
+    a->ctx = ctx;
+    a->w = width;
+    a->h = height;
+    DevAlloc m;
+    int st = ptk::dev_alloc(ctx, m, (size_t)width * height * 16);
+    if (st != PT_OK) {
+        delete a;
+        return st;
+    }
+    a->d_texels = static_cast<float*>(m.p);
+    hipError_t e = hipMemsetAsync(a->d_texels, 0, (size_t)width * height * 16, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(a->d_texels);
+        ctx->allocated -= (int64_t)width * height * 16;
+        delete a;
+        return ptk::hip_fail(ctx, e, "pt_accum_create");
+    }
+    *out = a;
+    return PT_OK;
+}
+
+pt_status pt_accum_reset(pt_accum* a) {
+    if (!a) return set_global(PT_ERR_INVALID_ARG, "pt_accum_reset: NULL argument");
+    (void)hipSetDevice(a->ctx->device);
+    hipError_t e = hipMemsetAsync(a->d_texels, 0, (size_t)a->w * a->h * 16, a->ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(a->ctx, e, "pt_accum_reset");
+    return PT_OK;
+}
+
+pt_status pt_accum_read_xyz(pt_accum* a, float* host, size_t n) {
+    if (!a || !host) return set_global(PT_ERR_INVALID_ARG, "pt_accum_read_xyz: NULL argument");
+    if (n != (size_t)a->w * a->h * 4) return set_err(a->ctx, PT_ERR_INVALID_ARG, "pt_accum_read_xyz: size mismatch");
+    (void)hipSetDevice(a->ctx->device);
+    hipError_t e = hipMemcpyAsync(host, a->d_texels, n * 4, hipMemcpyDeviceToHost, a->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(a->ctx, e, "pt_accum_read_xyz");
+    return PT_OK;
+}
+
+pt_status pt_accum_write_xyz(pt_accum* a, const float* host, size_t n) {
+    if (!a || !host) return set_global(PT_ERR_INVALID_ARG, "pt_accum_write_xyz: NULL argument");
+    if (n != (size_t)a->w * a->h * 4) return set_err(a->ctx, PT_ERR_INVALID_ARG, "pt_accum_write_xyz: size mismatch");
+    (void)hipSetDevice(a->ctx->device);
+    hipError_t e = hipMemcpyAsync(a->d_texels, host, n * 4, hipMemcpyHostToDevice, a->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(a->ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(a->ctx, e, "pt_accum_write_xyz");
+    return PT_OK;
+}
+
+void pt_accum_release(pt_accum* a) {
+    if (!a) return;
+    (void)hipSetDevice(a->ctx->device);
+    (void)hipStreamSynchronize(a->ctx->stream);
+    DevAlloc m;
+    m.p = a->d_texels;
+    m.bytes = (size_t)a->w * a->h * 16;
+    ptk::dev_free(a->ctx, m);
+    delete a;
+}
+
+pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc, pt_mode mode) {
+    if (!ctx || !s || !acc || !pc) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch: NULL argument");
+    if (s->ctx != ctx || acc->ctx != ctx) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch: objects of another context");
+    if (pc->resolution[0] != acc->w || pc->resolution[1] != acc->h)
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch: push-constant resolution differs from the accumulator");
+    if (pc->samplesPerFrame <= 0 || pc->pathLength < 0 || pc->pathLength > 1024 || pc->currentSamples < pc->samplesPerFrame)
+        return set_err(ctx, PT_ERR_INVALID_ARG,
+                       "pt_dispatch: need samplesPerFrame > 0, 0 <= pathLength <= 1024, currentSamples >= samplesPerFrame");
+    (void)hipSetDevice(ctx->device);
+    try {
+        int st;
+        if (mode == PT_MODE_REF) {
+            if (!s->ref_ok) return set_err(ctx, PT_ERR_CAPACITY, "reference mode cannot hold this scene: " + s->ref_err);
+            st = ptk::ref_dispatch(ctx, s, acc, pc);
+        } else if (mode == PT_MODE_EXT) {
+            if (!s->has_model) return set_err(ctx, PT_ERR_STATE, "extended mode needs a scene loaded from JSON");
+            st = ptk::ext_dispatch(ctx, s, acc, pc);
+        } else {
+            return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch: unknown mode");
+        }
+        if (st == PT_OK) st = ptk::count_dispatch(ctx);
+        return st;
+    } catch (const std::exception& e) {
+        return set_err(ctx, PT_ERR_HIP, std::string("pt_dispatch: ") + e.what());
+    }
+}
+
+pt_status pt_sync(pt_ctx* ctx) {
+    if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_sync: NULL context");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_sync");
+    return PT_OK;
+}
+
+pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
+    if (!ctx || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_counters_get: NULL argument");
+    (void)hipSetDevice(ctx->device);
+    unsigned long long h[8];
+    hipError_t e = hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_get");
+    out->segments = h[0];
+    out->shadow_segments = h[1];
+    out->sdf_steps = h[2];
+    out->paths = h[3];
+    out->nan_samples = h[4];
+    out->dispatches = h[5];
+    return PT_OK;
+}
+
+pt_status pt_counters_reset(pt_ctx* ctx) {
+    if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_counters_reset: NULL context");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_reset");
+    return PT_OK;
+}
+
+pt_status pt_resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb, size_t n_bytes) {
+    if (!ctx || !acc || !host_rgb) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_resolve_srgb8: NULL argument");
+    if (n_bytes != (size_t)acc->w * acc->h * 3) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_resolve_srgb8: size mismatch");
+    if (tonemap < 0 || tonemap > 3) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_resolve_srgb8: tonemap must be 0..3");
+    (void)hipSetDevice(ctx->device);
+    return ptk::resolve_srgb8(ctx, acc, tonemap, host_rgb);
+}
+
+pt_status pt_math_eval_host(int fn, const float* x, const float* y, float* out, size_t n) {
+    if (!x || !out || fn < 0 || fn > 7 || (fn == 4 && !y)) return set_global(PT_ERR_INVALID_ARG, "pt_math_eval_host: bad argument");
+    for (size_t i = 0; i < n; ++i) {
+        float a = x[i];
+        switch (fn) {
+            case 0: out[i] = ptm_sin(a); break;
+            case 1: out[i] = ptm_cos(a); break;
+            case 2: out[i] = ptm_exp(a); break;
+            case 3: out[i] = ptm_log(a); break;
+            case 4: out[i] = ptm_pow(a, y[i]); break;
+            case 5: out[i] = ptm_acos(a); break;
+            case 6: out[i] = ptm_sqrt(a); break;
+            default: out[i] = ptm_rsqrt(a); break;
+        }
+    }
+    return PT_OK;
+}
+
+pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n) {
+    if (!ctx || !x || !out || fn < 0 || fn > 7 || (fn == 4 && !y))
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_math_eval_device: bad argument");
+    (void)hipSetDevice(ctx->device);
+    return ptk::math_eval_device(ctx, fn, x, y, out, n);
+}
+
+}  // extern "C"

@@ -1,0 +1,319 @@
+// json.cpp — see json.hpp.
+#include "json.hpp"
+
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+namespace ptjson {
+
+const Value* Value::get(const std::string& key) const {
+    if (kind != Kind::Object) return nullptr;
+    for (const auto& kv : obj)
+        if (kv.first == key) return &kv.second;
+    return nullptr;
+}
+
+size_t Value::size() const {
+    if (kind == Kind::Array) return arr.size();
+    if (kind == Kind::Object) return obj.size();
+    return 0;
+}
+
+const Value& Value::at(size_t idx) const {
+    if (kind != Kind::Array || idx >= arr.size()) throw std::runtime_error("array index out of range");
+    return arr[idx];
+}
+
+namespace {
+
+struct Parser {
+    const char* p;
+    const char* end;
+    const char* begin;
+    std::string err;
+
+    void fail(const char* why) {
+        if (!err.empty()) return;
+        int line = 1, col = 1;
+        for (const char* q = begin; q < p && q < end; ++q) {
+            if (*q == '\n') { ++line; col = 1; } else { ++col; }
+        }
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "line %d col %d: %s", line, col, why);
+        err = buf;
+    }
+    void ws() {
+        while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
+    }
+    bool lit(const char* w) {
+        size_t n = std::strlen(w);
+        if ((size_t)(end - p) >= n && std::memcmp(p, w, n) == 0) { p += n; return true; }
+        return false;
+    }
+    static void put_utf8(std::string& s, uint32_t c) {
+        if (c < 0x80) s.push_back((char)c);
+        else if (c < 0x800) { s.push_back((char)(0xC0 | (c >> 6))); s.push_back((char)(0x80 | (c & 0x3F))); }
+        else if (c < 0x10000) {
+            s.push_back((char)(0xE0 | (c >> 12))); s.push_back((char)(0x80 | ((c >> 6) & 0x3F)));
+            s.push_back((char)(0x80 | (c & 0x3F)));
+        } else {
+            s.push_back((char)(0xF0 | (c >> 18))); s.push_back((char)(0x80 | ((c >> 12) & 0x3F)));
+            s.push_back((char)(0x80 | ((c >> 6) & 0x3F))); s.push_back((char)(0x80 | (c & 0x3F)));
+        }
+    }
+    bool hex4(uint32_t& v) {
+        if (end - p < 4) { fail("truncated \\u escape"); return false; }
+        v = 0;
+        for (int k = 0; k < 4; ++k) {
+            char c = *p++;
+            v <<= 4;
+            if (c >= '0' && c <= '9') v |= (uint32_t)(c - '0');
+            else if (c >= 'a' && c <= 'f') v |= (uint32_t)(c - 'a' + 10);
+            else if (c >= 'A' && c <= 'F') v |= (uint32_t)(c - 'A' + 10);
+            else { fail("bad \\u escape"); return false; }
+        }
+        return true;
+    }
+    bool string(std::string& s) {
+        ++p;  // opening quote
+        while (true) {
+            if (p >= end) { fail("unterminated string"); return false; }
+            char c = *p++;
+            if (c == '"') return true;
+            if ((unsigned char)c < 0x20) { fail("control character in string"); return false; }
+            if (c != '\\') { s.push_back(c); continue; }
+            if (p >= end) { fail("unterminated escape"); return false; }
+            char e = *p++;
+            switch (e) {
+                case '"': s.push_back('"'); break;
+                case '\\': s.push_back('\\'); break;
+                case '/': s.push_back('/'); break;
+                case 'b': s.push_back('\b'); break;
+                case 'f': s.push_back('\f'); break;
+                case 'n': s.push_back('\n'); break;
+                case 'r': s.push_back('\r'); break;
+                case 't': s.push_back('\t'); break;
+                case 'u': {
+                    uint32_t cp;
+                    if (!hex4(cp)) return false;
+                    if (cp >= 0xD800 && cp < 0xDC00) {
+                        uint32_t lo;
+                        if (!(lit("\\u") && hex4(lo)) || lo < 0xDC00 || lo > 0xDFFF) {
+                            fail("bad surrogate pair");
+                            return false;
+                        }
+                        cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
+                    }
+                    put_utf8(s, cp);
+                    break;
+                }
+                default: fail("bad escape"); return false;
+            }
+        }
+    }
+    bool number(Value& v) {
+        const char* s = p;
+        bool real = false;
+        if (p < end && *p == '-') ++p;
+        if (p >= end || !(*p >= '0' && *p <= '9')) { fail("bad number"); return false; }
+        if (*p == '0') ++p;
+        else while (p < end && *p >= '0' && *p <= '9') ++p;
+        if (p < end && *p == '.') {
+            real = true; ++p;
+            if (p >= end || !(*p >= '0' && *p <= '9')) { fail("bad fraction"); return false; }
+            while (p < end && *p >= '0' && *p <= '9') ++p;
+        }
+        if (p < end && (*p == 'e' || *p == 'E')) {
+            real = true; ++p;
+            if (p < end && (*p == '+' || *p == '-')) ++p;
+            if (p >= end || !(*p >= '0' && *p <= '9')) { fail("bad exponent"); return false; }
+            while (p < end && *p >= '0' && *p <= '9') ++p;
+        }
+        std::string tok(s, p);
+        if (!real) {
+            errno = 0;
+            long long ll = std::strtoll(tok.c_str(), nullptr, 10);
+            if (errno == ERANGE) real = true;  // nlohmann falls back to a double too
+            else { v.kind = Kind::Int; v.i = ll; v.d = (double)ll; return true; }
+        }
+        v.kind = Kind::Real;
+        v.d = std::strtod(tok.c_str(), nullptr);  // correctly rounded in glibc
+        return true;
+    }
+    bool value(Value& v, int depth) {
+        if (depth > 256) { fail("nesting too deep"); return false; }
+        ws();
+        if (p >= end) { fail("unexpected end of input"); return false; }
+        char c = *p;
+        if (c == '{') {
+            ++p;
+            v.kind = Kind::Object;
+            ws();
+            if (p < end && *p == '}') { ++p; return true; }
+            while (true) {
+                ws();
+                if (p >= end || *p != '"') { fail("expected object key"); return false; }
+                std::string key;
+                if (!string(key)) return false;
+                ws();
+                if (p >= end || *p != ':') { fail("expected ':'"); return false; }
+                ++p;
+                Value child;
+                if (!value(child, depth + 1)) return false;
+                v.obj.emplace_back(std::move(key), std::move(child));
+                ws();
+                if (p < end && *p == ',') { ++p; continue; }
+                if (p < end && *p == '}') { ++p; return true; }
+                fail("expected ',' or '}'");
+                return false;
+            }
+        }
+        if (c == '[') {
+            ++p;
+            v.kind = Kind::Array;
+            ws();
+            if (p < end && *p == ']') { ++p; return true; }
+            while (true) {
+                Value child;
+                if (!value(child, depth + 1)) return false;
+                v.arr.push_back(std::move(child));
+                ws();
+                if (p < end && *p == ',') { ++p; continue; }
+                if (p < end && *p == ']') { ++p; return true; }
+                fail("expected ',' or ']'");
+                return false;
+            }
+        }
+        if (c == '"') { v.kind = Kind::String; return string(v.s); }
+        if (lit("true")) { v.kind = Kind::Bool; v.b = true; return true; }
+        if (lit("false")) { v.kind = Kind::Bool; v.b = false; return true; }
+        if (lit("null")) { v.kind = Kind::Null; return true; }
+        if (c == '-' || (c >= '0' && c <= '9')) return number(v);
+        fail("unexpected character");
+        return false;
+    }
+};
+
+}  // namespace
+
+bool parse(const char* text, size_t len, Value& out, std::string& err) {
+    Parser ps{text, text + len, text, {}};
+    out = Value{};
+    if (!ps.value(out, 0)) { err = ps.err; return false; }
+    ps.ws();
+    if (ps.p != ps.end) { ps.fail("trailing characters after document"); err = ps.err; return false; }
+    return true;
+}
+
+static std::runtime_error type_error(const char* what, const char* want) {
+    return std::runtime_error(std::string("scene key '") + what + "' must be " + want);
+}
+
+double as_double(const Value& v, const char* what) {
+    if (v.kind == Kind::Int) return (double)v.i;
+    if (v.kind == Kind::Real) return v.d;
+    if (v.kind == Kind::Bool) return v.b ? 1.0 : 0.0;
+    throw type_error(what, "a number");
+}
+
+float as_float(const Value& v, const char* what) {
+    if (v.kind == Kind::Int) return static_cast<float>(v.i);
+    if (v.kind == Kind::Real) return static_cast<float>(v.d);
+    if (v.kind == Kind::Bool) return v.b ? 1.0f : 0.0f;
+    throw type_error(what, "a number");
+}
+
+int as_int(const Value& v, const char* what) {
+    if (v.kind == Kind::Int) return static_cast<int>(v.i);
+    if (v.kind == Kind::Real) return static_cast<int>(v.d);
+    if (v.kind == Kind::Bool) return v.b ? 1 : 0;
+    throw type_error(what, "a number");
+}
+
+bool as_bool(const Value& v, const char* what) {
+    if (v.kind == Kind::Bool) return v.b;
+    if (v.kind == Kind::Int) return v.i != 0;
+    if (v.kind == Kind::Real) return v.d != 0.0;
+    throw type_error(what, "a boolean");
+}
+
+const std::string& as_string(const Value& v, const char* what) {
+    if (v.kind != Kind::String) throw type_error(what, "a string");
+    return v.s;
+}
+
+static void dump_string(std::string& o, const std::string& s) {
+    o.push_back('"');
+    for (unsigned char c : s) {
+        switch (c) {
+            case '"': o += "\\\""; break;
+            case '\\': o += "\\\\"; break;
+            case '\n': o += "\\n"; break;
+            case '\r': o += "\\r"; break;
+            case '\t': o += "\\t"; break;
+            case '\b': o += "\\b"; break;
+            case '\f': o += "\\f"; break;
+            default:
+                if (c < 0x20) { char b[8]; std::snprintf(b, sizeof b, "\\u%04x", c); o += b; }
+                else o.push_back((char)c);
+        }
+    }
+    o.push_back('"');
+}
+
+static void dump_rec(std::string& o, const Value& v, int indent, int level) {
+    auto nl = [&](int lv) {
+        o.push_back('\n');
+        o.append((size_t)(indent * lv), ' ');
+    };
+    switch (v.kind) {
+        case Kind::Null: o += "null"; break;
+        case Kind::Bool: o += v.b ? "true" : "false"; break;
+        case Kind::Int: o += std::to_string(v.i); break;
+        case Kind::Real: {
+            char b[40];
+            std::snprintf(b, sizeof b, "%.17g", v.d);
+            std::string t = b;
+            if (t.find_first_of(".eEn") == std::string::npos) t += ".0";
+            o += t;
+            break;
+        }
+        case Kind::String: dump_string(o, v.s); break;
+        case Kind::Array:
+            if (v.arr.empty()) { o += "[]"; break; }
+            o.push_back('[');
+            for (size_t k = 0; k < v.arr.size(); ++k) {
+                nl(level + 1);
+                dump_rec(o, v.arr[k], indent, level + 1);
+                if (k + 1 < v.arr.size()) o.push_back(',');
+            }
+            nl(level);
+            o.push_back(']');
+            break;
+        case Kind::Object:
+            if (v.obj.empty()) { o += "{}"; break; }
+            o.push_back('{');
+            for (size_t k = 0; k < v.obj.size(); ++k) {
+                nl(level + 1);
+                dump_string(o, v.obj[k].first);
+                o += ": ";
+                dump_rec(o, v.obj[k].second, indent, level + 1);
+                if (k + 1 < v.obj.size()) o.push_back(',');
+            }
+            nl(level);
+            o.push_back('}');
+            break;
+    }
+}
+
+std::string dump(const Value& v, int indent) {
+    std::string o;
+    dump_rec(o, v, indent, 0);
+    return o;
+}
+
+}  // namespace ptjson

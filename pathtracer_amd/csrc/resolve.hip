@@ -1,0 +1,113 @@
+// resolve.hip — display resolve of the XYZ accumulator and device-side
+// diagnostics.
+#include <vector>
+
+#include "tracer.hpp"
+
+namespace ptk {
+
+using namespace ptd;
+
+// sRGBCompanding (src/pathtracer.cpp:1061-1070, src/shader.frag:46-55).
+__device__ __forceinline__ float srgb_oetf(float x) {
+    x = ptm_clamp(x, 0.0f, 1.0f);
+    if (x <= 0.0031308f) return 12.92f * x;
+    return 1.055f * ptm_pow(x, 1.0f / 2.4f) - 0.055f;
+}
+__device__ __forceinline__ float tonemap1(float x, int tm) {
+    if (tm == 1) x = x / (1.0f + x);                                                     // Reinhard
+    if (tm == 2) x = x * (2.51f * x + 0.03f) / (x * (2.43f * x + 0.59f) + 0.14f);       // ACES (Narkowicz 2016)
+    if (tm == 3) x = ptm_exp(-0.25f / x);                                                // DEUCES
+    return x;
+}
+
+// SaveRender's per-pixel conversion (src/pathtracer.cpp:3501-3511): Bradford
+// E -> D65, XYZ -> linear sRGB (Lindbloom matrices), clamp at 0, tonemap,
+// OETF, then truncation to 8 bits.
+__global__ void k_resolve(const float4* texels, int n, int tm, uint8_t* rgb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 t = texels[i];
+    f3 c = mk3(t.x, t.y, t.z);
+    f3 e = mk3(0.9531874f * c.x + -0.0265906f * c.y + 0.0238731f * c.z,
+               -0.0382467f * c.x + 1.0288406f * c.y + 0.0094060f * c.z,
+               0.0026068f * c.x + -0.0030332f * c.y + 1.0892565f * c.z);
+    f3 r = mk3(3.2404542f * e.x + -1.5371385f * e.y + -0.4985314f * e.z,
+               -0.9692660f * e.x + 1.8760108f * e.y + 0.0415560f * e.z,
+               0.0556434f * e.x + -0.2040259f * e.y + 1.0572252f * e.z);
+    r = max3(r, mk3(0.0f, 0.0f, 0.0f));
+    float v[3] = {r.x, r.y, r.z};
+    for (int k = 0; k < 3; ++k) {
+        float x = srgb_oetf(tonemap1(v[k], tm));
+        rgb[3 * i + k] = (uint8_t)(int)((double)x * 255.0);
+    }
+}
+
+int resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb) {
+    int n = acc->w * acc->h;
+    DevAlloc buf;
+    int st = dev_alloc(ctx, buf, (size_t)n * 3);
+    if (st != PT_OK) return st;
+    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<const float4*>(acc->d_texels), n, tonemap, static_cast<uint8_t*>(buf.p));
+    hipError_t e = hipMemcpyAsync(host_rgb, buf.p, (size_t)n * 3, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, buf);
+    if (e != hipSuccess) return hip_fail(ctx, e, "pt_resolve_srgb8");
+    return PT_OK;
+}
+
+__global__ void k_math(int fn, const float* x, const float* y, float* out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a = x[i];
+    float r;
+    switch (fn) {
+        case 0: r = ptm_sin(a); break;
+        case 1: r = ptm_cos(a); break;
+        case 2: r = ptm_exp(a); break;
+        case 3: r = ptm_log(a); break;
+        case 4: r = ptm_pow(a, y[i]); break;
+        case 5: r = ptm_acos(a); break;
+        case 6: r = ptm_sqrt(a); break;
+        default: r = ptm_rsqrt(a); break;
+    }
+    out[i] = r;
+}
+
+int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n) {
+    if (n == 0) return PT_OK;
+    if (n > (size_t)1 << 26) {
+        ctx->err = "pt_math_eval_device: at most 2^26 values per call";
+        return PT_ERR_INVALID_ARG;
+    }
+    DevAlloc buf;
+    int st = dev_alloc(ctx, buf, n * 12);
+    if (st != PT_OK) return st;
+    float* dx = static_cast<float*>(buf.p);
+    float* dy = dx + n;
+    float* dout = dy + n;
+    hipError_t e = hipMemcpyAsync(dx, x, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && y) e = hipMemcpyAsync(dy, y, n * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, fn, dx, dy, dout,
+                           (int)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, buf);
+    if (e != hipSuccess) return hip_fail(ctx, e, "pt_math_eval_device");
+    return PT_OK;
+}
+
+__global__ void k_bump_dispatch(unsigned long long* stats) { stats[5] += 1ull; }
+
+int count_dispatch(pt_ctx* ctx) {
+    hipLaunchKernelGGL(k_bump_dispatch, dim3(1), dim3(1), 0, ctx->stream, ctx->d_stats);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "dispatch counter");
+    return PT_OK;
+}
+
+}  // namespace ptk

@@ -1,0 +1,596 @@
+// scene.cpp — see scene.hpp.
+#include "scene.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "json.hpp"
+
+namespace pt {
+
+static const float kCie[PT_CIE_SIZE] = {
+#include "cie1931_2deg.inc"
+};
+
+const float* cie_table() { return kCie; }
+void copy_cie_table(float* dst) { std::memcpy(dst, kCie, sizeof kCie); }
+
+using ptjson::Value;
+
+namespace {
+
+// nlohmann's operator[] on a missing key yields null, whose size() is 0; the
+// reference relies on that for absent object lists (e.g. scene8 has no "plane").
+const Value& member(const Value& v, const char* key) {
+    static const Value null_value;
+    const Value* m = v.get(key);
+    return m ? *m : null_value;
+}
+const Value& need(const Value& v, const char* key) {
+    const Value* m = v.get(key);
+    if (!m) throw std::runtime_error(std::string("missing scene key '") + key + "'");
+    return *m;
+}
+void read3(const Value& v, const char* what, float out[3]) {
+    if (v.kind != ptjson::Kind::Array || v.size() < 3)
+        throw std::runtime_error(std::string("scene key '") + what + "' must be an array of 3 numbers");
+    for (int k = 0; k < 3; ++k) out[k] = ptjson::as_float(v.at(k), what);
+}
+
+void load_material(const Value& m, Material& mat) {
+    const Value* refl = m.get("reflection");
+    if (refl) {
+        mat.reflection[0] = ptjson::as_float(need(*refl, "peakWavelength"), "peakWavelength");
+        mat.reflection[1] = ptjson::as_float(need(*refl, "sigma"), "sigma");
+        mat.reflection[2] = ptjson::as_float(need(*refl, "isInvert"), "isInvert");
+    }
+    const Value* type = m.get("type");
+    std::string t = type ? ptjson::as_string(*type, "type") : std::string("lambert");
+    if (t == "lambert") {
+        const Value* g = m.get("gaussians");
+        if (g) {
+            if (g->kind != ptjson::Kind::Array || g->size() < 1 || g->size() > 3)
+                throw std::runtime_error("material 'gaussians' must hold 1 to 3 [weight, mean, sigma] entries");
+            mat.type = MAT_LAMBERT_GAUSS;
+            mat.n_gauss = (int)g->size();
+            for (int i = 0; i < mat.n_gauss; ++i) read3(g->at((size_t)i), "gaussians", &mat.gauss[3 * i]);
+        } else {
+            if (!refl) throw std::runtime_error("lambert material needs 'reflection' or 'gaussians'");
+            mat.type = MAT_LAMBERT_LOBE;
+        }
+    } else if (t == "dielectric") {
+        mat.type = MAT_DIELECTRIC;
+        const Value* c = m.get("cauchy");
+        const Value* s = m.get("sellmeier");
+        if (c) {
+            if (c->kind != ptjson::Kind::Array || c->size() != 2)
+                throw std::runtime_error("dielectric 'cauchy' must be [A, B]");
+            mat.dispersion = DISP_CAUCHY;
+            mat.disp[0] = ptjson::as_float(c->at(0), "cauchy");
+            mat.disp[1] = ptjson::as_float(c->at(1), "cauchy");
+        } else if (s) {
+            if (s->kind != ptjson::Kind::Array || s->size() != 6)
+                throw std::runtime_error("dielectric 'sellmeier' must be [B1, B2, B3, C1, C2, C3]");
+            mat.dispersion = DISP_SELLMEIER;
+            for (int k = 0; k < 6; ++k) mat.disp[k] = ptjson::as_float(s->at((size_t)k), "sellmeier");
+        } else {
+            throw std::runtime_error("dielectric material needs 'cauchy' or 'sellmeier'");
+        }
+    } else if (t == "conductor") {
+        mat.type = MAT_CONDUCTOR;
+        mat.cond_n = ptjson::as_float(need(m, "n"), "n");
+        mat.cond_k = ptjson::as_float(need(m, "k"), "k");
+    } else {
+        throw std::runtime_error("unknown material type '" + t + "'");
+    }
+}
+
+int load_sdf(const Value& o, RSdf& sd, std::string& err) {
+    read3(need(o, "position"), "position", sd.pos);
+    read3(need(o, "boundingSize"), "boundingSize", sd.size);
+    std::memset(&sd.desc, 0, sizeof sd.desc);
+    const Value* g = o.get("glsl");
+    const Value* kind = o.get("kind");
+    if (g) {
+        sd.glsl = ptjson::as_string(*g, "glsl");
+        int st = recognize_sdf(sd.glsl, sd.desc, err);
+        if (st != PT_OK) return st;
+    } else if (kind) {
+        // Extended form: {"kind": "mandelbulb", "iterations": 8, "bailout": 16,
+        //                 "material": 0.0 | [a, b, d] | [m0, m1, m2, m3]}
+        std::string k = ptjson::as_string(*kind, "kind");
+        pt_sdf_desc& d = sd.desc;
+        if (k == "mandelbulb") { d.kind = PT_SDF_MANDELBULB; d.iterations = 4; d.bailout = 16.0f; }
+        else if (k == "menger") { d.kind = PT_SDF_MENGER; d.iterations = 5; }
+        else if (k == "blob") { d.kind = PT_SDF_BLOB; d.material_kind = PT_SDFMAT_BLOB; d.material[0] = 3; d.material[1] = 4; d.material[2] = 5; d.material[3] = 5; }
+        else if (k == "terrain") { d.kind = PT_SDF_TERRAIN; }
+        else { err = "unknown sdf kind '" + k + "'"; return PT_ERR_UNSUPPORTED_SDF; }
+        if (const Value* it = o.get("iterations")) d.iterations = ptjson::as_int(*it, "iterations");
+        if (const Value* b = o.get("bailout")) d.bailout = ptjson::as_float(*b, "bailout");
+        if (const Value* m = o.get("material")) {
+            if (m->is_number()) { d.material_kind = PT_SDFMAT_CONSTANT; d.material[0] = ptjson::as_float(*m, "material"); }
+            else if (m->kind == ptjson::Kind::Array && m->size() == 3 && d.kind != PT_SDF_BLOB) {
+                d.material_kind = PT_SDFMAT_RADIAL;
+                for (int q = 0; q < 3; ++q) d.material[q] = ptjson::as_float(m->at((size_t)q), "material");
+            } else if (m->kind == ptjson::Kind::Array && m->size() == 4 && d.kind == PT_SDF_BLOB) {
+                for (int q = 0; q < 4; ++q) d.material[q] = ptjson::as_float(m->at((size_t)q), "material");
+            } else { err = "bad sdf 'material'"; return PT_ERR_JSON; }
+        }
+        if (d.iterations < 0 || d.iterations > 64) { err = "sdf iterations out of range [0, 64]"; return PT_ERR_JSON; }
+    } else {
+        err = "sdf entry needs 'glsl' or 'kind'";
+        return PT_ERR_JSON;
+    }
+    for (int k = 0; k < 3; ++k) { sd.desc.pos[k] = sd.pos[k]; sd.desc.size[k] = sd.size[k]; }
+    return PT_OK;
+}
+
+}  // namespace
+
+int load_scene_json(const char* text, size_t len, SceneModel& s, std::string& err) {
+    Value root;
+    if (!ptjson::parse(text, len, root, err)) return PT_ERR_JSON;
+    if (root.kind != ptjson::Kind::Object) { err = "scene JSON must be an object"; return PT_ERR_JSON; }
+    s = SceneModel{};
+    try {
+        // Camera (UpdateFromJSON, src/pathtracer.cpp:2577-2606).
+        const Value& cam = need(root, "camera");
+        int nshots = ptjson::as_int(need(cam, "numShots"), "numShots");
+        if (nshots < 1) throw std::runtime_error("camera numShots must be >= 1");
+        const Value& pos = need(cam, "position");
+        const Value& ang = need(cam, "angle");
+        s.shots.resize((size_t)nshots);
+        for (int i = 0; i < nshots; ++i) {
+            read3(pos.at((size_t)i), "position", s.shots[(size_t)i].pos);
+            const Value& a = ang.at((size_t)i);
+            s.shots[(size_t)i].angle[0] = ptjson::as_float(a.at(0), "angle");
+            s.shots[(size_t)i].angle[1] = ptjson::as_float(a.at(1), "angle");
+        }
+        s.cam.ISO = ptjson::as_int(need(cam, "ISO"), "ISO");
+        s.cam.size = ptjson::as_float(need(cam, "size"), "size");
+        s.cam.apertureSize = ptjson::as_float(need(cam, "apertureSize"), "apertureSize");
+        s.cam.apertureDist = ptjson::as_float(need(cam, "apertureDistance"), "apertureDistance");
+        s.cam.lensRadius = ptjson::as_float(need(cam, "lensRadius"), "lensRadius");
+        s.cam.lensFocalLength = ptjson::as_float(need(cam, "lensFocalLength"), "lensFocalLength");
+        s.cam.lensThickness = ptjson::as_float(need(cam, "lensThickness"), "lensThickness");
+        s.cam.lensDistance = ptjson::as_float(need(cam, "lensDistance"), "lensDistance");
+        if (const Value* model = cam.get("model")) {
+            std::string m = ptjson::as_string(*model, "model");
+            if (m == "lens") s.render.camera = 0;
+            else if (m == "pinhole") s.render.camera = 1;
+            else throw std::runtime_error("camera model must be 'lens' or 'pinhole'");
+        }
+
+        const Value& sph = member(root, "sphere");
+        for (size_t i = 0; i < sph.size(); ++i) {
+            const Value& o = sph.at(i);
+            RSphere r{};
+            read3(need(o, "position"), "position", r.pos);
+            r.radius = ptjson::as_float(need(o, "radius"), "radius");
+            r.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            r.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.spheres.push_back(r);
+        }
+        const Value& pln = member(root, "plane");
+        for (size_t i = 0; i < pln.size(); ++i) {
+            const Value& o = pln.at(i);
+            RPlane r{};
+            read3(need(o, "position"), "position", r.pos);
+            r.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            r.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.planes.push_back(r);
+        }
+        const Value& bx = member(root, "box");
+        for (size_t i = 0; i < bx.size(); ++i) {
+            const Value& o = bx.at(i);
+            RBox r{};
+            read3(need(o, "position"), "position", r.pos);
+            read3(need(o, "rotation"), "rotation", r.rotation);
+            read3(need(o, "size"), "size", r.size);
+            r.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            r.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.boxes.push_back(r);
+        }
+        const Value& ln = member(root, "lens");
+        for (size_t i = 0; i < ln.size(); ++i) {
+            const Value& o = ln.at(i);
+            RLens r{};
+            read3(need(o, "position"), "position", r.pos);
+            read3(need(o, "rotation"), "rotation", r.rotation);
+            r.radius = ptjson::as_float(need(o, "radius"), "radius");
+            r.focalLength = ptjson::as_float(need(o, "focalLength"), "focalLength");
+            r.thickness = ptjson::as_float(need(o, "thickness"), "thickness");
+            r.isConverging = ptjson::as_bool(need(o, "isConverging"), "isConverging");
+            r.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            r.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.lenses.push_back(r);
+        }
+        const Value& cy = member(root, "cyclide");
+        for (size_t i = 0; i < cy.size(); ++i) {
+            const Value& o = cy.at(i);
+            RCyclide r{};
+            read3(need(o, "position"), "position", r.pos);
+            read3(need(o, "rotation"), "rotation", r.rotation);
+            read3(need(o, "scale"), "scale", r.scale);
+            r.a = ptjson::as_float(need(o, "a"), "a");
+            r.b = ptjson::as_float(need(o, "b"), "b");
+            r.c = ptjson::as_float(need(o, "c"), "c");
+            r.d = ptjson::as_float(need(o, "d"), "d");
+            r.brad = ptjson::as_float(need(o, "boundingRadius"), "boundingRadius");
+            r.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            r.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.cyclides.push_back(r);
+        }
+        const Value& sd = member(root, "sdf");
+        for (size_t i = 0; i < sd.size(); ++i) {
+            RSdf r{};
+            int st = load_sdf(sd.at(i), r, err);
+            if (st != PT_OK) {
+                err = "sdf " + std::to_string(i) + ": " + err;
+                return st;
+            }
+            s.sdfs.push_back(r);
+        }
+        const Value& qd = member(root, "quad");
+        for (size_t i = 0; i < qd.size(); ++i) {
+            const Value& o = qd.at(i);
+            Quad q{};
+            read3(need(o, "position"), "position", q.corner);
+            read3(need(o, "edge1"), "edge1", q.e1);
+            read3(need(o, "edge2"), "edge2", q.e2);
+            q.materialID = ptjson::as_int(need(o, "materialID"), "materialID");
+            q.lightID = ptjson::as_int(need(o, "lightID"), "lightID");
+            s.quads.push_back(q);
+        }
+        const Value& mt = member(root, "material");
+        for (size_t i = 0; i < mt.size(); ++i) {
+            Material m;
+            load_material(mt.at(i), m);
+            s.materials.push_back(m);
+        }
+        const Value& lt = member(root, "light");
+        for (size_t i = 0; i < lt.size(); ++i) {
+            const Value& em = need(lt.at(i), "emission");
+            Light l{};
+            l.emission[0] = ptjson::as_float(need(em, "temperature"), "temperature");
+            l.emission[1] = ptjson::as_float(need(em, "luminosity"), "luminosity");
+            s.lights.push_back(l);
+        }
+        if (const Value* env = root.get("environment")) {
+            s.env_enabled = true;
+            s.env_radiance = ptjson::as_float(need(*env, "radiance"), "radiance");
+        }
+        if (const Value* r = root.get("render")) {
+            if (const Value* v = r->get("wavelengths")) s.render.n_lambda = ptjson::as_int(*v, "wavelengths");
+            if (const Value* v = r->get("lambdaMin")) s.render.lambda_min = ptjson::as_float(*v, "lambdaMin");
+            if (const Value* v = r->get("lambdaMax")) s.render.lambda_max = ptjson::as_float(*v, "lambdaMax");
+            if (const Value* v = r->get("russianRouletteDepth")) s.render.rr_depth = ptjson::as_int(*v, "russianRouletteDepth");
+            if (const Value* v = r->get("seed")) s.render.seed = (uint32_t)ptjson::as_double(*v, "seed");
+            if (s.render.n_lambda != 1 && s.render.n_lambda != 4)
+                throw std::runtime_error("render wavelengths must be 1 or 4");
+            if (!(s.render.lambda_min >= 360.0f && s.render.lambda_max <= 800.0f &&
+                  s.render.lambda_min < s.render.lambda_max))
+                throw std::runtime_error("render wavelength range must lie in [360, 800] nm");
+        }
+    } catch (const std::exception& e) {
+        err = e.what();
+        return PT_ERR_JSON;
+    }
+    return PT_OK;
+}
+
+int pack_reference_ubo(const SceneModel& s, pt_packed_scene& ubo, std::string& err) {
+    std::memset(&ubo, 0, sizeof ubo);
+    std::vector<float> objects, lightIDs;
+    for (size_t i = 0; i < s.spheres.size(); ++i) {
+        const RSphere& o = s.spheres[i];
+        objects.insert(objects.end(), {o.pos[0], o.pos[1], o.pos[2], o.radius, (float)o.materialID, (float)o.lightID});
+        if (o.lightID > 0) lightIDs.push_back((float)i);
+    }
+    for (size_t i = 0; i < s.planes.size(); ++i) {
+        const RPlane& o = s.planes[i];
+        objects.insert(objects.end(), {o.pos[0], o.pos[1], o.pos[2], (float)o.materialID, (float)o.lightID});
+        if (o.lightID > 0) lightIDs.push_back((float)(s.spheres.size() + i));
+    }
+    for (size_t i = 0; i < s.boxes.size(); ++i) {
+        const RBox& o = s.boxes[i];
+        objects.insert(objects.end(), {o.pos[0], o.pos[1], o.pos[2], o.rotation[0], o.rotation[1], o.rotation[2],
+                                       o.size[0], o.size[1], o.size[2], (float)o.materialID, (float)o.lightID});
+        if (o.lightID > 0) lightIDs.push_back((float)(s.spheres.size() + s.planes.size() + i));
+    }
+    // The reference tests planes[i].lightID for lenses and cyclides
+    // (src/pathtracer.cpp:3704, 3728); an index past the plane list is read
+    // here as "not a light".
+    auto plane_light = [&](size_t i) { return i < s.planes.size() ? s.planes[i].lightID : 0; };
+    for (size_t i = 0; i < s.lenses.size(); ++i) {
+        const RLens& o = s.lenses[i];
+        objects.insert(objects.end(), {o.pos[0], o.pos[1], o.pos[2], o.rotation[0], o.rotation[1], o.rotation[2],
+                                       o.radius, o.focalLength, o.thickness, (float)o.isConverging,
+                                       (float)o.materialID, (float)o.lightID});
+        if (plane_light(i) > 0) lightIDs.push_back((float)(s.spheres.size() + s.planes.size() + s.boxes.size() + i));
+    }
+    for (size_t i = 0; i < s.cyclides.size(); ++i) {
+        const RCyclide& o = s.cyclides[i];
+        float m = std::max(std::max(o.scale[0], o.scale[1]), o.scale[2]);
+        objects.insert(objects.end(), {o.pos[0], o.pos[1], o.pos[2], o.rotation[0], o.rotation[1], o.rotation[2],
+                                       o.scale[0], o.scale[1], o.scale[2], o.a, o.b, o.c, o.d,
+                                       o.brad * o.brad * (m * m), (float)o.materialID, (float)o.lightID});
+        if (plane_light(i) > 0)
+            lightIDs.push_back((float)(s.spheres.size() + s.planes.size() + s.boxes.size() + s.lenses.size() + i));
+    }
+    if (s.quads.size() > 0) {
+        err = "quads have no reference packing";
+        return PT_ERR_CAPACITY;
+    }
+    if (objects.size() > PT_MAX_OBJECTS_SIZE) {
+        err = "objects need " + std::to_string(objects.size()) + " floats, the reference block holds " +
+              std::to_string(PT_MAX_OBJECTS_SIZE);
+        return PT_ERR_CAPACITY;
+    }
+    if (s.sdfs.size() > PT_MAX_SDF_OBJECTS) {
+        err = "more than 32 SDFs (the reference's set mask holds 32)";
+        return PT_ERR_CAPACITY;
+    }
+    if (s.materials.size() * 3 > PT_MAX_MATERIALS_SIZE) { err = "too many materials"; return PT_ERR_CAPACITY; }
+    if (s.lights.size() * 2 > PT_MAX_LIGHTS_SIZE) { err = "too many lights"; return PT_ERR_CAPACITY; }
+    if (lightIDs.size() > PT_MAX_LIGHTIDS_SIZE) { err = "too many emissive objects"; return PT_ERR_CAPACITY; }
+
+    ubo.numObjects[0] = (float)s.spheres.size();
+    ubo.numObjects[1] = (float)s.planes.size();
+    ubo.numObjects[2] = (float)s.boxes.size();
+    ubo.numObjects[3] = (float)s.lenses.size();
+    ubo.numObjects[4] = (float)s.cyclides.size();
+    ubo.numObjects[5] = (float)s.sdfs.size();
+    ubo.numObjects[6] = (float)lightIDs.size();
+    std::copy(objects.begin(), objects.end(), ubo.objects);
+    for (size_t i = 0; i < s.sdfs.size(); ++i)
+        for (int k = 0; k < 3; ++k) {
+            ubo.sdfs[6 * i + k] = s.sdfs[i].pos[k];
+            ubo.sdfs[6 * i + 3 + k] = s.sdfs[i].size[k];
+        }
+    for (size_t i = 0; i < s.materials.size(); ++i)
+        for (int k = 0; k < 3; ++k) ubo.materials[3 * i + k] = s.materials[i].reflection[k];
+    for (size_t i = 0; i < s.lights.size(); ++i) {
+        ubo.lights[2 * i] = s.lights[i].emission[0];
+        ubo.lights[2 * i + 1] = s.lights[i].emission[1];
+    }
+    std::copy(lightIDs.begin(), lightIDs.end(), ubo.lightIDs);
+    copy_cie_table(ubo.cie);
+    return PT_OK;
+}
+
+int camera_to_push(const SceneModel& s, int shot, pt_push_constants& pc, std::string& err) {
+    if (shot < 1 || (size_t)shot > s.shots.size()) {
+        err = "camera shot " + std::to_string(shot) + " out of range [1, " + std::to_string(s.shots.size()) + "]";
+        return PT_ERR_INVALID_ARG;
+    }
+    const CameraShot& c = s.shots[(size_t)shot - 1];
+    pc.cameraAngle[0] = -c.angle[1];
+    pc.cameraAngle[1] = c.angle[0];
+    pc.cameraPosX = c.pos[0];
+    pc.cameraPosY = c.pos[1];
+    pc.cameraPosZ = c.pos[2];
+    pc.ISO = s.cam.ISO;
+    pc.cameraSize = s.cam.size;
+    pc.apertureSize = s.cam.apertureSize;
+    pc.apertureDist = s.cam.apertureDist;
+    pc.lensRadius = s.cam.lensRadius;
+    pc.lensFocalLength = s.cam.lensFocalLength;
+    pc.lensThickness = s.cam.lensThickness;
+    pc.lensDistance = s.cam.lensDistance;
+    return PT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SDF recogniser.
+//
+// The GLSL text is tokenised (comments dropped) and split into functions. The
+// `sdf` body, its helpers and the `sdfmaterial` body are classified by the
+// built-in functions they call and verified against the exact sequence of
+// numeric literals each built-in kernel implements; the free parameters
+// (iterations, escape radius, material values) are read from that sequence.
+// Anything else is rejected.
+namespace {
+
+struct Tok { std::string t; bool num; };
+
+std::vector<Tok> tokenize(const std::string& src) {
+    std::vector<Tok> out;
+    size_t i = 0, n = src.size();
+    while (i < n) {
+        char c = src[i];
+        if (std::isspace((unsigned char)c)) { ++i; continue; }
+        if (c == '/' && i + 1 < n && src[i + 1] == '/') {
+            while (i < n && src[i] != '\n') ++i;
+            continue;
+        }
+        if (c == '/' && i + 1 < n && src[i + 1] == '*') {
+            size_t e = src.find("*/", i + 2);
+            i = (e == std::string::npos) ? n : e + 2;
+            continue;
+        }
+        if (std::isalpha((unsigned char)c) || c == '_') {
+            size_t j = i;
+            while (j < n && (std::isalnum((unsigned char)src[j]) || src[j] == '_')) ++j;
+            out.push_back({src.substr(i, j - i), false});
+            i = j;
+            continue;
+        }
+        if (std::isdigit((unsigned char)c) || (c == '.' && i + 1 < n && std::isdigit((unsigned char)src[i + 1]))) {
+            size_t j = i;
+            while (j < n && (std::isdigit((unsigned char)src[j]) || src[j] == '.')) ++j;
+            if (j < n && (src[j] == 'e' || src[j] == 'E')) {
+                ++j;
+                if (j < n && (src[j] == '+' || src[j] == '-')) ++j;
+                while (j < n && std::isdigit((unsigned char)src[j])) ++j;
+            }
+            if (j < n && (src[j] == 'f' || src[j] == 'F')) ++j;
+            out.push_back({src.substr(i, j - i), true});
+            i = j;
+            continue;
+        }
+        static const char* two[] = {"++", "--", "<=", ">=", "==", "!=", "&&", "||", "+=", "-=", "*=", "/="};
+        bool matched = false;
+        for (const char* t : two)
+            if (i + 1 < n && src[i] == t[0] && src[i + 1] == t[1]) {
+                out.push_back({std::string(t), false});
+                i += 2;
+                matched = true;
+                break;
+            }
+        if (matched) continue;
+        out.push_back({std::string(1, c), false});
+        ++i;
+    }
+    return out;
+}
+
+struct Func { std::string name; std::vector<Tok> body; };
+
+// Splits `float name(...) { ... }` definitions.
+bool split_functions(const std::vector<Tok>& t, std::vector<Func>& out) {
+    size_t i = 0;
+    while (i < t.size()) {
+        if (i + 2 < t.size() && !t[i].num && !t[i + 1].num && t[i + 2].t == "(") {
+            Func f;
+            f.name = t[i + 1].t;
+            size_t j = i + 3;
+            while (j < t.size() && t[j].t != ")") ++j;
+            ++j;
+            if (j >= t.size() || t[j].t != "{") return false;
+            int depth = 0;
+            size_t k = j;
+            for (; k < t.size(); ++k) {
+                if (t[k].t == "{") ++depth;
+                else if (t[k].t == "}") { if (--depth == 0) break; }
+            }
+            if (k >= t.size()) return false;
+            f.body.assign(t.begin() + (long)j + 1, t.begin() + (long)k);
+            out.push_back(std::move(f));
+            i = k + 1;
+        } else {
+            return false;
+        }
+    }
+    return true;
+}
+
+std::vector<double> literals(const std::vector<Tok>& body) {
+    std::vector<double> v;
+    for (const Tok& k : body)
+        if (k.num) v.push_back(std::strtod(k.t.c_str(), nullptr));
+    return v;
+}
+bool calls(const std::vector<Tok>& body, const char* name) {
+    for (size_t i = 0; i + 1 < body.size(); ++i)
+        if (body[i].t == name && body[i + 1].t == "(") return true;
+    return false;
+}
+int count_calls(const std::vector<Tok>& body, const char* name) {
+    int n = 0;
+    for (size_t i = 0; i + 1 < body.size(); ++i)
+        if (body[i].t == name && body[i + 1].t == "(") ++n;
+    return n;
+}
+// NaN in `want` marks a free parameter, written to `got` in order.
+bool match_literals(const std::vector<double>& have, const std::vector<double>& want, std::vector<double>& got) {
+    if (have.size() != want.size()) return false;
+    got.clear();
+    for (size_t i = 0; i < want.size(); ++i) {
+        if (want[i] != want[i]) got.push_back(have[i]);
+        else if (have[i] != want[i]) return false;
+    }
+    return true;
+}
+
+const double P = std::nan("");  // placeholder
+
+}  // namespace
+
+int recognize_sdf(const std::string& glsl, pt_sdf_desc& d, std::string& err) {
+    std::memset(&d, 0, sizeof d);
+    std::vector<Func> fns;
+    if (!split_functions(tokenize(glsl), fns)) {
+        err = "SDF text is not a list of function definitions";
+        return PT_ERR_UNSUPPORTED_SDF;
+    }
+    const Func* sdf = nullptr;
+    const Func* mat = nullptr;
+    std::vector<const Func*> helpers;
+    for (const Func& f : fns) {
+        if (f.name == "sdf") sdf = &f;
+        else if (f.name == "sdfmaterial") mat = &f;
+        else helpers.push_back(&f);
+    }
+    if (!sdf || !mat) {
+        err = "SDF text must define sdf() and sdfmaterial()";
+        return PT_ERR_UNSUPPORTED_SDF;
+    }
+    std::vector<double> lit = literals(sdf->body), got;
+
+    // Mandelbulb, power 8 via Chebyshev polynomials (sdfs/mandelbulb.glsl).
+    static const std::vector<double> kMandel = {1.0, 0, P, 8.0, 7.0, 1.0, 1.0, 128.0, 256.0, 160.0, 32.0, 1.0,
+                                                128.0, 256.0, 160.0, 32.0, 1.0, 128.0, 192.0, 80.0, 8.0,
+                                                128.0, 192.0, 80.0, 8.0, P, 0.5};
+    // Menger sponge (sdfs/menger.glsl).
+    static const std::vector<double> kMenger = {1.0, 0.0, 0.0, 1.0, 0, P, 2.0, 1.0, 3.0, 1.0, 3.0, 1.0};
+    // Blob of four smooth-min spheres (sdfs/blob.glsl).
+    static const std::vector<double> kBlob = {1.0, 1.0, 0.7, 0.25, 1.25, 0.5, 0.25, 0.25, 1.25, 0.5, 0.25};
+    static const std::vector<double> kBlobMat = {1.0, P, 1.0, 0.7, P, 0.25, 1.25, 0.5, 0.25, P, 0.25, 1.25, 0.5, 0.25, P};
+    // Terrain height h(x) (sdfs/terrain.glsl).
+    static const std::vector<double> kTerrain = {0.0625, 0.125, 10.0, 0.25, 0.25, 0.125, 0.5, 0.25, 0.25, 2.0, 1.0,
+                                                 0.125, 4.0, 2.0, 0.0625, 8.0, 1.0, 0.03125, 16.0, 5.0, 0.015625,
+                                                 32.0, 0.0078125, 64.0};
+
+    if (helpers.empty() && calls(sdf->body, "pow") && calls(sdf->body, "inversesqrt") && calls(sdf->body, "log") &&
+        match_literals(lit, kMandel, got)) {
+        d.kind = PT_SDF_MANDELBULB;
+        d.iterations = (int)got[0];
+        d.bailout = (float)got[1];
+        if (got[0] != (double)d.iterations || d.iterations < 0 || d.iterations > 64) {
+            err = "mandelbulb iteration count must be an integer in [0, 64]";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+    } else if (helpers.empty() && calls(sdf->body, "mod") && match_literals(lit, kMenger, got)) {
+        d.kind = PT_SDF_MENGER;
+        d.iterations = (int)got[0];
+        if (got[0] != (double)d.iterations || d.iterations < 0 || d.iterations > 16) {
+            err = "menger level count must be an integer in [0, 16]";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+    } else if (helpers.empty() && count_calls(sdf->body, "smin") == 3 && match_literals(lit, kBlob, got)) {
+        d.kind = PT_SDF_BLOB;
+    } else if (helpers.size() == 1 && count_calls(helpers[0]->body, "sin") == 11 &&
+               match_literals(literals(helpers[0]->body), kTerrain, got) && lit.empty() &&
+               count_calls(sdf->body, helpers[0]->name.c_str()) == 2) {
+        d.kind = PT_SDF_TERRAIN;
+    } else {
+        err = "unrecognised SDF program (supported: mandelbulb, menger, blob, terrain)";
+        return PT_ERR_UNSUPPORTED_SDF;
+    }
+
+    std::vector<double> mlit = literals(mat->body);
+    if (d.kind == PT_SDF_BLOB) {
+        if (count_calls(mat->body, "smin") != 3 || !match_literals(mlit, kBlobMat, got)) {
+            err = "unrecognised blob sdfmaterial()";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+        d.material_kind = PT_SDFMAT_BLOB;
+        for (int k = 0; k < 4; ++k) d.material[k] = (float)got[(size_t)k];
+    } else if (mlit.size() == 1 && mat->body.size() == 3 && mat->body[0].t == "return") {
+        d.material_kind = PT_SDFMAT_CONSTANT;
+        d.material[0] = (float)mlit[0];
+    } else if (mlit.size() == 3 && calls(mat->body, "mix") && calls(mat->body, "dot")) {
+        d.material_kind = PT_SDFMAT_RADIAL;
+        for (int k = 0; k < 3; ++k) d.material[k] = (float)mlit[(size_t)k];
+    } else {
+        err = "unrecognised sdfmaterial() (supported: constant, radial mix, blob)";
+        return PT_ERR_UNSUPPORTED_SDF;
+    }
+    return PT_OK;
+}
+
+}  // namespace pt

@@ -1,0 +1,164 @@
+// tracer.hpp — internal definitions shared by the C ABI (capi.cpp) and the
+// kernel translation units: device-side scene records, wave buffers, and the
+// opaque pt_ctx / pt_scene / pt_accum objects.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pathtracer.h"
+#include "ptdev.h"
+#include "scene.hpp"
+
+namespace ptk {
+
+using ptd::f3;
+using ptd::m3;
+
+constexpr float kMaxDist = 1e5f;  // MAXDIST, src/shader.comp:8
+constexpr int kUboFloats = 4097;
+constexpr int kUboObj = 7, kUboSdf = 7 + 1024, kUboMat = kUboSdf + 768, kUboLight = kUboMat + 783,
+              kUboLightId = kUboLight + 128, kUboCie = kUboLightId + 64;
+
+// ---- reference mode: objects decoded once from the uniform block -----------------
+// Values that src/shader.comp recomputes per call from the packed floats
+// (rotation matrices, integer IDs, bounding radii, lens slice offsets) are
+// computed once here with the same operations, so they carry the same bits.
+struct RefSphere { float px, py, pz, r, mat, light; };
+struct RefPlane { float px, py, pz, mat, light; };
+struct RefBox { f3 pos, size; m3 M; float bsr2, mat, light; };
+struct RefLens {
+    f3 pos; m3 M;
+    float radius, half, slicePos, br2, twoF, mat, light;
+    int conv;
+};
+struct RefCyclide { f3 pos, scale; m3 M; float a, b, c, d, brad2, mat, light; };
+struct RefSdf { f3 pos, size; pt_sdf_desc d; };
+
+struct RefSceneDev {
+    const float* ubo;  // the 4097-float uniform block
+    const RefSphere* sph;
+    const RefPlane* pln;
+    const RefBox* box;
+    const RefLens* lens;
+    const RefCyclide* cyc;
+    const RefSdf* sdf;
+    int ns, np, nb, nl, nc, nsdf;
+    float numLights;   // numObjects[6]
+    int sdf_kind;      // PT_SDF_* when all SDFs share one kind, 0 when mixed or none
+};
+
+// Per-dispatch constants (push constants with the camera precomputed).
+struct RefFrame {
+    int W, H, frame, spf, pathLength;
+    float camSize, apSize, apDist;
+    f3 camPos, fwd;
+    m3 camM;
+    // camera lens of TracePathLens (src/shader.comp:1409-1444)
+    f3 lensPos;
+    m3 lensM;
+    float lensRadius, lensHalf, lensSlicePos, lensTwoF;
+    // the wave: pixels [p0, p0 + pc) x samples [k0, k0 + m)
+    int p0, pc, k0, m;
+};
+
+// SoA path state of one wave (capacity N paths). Records are written once and
+// read once per stage.
+struct RefWave {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    uint32_t* seed;
+    float* lh;
+    float4* T;  // throughput per wavelength
+    float4* L;  // radiance per wavelength
+    float* w;   // MIS weight of the next emitter hit
+    float *ht, *hnx, *hny, *hnz, *hmat, *hlight;
+    int* q[2];  // ping-pong path queues
+    // shadow (NEE) records
+    int* s_path;
+    float *sox, *soy, *soz, *sdx, *sdy, *sdz;
+    int* s_lobj;
+    float4* s_c;
+    float* s_t;
+    int* s_id;
+    float* sum;      // per-pixel running sums, 3 x pc
+    int* counts;     // [0..1] queue lengths, [2] shadow length
+    unsigned long long* stats;  // pt_counters fields
+    int cap;
+};
+
+// ---- extended mode ----------------------------------------------------------------
+struct ExtScene;   // defined in ext_tracer.hip
+struct ExtWave;
+
+}  // namespace ptk
+
+// ---- opaque C-ABI objects ------------------------------------------------------------
+struct DevAlloc {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    pt_config cfg{};
+    std::string err;
+    int64_t allocated = 0;
+    unsigned long long* d_stats = nullptr;  // 6 counters
+    // reference-mode wave
+    DevAlloc ref_mem;
+    ptk::RefWave ref_wave{};
+    // extended-mode wave (owned by ext_tracer.hip)
+    void* ext_state = nullptr;
+};
+
+struct pt_scene {
+    pt_ctx* ctx = nullptr;
+    pt::SceneModel model;
+    bool has_model = false;
+    bool ref_ok = false;
+    std::string ref_err;
+    pt_packed_scene ubo{};
+    std::vector<pt_sdf_desc> sdfs;
+    // reference-mode device data (one allocation)
+    DevAlloc ref_mem;
+    ptk::RefSceneDev ref_dev{};
+    // extended-mode device data (built on first use)
+    void* ext = nullptr;
+    int64_t device_bytes = 0;
+};
+
+struct pt_accum {
+    pt_ctx* ctx = nullptr;
+    int w = 0, h = 0;
+    float* d_texels = nullptr;
+};
+
+namespace ptk {
+
+// Device memory with the context's budget applied. Returns PT_OK,
+// PT_ERR_BUDGET or PT_ERR_HIP (message in ctx->err).
+int dev_alloc(pt_ctx* ctx, DevAlloc& a, size_t bytes);
+void dev_free(pt_ctx* ctx, DevAlloc& a);
+int hip_fail(pt_ctx* ctx, hipError_t e, const char* what);
+
+// Reference mode (ref_tracer.hip).
+int ref_scene_upload(pt_ctx* ctx, pt_scene* s);
+int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc);
+void ref_wave_release(pt_ctx* ctx);
+
+// Extended mode (ext_tracer.hip).
+int ext_scene_build(pt_ctx* ctx, pt_scene* s);
+void ext_scene_release(pt_ctx* ctx, pt_scene* s);
+int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc);
+void ext_wave_release(pt_ctx* ctx);
+
+// Display resolve and diagnostics (resolve.hip).
+int resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb);
+int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n);
+int count_dispatch(pt_ctx* ctx);
+
+}  // namespace ptk
