@@ -1,0 +1,185 @@
+"""Benchmark of the extended spectral wavefront tracer on one MI355X.
+
+Workload: Cornell box plus 4096 random spheres (60 % Lambert, 25 % dispersive
+dielectric, 15 % conductor), 1920x1080, path depth 12, 4 hero wavelengths.
+A step is one dispatch tracing `--spp` samples of every pixel (one wave of
+paths); the timed region is exactly `--steps` dispatches bracketed by stream
+synchronisation, with the scene, BVH and path buffers resident in HBM.
+
+Prints one JSON line: metric, value (spectral path-segments/s), roofline of
+the dominant kernel (algorithmic bytes / its average HIP-event duration), and
+the CPU baseline (the oracle timed on this host on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "spectral path-segments/s; p50/p99 time-to-RMSE≤1% per render request"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per unit of work of each kernel: the SoA records the
+# kernel must read and write once (path state, hit, queue entries), excluding
+# the BVH and scene tables that stay in L2.
+#   unit of k_ext_intersect = one extension ray (segment):
+#     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8
+#     + primitive material/light lookup 8 + material-queue entry 4      = 48 B
+#   unit of k_ext_shadow = one shadow ray:
+#     record (origin 12, direction 12, tmax 4, XYZ 12, path 4) 44 + path XYZ read+write 24 = 68 B
+#   unit of k_ext_shade_lambert = one Lambert vertex:
+#     queue 4 + ray 24 + hit 8 + throughput 16 + wavelength 4 + key 4 + flags 4
+#     + writes: ray 24, throughput 16, pdf 4, flags 4, shadow record 44, next-queue 4 = 160 B
+BYTES_PER_UNIT = {
+    "k_ext_intersect": (48, "segments"),
+    "k_ext_shadow": (68, "shadow_segments"),
+    "k_ext_shade_lambert": (160, "lambert_hits"),
+}
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--spp", type=int, default=2, help="samples per pixel per step")
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--spheres", type=int, default=4096)
+    ap.add_argument("--bvh-width", type=int, default=0)
+    ap.add_argument("--wave-paths", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary."""
+    for f in sorted((ROOT / "profiles").glob(f"pmc_{kernel}*.txt")):
+        for line in f.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                try:
+                    return float(line.split(":", 1)[1].split()[0])
+                except ValueError:
+                    return None
+    return None
+
+
+def cpu_baseline(args, scene_text: str):
+    """The FP64 oracle tracer (CPU restatement with its own BVH) on a bounded
+    sample of the same workload: full resolution, a number of samples per pixel
+    chosen to take about --cpu-seconds, all host threads."""
+    try:
+        from oracle import ptext
+    except Exception as e:  # oracle not built
+        return {"value": None, "unit": "segments/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    threads = os.cpu_count() or 1
+    probe = ptext.render(scene_text, args.width, args.height, spp=1, depth=args.depth, threads=threads,
+                         rows=(0, max(1, args.height // 16)))
+    rate = probe["segments"] / max(probe["seconds"], 1e-9)
+    rows = max(1, min(args.height, int(args.cpu_seconds * rate / max(1, probe["segments"]) * (args.height // 16))))
+    run = ptext.render(scene_text, args.width, args.height, spp=1, depth=args.depth, threads=threads, rows=(0, rows))
+    return {
+        "value": run["segments"] / run["seconds"],
+        "unit": "segments/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP64 CPU oracle with BVH, {args.width}x{rows} pixels of the same frame x 1 spp, "
+                  f"depth {args.depth}: {run['segments']} segments in {run['seconds']:.2f} s",
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is sample-split with one accumulator reduce, not built)", file=sys.stderr)
+        sys.exit(2)
+    import pathtracer_amd as pt
+    from pathtracer_amd import scenes
+
+    text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
+    W, H = args.width, args.height
+    ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width)
+    scene = ctx.scene_from_json(text)
+    r = pt.Renderer(ctx, scene, W, H, spf=args.spp, path_length=args.depth, mode=pt.MODE_EXT)
+    for _ in range(args.warmup):
+        r.render_frame()
+    ctx.sync()
+    ctx.reset_counters()
+    if not args.no_profile:
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.render_frame()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    cnt = ctx.counters()
+    prof = ctx.profile_read(reset=True) if not args.no_profile else {}
+    ctx.profile(False)
+    seg_rate = cnt["segments"] / dt
+
+    roofline = None
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        name, (launches, total_ms) = dom
+        bpu, unit = BYTES_PER_UNIT.get(name, (None, None))
+        if bpu is not None and launches > 0:
+            units = cnt.get(unit, 0)
+            achieved = bpu * units / (total_ms * 1e-3) / 1e9
+            traffic = traffic_from_profiles(name)
+            roofline = {
+                "bound": "hbm",
+                "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic,
+                "kernel": name,
+                "avg_launch_ms": round(total_ms / launches, 4),
+                "algorithmic_bytes_per_launch": round(bpu * units / launches),
+            }
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, text)
+    line = {
+        "metric": METRIC,
+        "value": seg_rate,
+        "unit": "segments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Cornell box + {args.spheres} random spheres (60% Lambert / 25% dielectric / 15% conductor), "
+                        f"{W}x{H}, depth {args.depth}, 4 hero wavelengths",
+            "spp_per_step": args.spp,
+            "paths_per_step": W * H * args.spp,
+            "segments": cnt["segments"],
+            "shadow_segments": cnt["shadow_segments"],
+            "segments_per_path": cnt["segments"] / max(1, cnt["paths"]),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if prof:
+        line["kernels_ms"] = {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

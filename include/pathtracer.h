@@ -137,6 +137,12 @@ typedef struct pt_counters {
     uint64_t paths;           /* camera paths started */
     uint64_t nan_samples;     /* samples discarded as NaN (src/shader.comp:1479-1487) */
     uint64_t dispatches;
+    /* extended mode: path vertices by class */
+    uint64_t escaped;         /* rays leaving the scene */
+    uint64_t emitter_hits;
+    uint64_t lambert_hits;
+    uint64_t dielectric_hits;
+    uint64_t conductor_hits;
 } pt_counters;
 
 typedef struct pt_ctx pt_ctx;
@@ -202,6 +208,19 @@ pt_status pt_sync(pt_ctx* ctx);
 /* Counters summed over dispatches since creation or the last reset (synchronises). */
 pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out);
 pt_status pt_counters_reset(pt_ctx* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream (the reference
+ * has only ImGui frame times, src/pathtracer.cpp:2963-2964). While enabled,
+ * every kernel launch is bracketed by two events; pt_profile_read
+ * synchronises and returns, per kernel name, the launch count and the summed
+ * GPU time in milliseconds, and with `reset` != 0 clears the totals. */
+typedef struct pt_kernel_stat {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+} pt_kernel_stat;
+pt_status pt_profile_enable(pt_ctx* ctx, int enable);
+pt_status pt_profile_read(pt_ctx* ctx, pt_kernel_stat* out, int cap, int* n_out, int reset);
 
 /* Display resolve of SaveRender / shader.frag (src/pathtracer.cpp:3491-3518,
  * src/shader.frag:81-99): XYZ -> Bradford E->D65 -> linear sRGB -> tonemap

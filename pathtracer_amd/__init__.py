@@ -99,6 +99,10 @@ class Config(C.Structure):
     ]
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
 class Counters(C.Structure):
     _fields_ = [
         ("segments", C.c_uint64),
@@ -107,6 +111,11 @@ class Counters(C.Structure):
         ("paths", C.c_uint64),
         ("nan_samples", C.c_uint64),
         ("dispatches", C.c_uint64),
+        ("escaped", C.c_uint64),
+        ("emitter_hits", C.c_uint64),
+        ("lambert_hits", C.c_uint64),
+        ("dielectric_hits", C.c_uint64),
+        ("conductor_hits", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -145,6 +154,8 @@ SIGNATURES = {
     "pt_sync": (C.c_int, [_P]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
     "pt_counters_reset": (C.c_int, [_P]),
+    "pt_profile_enable": (C.c_int, [_P, C.c_int]),
+    "pt_profile_read": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, C.POINTER(C.c_int), C.c_int]),
     "pt_resolve_srgb8": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint8), C.c_size_t]),
     "pt_math_eval_host": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
     "pt_math_eval_device": (
@@ -278,6 +289,17 @@ class Context:
 
     def reset_counters(self) -> None:
         self.check(lib.pt_counters_reset(self.h))
+
+    def profile(self, enable: bool = True) -> None:
+        self.check(lib.pt_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{kernel name: (launches, total GPU ms)} since the last reset."""
+        n = C.c_int()
+        self.check(lib.pt_profile_read(self.h, None, 0, C.byref(n), 0))
+        arr = (KernelStat * max(1, n.value))()
+        self.check(lib.pt_profile_read(self.h, arr, n.value, C.byref(n), 1 if reset else 0))
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(n.value)}
 
     def math_eval(self, fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -87,8 +87,8 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     }
     hipError_t e = hipSetDevice(device_ordinal);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 8 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(ctx->d_stats, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 16 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(ctx->d_stats, 0, 16 * sizeof(unsigned long long));
     if (e != hipSuccess) {
         std::string msg = std::string("pt_create: ") + hipGetErrorString(e);
         if (ctx->d_stats) (void)hipFree(ctx->d_stats);
@@ -106,6 +106,11 @@ void pt_destroy(pt_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ptk::ref_wave_release(ctx);
     ptk::ext_wave_release(ctx);
+    for (const ProfPending& p : ctx->prof_pending) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (hipEvent_t ev : ctx->prof_pool) (void)hipEventDestroy(ev);
     if (ctx->d_stats) (void)hipFree(ctx->d_stats);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -347,7 +352,7 @@ pt_status pt_sync(pt_ctx* ctx) {
 pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     if (!ctx || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_counters_get: NULL argument");
     (void)hipSetDevice(ctx->device);
-    unsigned long long h[8];
+    unsigned long long h[16];
     hipError_t e = hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_get");
@@ -357,15 +362,58 @@ pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     out->paths = h[3];
     out->nan_samples = h[4];
     out->dispatches = h[5];
+    out->escaped = h[6];
+    out->emitter_hits = h[7];
+    out->lambert_hits = h[8];
+    out->dielectric_hits = h[9];
+    out->conductor_hits = h[10];
     return PT_OK;
 }
 
 pt_status pt_counters_reset(pt_ctx* ctx) {
     if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_counters_reset: NULL context");
     (void)hipSetDevice(ctx->device);
-    hipError_t e = hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream);
+    hipError_t e = hipMemsetAsync(ctx->d_stats, 0, 16 * sizeof(unsigned long long), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_reset");
+    return PT_OK;
+}
+
+pt_status pt_profile_enable(pt_ctx* ctx, int enable) {
+    if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_profile_enable: NULL context");
+    ctx->prof = enable != 0;
+    return PT_OK;
+}
+
+pt_status pt_profile_read(pt_ctx* ctx, pt_kernel_stat* out, int cap, int* n_out, int reset) {
+    if (!ctx || cap < 0 || (cap > 0 && !out)) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_profile_read: bad argument");
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_profile_read");
+    for (const ProfPending& p : ctx->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            ctx->prof_totals[(size_t)p.kernel].ms += ms;
+            ctx->prof_totals[(size_t)p.kernel].launches += 1;
+        }
+        ctx->prof_pool.push_back(p.start);
+        ctx->prof_pool.push_back(p.stop);
+    }
+    ctx->prof_pending.clear();
+    int n = 0;
+    for (const ProfTotal& t : ctx->prof_totals) {
+        if (t.launches == 0) continue;
+        if (n < cap) {
+            std::memset(out[n].name, 0, sizeof out[n].name);
+            std::strncpy(out[n].name, t.name.c_str(), sizeof out[n].name - 1);
+            out[n].launches = t.launches;
+            out[n].total_ms = t.ms;
+        }
+        ++n;
+    }
+    if (n_out) *n_out = n;
+    if (reset)
+        for (ProfTotal& t : ctx->prof_totals) { t.launches = 0; t.ms = 0.0; }
     return PT_OK;
 }
 

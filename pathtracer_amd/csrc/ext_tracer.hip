@@ -1,17 +1,1478 @@
-// ext_tracer.hip — PT_MODE_EXT (extended physically based mode).
+// ext_tracer.hip — PT_MODE_EXT: the extended, physically based spectral
+// wavefront tracer for gfx950.
+//
+// Scene: spheres in a host-built W-wide BVH (W = 2, 4, 8) whose leaves index a
+// leaf-ordered copy of the sphere records (one load per primitive test),
+// traversed with a per-lane stack of 16-bit node indices in LDS;
+// parallelograms (boxes become six) tested directly when there are at most 16
+// of them, through their own BVH otherwise; infinite horizontal planes; SDF
+// objects (mandelbulb, menger, blob, terrain) sphere-traced by one kernel per
+// kind.
+// Materials: Lambert with the reference's one-lobe spectrum or a sum of up to
+// three Gaussians; smooth dielectric with Cauchy or Sellmeier dispersion;
+// smooth conductor with a complex index n + ik. Emitters are the reference's
+// normalised blackbodies; an optional uniform environment radiance.
+// Wavelengths: hero wavelength uniform in [lambdaMin, lambdaMax) plus
+// n_lambda - 1 rotated companions (Wilkie et al. 2014); a dispersive
+// refraction keeps only the hero.
+// Integrator: BSDF sampling + next-event estimation at Lambert vertices with
+// the power heuristic, Russian roulette after `rr_depth` bounces.
+// RNG: counter-based pcg_hash keyed on (seed, pixel, sample, bounce, dim), so a
+// pixel's samples do not depend on how paths are batched into waves.
+//
+// Per bounce: intersect (+ classify into material queues) -> march<kind> per
+// SDF kind [-> classify] -> shade_{miss, emit, lambert, dielectric, conductor}
+// -> shadow (any-hit) -> next bounce. Queues are compacted with wave64 ballots;
+// kernels run grid-stride over their queue.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "bvh.hpp"
 #include "tracer.hpp"
 
 namespace ptk {
 
-int ext_scene_build(pt_ctx* ctx, pt_scene*) {
-    ctx->err = "extended mode is not built yet";
-    return PT_ERR_STATE;
+using namespace ptd;
+
+// ---------------------------------------------------------------------------------
+// Device scene.
+enum : uint32_t { REF_SPHERE = 0u, REF_QUAD = 1u, REF_PLANE = 2u, REF_SDF = 3u };
+constexpr uint32_t kRefShift = 29u, kRefMask = (1u << 29) - 1u, kMiss = 0xFFFFFFFFu;
+enum : int { Q_MISS = 0, Q_EMIT = 1, Q_LAMBERT = 2, Q_DIELECTRIC = 3, Q_CONDUCTOR = 4, N_MATQ = 5 };
+constexpr int kQuadBrute = 16;  // at most this many parallelograms are tested without a BVH
+
+struct ExtMat {
+    int type;        // pt::MaterialType
+    int n;           // gaussian count | dispersion model
+    int dispersive;  // dielectric index depends on wavelength
+    int pad;
+    float p[9];      // lobe (peak, sigma, invert) | gaussians (w, mean, sigma) x n | dispersion | (n, k)
+};
+struct ExtLight {
+    int type;        // REF_SPHERE or REF_QUAD
+    int prim;
+    float T, lum;    // blackbody temperature, luminosity
+    f3 p, e1, e2, n; // quad: corner, edges, unit normal; sphere: centre in p
+    float area, radius;
+};
+struct ExtSdf {
+    int kind, iters, mat, pad;
+    float bailout;
+    f3 lo, hi, c;
+};
+// A BVH over one primitive type with leaf-ordered primitive records.
+struct BvhDev {
+    const uint32_t* nodes;  // 8 * width words per node (bvh.hpp)
+    const float4* prims;    // records in leaf order (1 float4 per sphere, 4 per quad)
+    const int* ids;         // leaf slot -> primitive index
+    int count;
+};
+
+struct ExtSceneDev {
+    const float4* sph;  // (centre, radius) by sphere index
+    const int2* sph_ml; // (material, light or -1)
+    const float4* quad; // 4 float4 per quad: (Q, w.Q), (w, area), (e2 x w), (w x e1)
+    const int2* quad_ml;
+    const float* plane_y;
+    const int2* plane_ml;
+    const ExtSdf* sdf;
+    const ExtMat* mat;
+    const ExtLight* light;
+    BvhDev sbvh, qbvh;
+    int nsph, nquad, nplane, nsdf, nmat, nlight;
+    int quad_bvh;       // parallelograms go through qbvh
+    int has_env;
+    float env;
+    int n_lambda;
+    float lmin, lrange;
+    int rr_depth;
+    uint32_t seed;
+    int camera;  // 0 lens, 1 pinhole
+    int sdf_steps_max;
+};
+
+struct ExtFrame {
+    int W, H, frame, spf, maxDepth;
+    float camSize, apSize, apDist;
+    f3 camPos, fwd;
+    m3 camM;
+    f3 lensPos;
+    m3 lensM;
+    float lensHalf, lensSlicePos, lensTwoF;
+    int p0, pc, k0, m;
+};
+
+// SoA state of one extended wave, in 16-byte records.
+struct ExtWave {
+    float4* org;   // ray origin, w unused
+    float4* dir;   // ray direction
+    float4* T;     // throughput per wavelength
+    float4* acc;   // XYZ gathered so far, w = hero wavelength
+    uint4* aux;    // x: RNG key, y: flags (bit 0 companions dropped, bit 1 last bounce specular), z: BSDF pdf bits
+    float2* hit;   // (t, primitive reference bits)
+    int* q[2];
+    int* mq[N_MATQ];
+    float4* s_org; // shadow ray origin, w = tmax
+    float4* s_dir; // direction, w = path index bits
+    float4* s_xyz; // unoccluded contribution
+    float* sum;
+    int* counts;   // [0..1] active, [2] shadow, [3..7] material queues
+    unsigned long long* stats;
+    int cap;
+};
+
+static __constant__ float c_cie[PT_CIE_SIZE] = {
+#include "cie1931_2deg.inc"
+};
+
+constexpr int kBlk = 256;
+constexpr float kPiF = 3.14159265358979f;
+constexpr float kEps = 1e-6f;
+
+// ---------------------------------------------------------------------------------
+// Counter-based random numbers: pcg_hash of (path key, bounce, dimension).
+__device__ __forceinline__ uint32_t path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    return pcg(pcg(pcg(seed) + pixel) + sample);
 }
-void ext_scene_release(pt_ctx*, pt_scene*) {}
-int ext_dispatch(pt_ctx* ctx, const pt_scene*, pt_accum*, const pt_push_constants*) {
-    ctx->err = "extended mode is not built yet";
-    return PT_ERR_STATE;
+__device__ __forceinline__ float rnd(uint32_t key, uint32_t bounce, uint32_t dim) {
+    uint32_t h = pcg(key + pcg((bounce << 4) | dim));
+    return (float)(h >> 8) * 5.9604644775390625e-8f;  // [0, 1) in steps of 2^-24
 }
-void ext_wave_release(pt_ctx*) {}
+constexpr uint32_t kCamBounce = 0xFFFu;
+
+__device__ __forceinline__ f3 cmf(float l) {
+    float x = fminf(fmaxf(l - 360.0f, 0.0f), 439.999f);
+    int i = (int)x;
+    float t = x - (float)i;
+    const float* a = &c_cie[3 * i];
+    return mk3(a[0] + (a[3] - a[0]) * t, a[1] + (a[4] - a[1]) * t, a[2] + (a[5] - a[2]) * t);
+}
+__device__ __forceinline__ float planck_norm(float lnm, float T, float lum) {
+    // The reference's emitter: Planck's law at lnm normalised by its Wien peak, times luminosity.
+    float t = fmaxf(T, 0.0f);
+    float l = lnm * 1e-9f;
+    float l2 = l * l;
+    float b = 1.1910429724e-16f / (l2 * l2 * l) / (__expf(0.014387768775f / (l * t)) - 1.0f);
+    float t2 = t * t;
+    float peak = 4.0956746759e-6f * (t2 * t2 * t);
+    return (b / peak) * fmaxf(lum, 0.0f);
+}
+__device__ __forceinline__ float reflectance(const ExtMat& m, float l) {
+    if (m.type == pt::MAT_LAMBERT_LOBE) {
+        float x = (l - m.p[0]) / (2.0f * m.p[1] * m.p[1]);
+        float r = __expf(-x * x);
+        return m.p[2] != 0.0f ? 1.0f - r : r;
+    }
+    float s = 0.0f;
+    for (int i = 0; i < m.n; ++i) {
+        float d = (l - m.p[3 * i + 1]) / m.p[3 * i + 2];
+        s += m.p[3 * i] * __expf(-0.5f * d * d);
+    }
+    return fminf(s, 1.0f);
+}
+__device__ __forceinline__ float ior(const ExtMat& m, float lnm) {
+    float lu = lnm * 1e-3f;
+    float l2 = lu * lu;
+    if (m.n == pt::DISP_CAUCHY) return m.p[0] + m.p[1] / l2;
+    float n2 = 1.0f + m.p[0] * l2 / (l2 - m.p[3]) + m.p[1] * l2 / (l2 - m.p[4]) + m.p[2] * l2 / (l2 - m.p[5]);
+    return sqrtf(n2);
+}
+// Unpolarised Fresnel reflectance of a dielectric interface.
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float cos_t, float eta_i, float eta_t) {
+    float rs = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+    float rp = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+    return 0.5f * (rs * rs + rp * rp);
+}
+// Unpolarised Fresnel reflectance of a conductor with index n + ik seen from
+// vacuum, with cos_t = sqrt(1 - sin^2 / eta^2) in complex arithmetic.
+__device__ __forceinline__ float fresnel_conductor(float cos_i, float n, float k) {
+    float s2 = fmaxf(1.0f - cos_i * cos_i, 0.0f);
+    float er = n * n - k * k, ei = 2.0f * n * k;  // eta^2
+    float den = er * er + ei * ei;
+    float zr = 1.0f - s2 * er / den, zi = s2 * ei / den;
+    float m = sqrtf(zr * zr + zi * zi);
+    float cr = sqrtf(fmaxf(0.5f * (m + zr), 0.0f));
+    float ci = copysignf(sqrtf(fmaxf(0.5f * (m - zr), 0.0f)), zi);
+    float ect_r = n * cr - k * ci, ect_i = n * ci + k * cr;  // eta cos_t
+    float a_r = cos_i - ect_r, a_i = -ect_i, b_r = cos_i + ect_r, b_i = ect_i;
+    float rperp = (a_r * a_r + a_i * a_i) / (b_r * b_r + b_i * b_i);
+    float eci_r = n * cos_i, eci_i = k * cos_i;  // eta cos_i
+    float c_r = eci_r - cr, c_i = eci_i - ci, d_r = eci_r + cr, d_i = eci_i + ci;
+    float rpar = (c_r * c_r + c_i * c_i) / (d_r * d_r + d_i * d_i);
+    return 0.5f * (rperp + rpar);
+}
+// Orthonormal basis around n (Duff et al. 2017, "Building an orthonormal basis, revisited").
+__device__ __forceinline__ void onb(f3 n, f3& t, f3& b) {
+    float s = copysignf(1.0f, n.z);
+    float a = -1.0f / (s + n.z);
+    float c = n.x * n.y * a;
+    t = mk3(1.0f + s * n.x * n.x * a, s * c, -s * n.x);
+    b = mk3(c, s + n.y * n.y * a, -n.y);
+}
+__device__ __forceinline__ float offset_scale(f3 p) {
+    return 5e-6f * fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+}
+
+// ---------------------------------------------------------------------------------
+// Primitive intersection.
+__device__ __forceinline__ bool isect_sphere(float4 s, f3 o, f3 d, float tmax, float& t) {
+    f3 oc = o - mk3(s.x, s.y, s.z);
+    float b = dot(oc, d);
+    f3 f = oc - d * b;  // component of oc orthogonal to the ray (Haines et al., Ray Tracing Gems ch. 7)
+    float disc = s.w * s.w - dot(f, f);
+    if (disc < 0.0f) return false;
+    float q = sqrtf(disc);
+    float t0 = -b - q;
+    if (t0 > kEps && t0 < tmax) { t = t0; return true; }
+    float t1 = -b + q;
+    if (t1 > kEps && t1 < tmax) { t = t1; return true; }
+    return false;
+}
+__device__ __forceinline__ bool isect_quad(const float4* q, f3 o, f3 d, float tmax, float& t) {
+    float4 q0 = q[0], q1 = q[1];
+    f3 w = mk3(q1.x, q1.y, q1.z);
+    float den = dot(w, d);
+    if (fabsf(den) < 1e-20f) return false;
+    float tt = (q0.w - dot(w, o)) / den;
+    if (!(tt > kEps && tt < tmax)) return false;
+    float4 q2 = q[2], q3 = q[3];
+    f3 hp = o + d * tt - mk3(q0.x, q0.y, q0.z);
+    float a = q2.x * hp.x + q2.y * hp.y + q2.z * hp.z;
+    float b = q3.x * hp.x + q3.y * hp.y + q3.z * hp.z;
+    if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return false;
+    t = tt;
+    return true;
+}
+
+// ---------------------------------------------------------------------------------
+// Wide-BVH traversal. `stack` is this lane's column of the block's LDS stack
+// (stride kBlk); STACK bounds the entries (checked on the host against the
+// tree's exact worst case) and SE is the entry type (16-bit when the tree has
+// fewer than 65536 nodes). Returns the leaf slot of the closest hit (or of
+// any hit with ANY), -1 if none; tbest shrinks to the hit distance.
+template <int W, int STACK, typename SE, bool ANY, bool QUAD>
+__device__ __forceinline__ int bvh_trace(const BvhDev& B, f3 o, f3 d, float& tbest, SE* stack) {
+    const float big = 1e30f;
+    f3 inv = mk3(d.x != 0.0f ? 1.0f / d.x : copysignf(big, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(big, d.y),
+                 d.z != 0.0f ? 1.0f / d.z : copysignf(big, d.z));
+    f3 oi = o * inv;
+    int hit = -1;
+    int sp = 1;
+    stack[0] = (SE)0;
+    while (sp > 0) {
+        int node = (int)stack[(--sp) * kBlk];
+        const float4* nd = reinterpret_cast<const float4*>(B.nodes + (size_t)node * (8 * W));
+        float lo_x[W], lo_y[W], lo_z[W], hi_x[W], hi_y[W], hi_z[W];
+        int kid[W], cnt[W];
+        {
+            const float* f = reinterpret_cast<const float*>(nd);
+            const int* iv = reinterpret_cast<const int*>(nd);
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                lo_x[j] = f[0 * W + j]; lo_y[j] = f[1 * W + j]; lo_z[j] = f[2 * W + j];
+                hi_x[j] = f[3 * W + j]; hi_y[j] = f[4 * W + j]; hi_z[j] = f[5 * W + j];
+                kid[j] = iv[6 * W + j]; cnt[j] = iv[7 * W + j];
+            }
+        }
+        float tn[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            float tx0 = fmaf(lo_x[j], inv.x, -oi.x), tx1 = fmaf(hi_x[j], inv.x, -oi.x);
+            float ty0 = fmaf(lo_y[j], inv.y, -oi.y), ty1 = fmaf(hi_y[j], inv.y, -oi.y);
+            float tz0 = fmaf(lo_z[j], inv.z, -oi.z), tz1 = fmaf(hi_z[j], inv.z, -oi.z);
+            float lo = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+            float hi = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
+            tn[j] = (lo <= hi && cnt[j] >= 0) ? lo : big;
+        }
+        // Leaves first (they can shrink tbest), then internal children.
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (tn[j] < big && cnt[j] > 0) {
+                for (int k = kid[j]; k < kid[j] + cnt[j]; ++k) {
+                    float t;
+                    bool h = QUAD ? isect_quad(B.prims + 4 * k, o, d, tbest, t) : isect_sphere(B.prims[k], o, d, tbest, t);
+                    if (h) {
+                        tbest = t;
+                        hit = k;
+                        if (ANY) return hit;
+                    }
+                }
+                tn[j] = big;
+            }
+        }
+        if (ANY) {
+#pragma unroll
+            for (int j = 0; j < W; ++j)
+                if (tn[j] < tbest && cnt[j] == 0) stack[(sp++) * kBlk] = (SE)kid[j];
+        } else {
+            // push far-to-near so the nearest child is visited next
+            float ds[W];
+            int ks[W];
+            int n = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                if (tn[j] < tbest && cnt[j] == 0) {
+                    float t = tn[j];
+                    int c = kid[j];
+                    int p = n++;
+                    while (p > 0 && ds[p - 1] < t) {
+                        ds[p] = ds[p - 1];
+                        ks[p] = ks[p - 1];
+                        --p;
+                    }
+                    ds[p] = t;
+                    ks[p] = c;
+                }
+            }
+            for (int j = 0; j < n; ++j) stack[(sp++) * kBlk] = (SE)ks[j];
+        }
+    }
+    return hit;
+}
+
+// Closest (or any, with ANY) hit over parallelograms, spheres and planes.
+template <int W, int STACK, typename SE, bool ANY>
+__device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref, SE* stack) {
+    bool hit = false;
+    if (S.quad_bvh) {
+        int k = bvh_trace<W, STACK, SE, ANY, true>(S.qbvh, o, d, t, stack);
+        if (k >= 0) {
+            ref = (REF_QUAD << kRefShift) | (uint32_t)S.qbvh.ids[k];
+            hit = true;
+            if (ANY) return true;
+        }
+    } else {
+        for (int i = 0; i < S.nquad; ++i) {  // uniform index: scalar loads
+            float tq;
+            if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
+                t = tq;
+                ref = (REF_QUAD << kRefShift) | (uint32_t)i;
+                hit = true;
+                if (ANY) return true;
+            }
+        }
+    }
+    if (S.nsph > 0) {
+        int k = bvh_trace<W, STACK, SE, ANY, false>(S.sbvh, o, d, t, stack);
+        if (k >= 0) {
+            ref = (REF_SPHERE << kRefShift) | (uint32_t)S.sbvh.ids[k];
+            hit = true;
+            if (ANY) return true;
+        }
+    }
+    for (int i = 0; i < S.nplane; ++i) {
+        float tp = (S.plane_y[i] - o.y) / d.y;
+        if (tp > kEps && tp < t) {
+            t = tp;
+            ref = (REF_PLANE << kRefShift) | (uint32_t)i;
+            hit = true;
+            if (ANY) return true;
+        }
+    }
+    return hit;
+}
+
+// ---------------------------------------------------------------------------------
+// SDF distance estimators (extended mode: the shapes of sdfs/*.glsl in plain
+// FP32 arithmetic).
+__device__ __forceinline__ float ext_mandelbulb(f3 p, int iters, float bailout) {
+    f3 z = p;
+    float r = length(z);
+    float dr = 1.0f;
+    for (int i = 0; i < iters; ++i) {
+        float r2 = r * r, r4 = r2 * r2;
+        dr = 8.0f * (r4 * r2 * r) * dr + 1.0f;
+        float invr = 1.0f / r;
+        float invl = rsqrtf(z.x * z.x + z.z * z.z);
+        float r8 = r4 * r4;
+        float cost = z.z * invl, cosp = z.y * invr, sint = z.x * invl, sinp = invr / invl;
+        float ct2 = cost * cost, cp2 = cosp * cosp;
+        float cost8 = fmaf(fmaf(fmaf(fmaf(128.0f, ct2, -256.0f), ct2, 160.0f), ct2, -32.0f), ct2, 1.0f);
+        float cosp8 = fmaf(fmaf(fmaf(fmaf(128.0f, cp2, -256.0f), cp2, 160.0f), cp2, -32.0f), cp2, 1.0f);
+        float sint8 = fmaf(fmaf(fmaf(128.0f, ct2, -192.0f), ct2, 80.0f), ct2, -8.0f) * sint * cost;
+        float sinp8 = fmaf(fmaf(fmaf(128.0f, cp2, -192.0f), cp2, 80.0f), cp2, -8.0f) * sinp * cosp;
+        z = r8 * mk3(sint8 * sinp8, cosp8, cost8 * sinp8) + p;
+        r = length(z);
+        if (r > bailout) break;
+    }
+    return 0.5f * __logf(r) * r / dr;
+}
+__device__ __forceinline__ float ext_menger(f3 p, int levels) {
+    f3 q = abs3(p) - mk3(1.0f, 1.0f, 1.0f);
+    float d = length(max3(q, mk3(0.0f, 0.0f, 0.0f))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
+    float s = 1.0f;
+    for (int m = 0; m < levels; ++m) {
+        f3 ps = p * s;
+        f3 a = mk3(ps.x - 2.0f * floorf(ps.x * 0.5f) - 1.0f, ps.y - 2.0f * floorf(ps.y * 0.5f) - 1.0f,
+                   ps.z - 2.0f * floorf(ps.z * 0.5f) - 1.0f);
+        s *= 3.0f;
+        f3 r = abs3(mk3(1.0f - 3.0f * fabsf(a.x), 1.0f - 3.0f * fabsf(a.y), 1.0f - 3.0f * fabsf(a.z)));
+        float da = fmaxf(r.x, r.y), db = fmaxf(r.y, r.z), dc = fmaxf(r.z, r.x);
+        d = fmaxf(d, (fminf(da, fminf(db, dc)) - 1.0f) / s);
+    }
+    return d;
+}
+__device__ __forceinline__ float ext_smin(float x, float y) {
+    const float k = 0.12f;
+    float h = fmaxf(k - fabsf(x - y), 0.0f) / k;
+    return fminf(x, y) - h * h * h * k * (1.0f / 6.0f);
+}
+__device__ __forceinline__ float ext_blob(f3 p) {
+    float s1 = length(p) - 1.0f;
+    float s2 = length(mk3(p.x, p.y - 1.0f, p.z)) - 0.7f;
+    float s3 = length(mk3(p.x - 0.25f, p.y - 1.25f, p.z - 0.5f)) - 0.25f;
+    float s4 = length(mk3(p.x + 0.25f, p.y - 1.25f, p.z - 0.5f)) - 0.25f;
+    return ext_smin(ext_smin(ext_smin(s1, s2), s3), s4);
+}
+__device__ __forceinline__ float ext_terrain_h(float x) {
+    return __sinf(0.0625f * x) + __sinf(0.125f * (x + 10.0f)) + 0.25f * __sinf(0.25f * x) +
+           0.125f * __sinf(0.5f * x) + 0.25f * __sinf(x) + 0.25f * __sinf(2.0f * x + 1.0f) +
+           0.125f * __sinf(4.0f * x + 2.0f) + 0.0625f * __sinf(8.0f * x + 1.0f) +
+           0.03125f * __sinf(16.0f * x + 5.0f) + 0.015625f * __sinf(32.0f * x) + 0.0078125f * __sinf(64.0f * x);
+}
+template <int KIND>
+__device__ __forceinline__ float ext_de(const ExtSdf& s, f3 p) {
+    f3 q = p - s.c;
+    const int kind = KIND ? KIND : s.kind;
+    switch (kind) {
+        case PT_SDF_MANDELBULB: return ext_mandelbulb(q, s.iters, s.bailout);
+        case PT_SDF_MENGER: return ext_menger(q, s.iters);
+        case PT_SDF_BLOB: return ext_blob(q);
+        default: return (q.y - (ext_terrain_h(q.x) + ext_terrain_h(q.z))) * 0.5f;  // gradient bound ~2
+    }
+}
+// Sphere tracing of one SDF object inside its box, up to tmax. Returns the hit
+// distance or a negative value.
+template <int KIND>
+__device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_steps, unsigned& steps) {
+    f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    f3 t0 = (s.lo - o) * inv, t1 = (s.hi - o) * inv;
+    f3 a = min3(t0, t1), b = max3(t0, t1);
+    float tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
+    float tout = fminf(fminf(fminf(b.x, b.y), b.z), tmax);
+    if (!(tin < tout)) return -1.0f;
+    float t = tin;
+    for (int i = 0; i < max_steps; ++i) {
+        float dist = ext_de<KIND>(s, o + d * t);
+        ++steps;
+        if (dist < 1e-4f) return t;
+        t += dist;
+        if (t > tout) return -1.0f;
+    }
+    return -1.0f;
+}
+template <int KIND>
+__device__ f3 ext_sdf_normal(const ExtSdf& s, f3 p) {
+    const float e = 1e-4f;
+    return normalize(mk3(ext_de<KIND>(s, p + mk3(e, 0, 0)) - ext_de<KIND>(s, p - mk3(e, 0, 0)),
+                         ext_de<KIND>(s, p + mk3(0, e, 0)) - ext_de<KIND>(s, p - mk3(0, e, 0)),
+                         ext_de<KIND>(s, p + mk3(0, 0, e)) - ext_de<KIND>(s, p - mk3(0, 0, e))));
+}
+
+// ---------------------------------------------------------------------------------
+// Hit attributes.
+struct Surf {
+    f3 p, n;   // position, geometric normal (unit, not face-forwarded)
+    int mat, light;
+};
+__device__ __forceinline__ Surf surface(const ExtSceneDev& S, uint32_t ref, f3 o, f3 d, float t) {
+    Surf s;
+    s.p = o + d * t;
+    uint32_t idx = ref & kRefMask;
+    switch (ref >> kRefShift) {
+        case REF_SPHERE: {
+            float4 c = S.sph[idx];
+            s.n = (s.p - mk3(c.x, c.y, c.z)) * (1.0f / c.w);
+            int2 ml = S.sph_ml[idx];
+            s.mat = ml.x;
+            s.light = ml.y;
+            break;
+        }
+        case REF_QUAD: {
+            float4 w = S.quad[4 * idx + 1];
+            s.n = normalize(mk3(w.x, w.y, w.z));
+            int2 ml = S.quad_ml[idx];
+            s.mat = ml.x;
+            s.light = ml.y;
+            break;
+        }
+        case REF_PLANE: {
+            s.n = mk3(0.0f, 1.0f, 0.0f);
+            int2 ml = S.plane_ml[idx];
+            s.mat = ml.x;
+            s.light = ml.y;
+            break;
+        }
+        default: {
+            const ExtSdf& q = S.sdf[idx];
+            s.n = ext_sdf_normal<0>(q, s.p);
+            s.mat = q.mat;
+            s.light = -1;
+            break;
+        }
+    }
+    return s;
+}
+__device__ __forceinline__ int hit_class(const ExtSceneDev& S, uint32_t ref) {
+    if (ref == kMiss) return Q_MISS;
+    uint32_t idx = ref & kRefMask;
+    int mat, light;
+    switch (ref >> kRefShift) {
+        case REF_SPHERE: { int2 ml = S.sph_ml[idx]; mat = ml.x; light = ml.y; break; }
+        case REF_QUAD: { int2 ml = S.quad_ml[idx]; mat = ml.x; light = ml.y; break; }
+        case REF_PLANE: { int2 ml = S.plane_ml[idx]; mat = ml.x; light = ml.y; break; }
+        default: mat = S.sdf[idx].mat; light = -1; break;
+    }
+    if (light >= 0) return Q_EMIT;
+    int type = S.mat[mat].type;
+    if (type == pt::MAT_DIELECTRIC) return Q_DIELECTRIC;
+    if (type == pt::MAT_CONDUCTOR) return Q_CONDUCTOR;
+    return Q_LAMBERT;
+}
+
+__device__ __forceinline__ void wavelengths(const ExtSceneDev& S, float l0, float l[4]) {
+    l[0] = l0;
+    float step = S.lrange / (float)S.n_lambda;
+    for (int j = 1; j < 4; ++j) {
+        float x = l0 - S.lmin + step * (float)j;
+        if (x >= S.lrange) x -= S.lrange;
+        l[j] = S.lmin + x;
+    }
+}
+// Radiance samples c_j at the path's wavelengths -> XYZ (uniform wavelength pdf).
+__device__ __forceinline__ f3 to_xyz(const ExtSceneDev& S, const float l[4], f4 c) {
+    float w = S.lrange / (float)S.n_lambda;
+    f3 r = cmf(l[0]) * c.x;
+    if (S.n_lambda > 1) r = r + cmf(l[1]) * c.y + cmf(l[2]) * c.z + cmf(l[3]) * c.w;
+    return r * w;
+}
+__device__ __forceinline__ f4 ld4(const float4* a, int p) {
+    float4 v = a[p];
+    return f4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ f3 ld3(const float4* a, int p) {
+    float4 v = a[p];
+    return mk3(v.x, v.y, v.z);
+}
+__device__ __forceinline__ void add_xyz(ExtWave& Wv, int p, f3 c) {
+    float4 a = Wv.acc[p];
+    Wv.acc[p] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w);
+}
+
+// ---------------------------------------------------------------------------------
+// Kernels.
+__global__ void k_ext_counts(int* counts, int n0) {
+    int i = threadIdx.x;
+    if (i < 16) counts[i] = (i == 0) ? n0 : 0;
+}
+__global__ void k_ext_reset(int* counts, int nxt) {
+    int i = threadIdx.x;
+    if (i == 0) counts[nxt] = 0;
+    if (i >= 2 && i < 3 + N_MATQ) counts[i] = 0;
+}
+
+// BK7 Sellmeier index (the reference camera lens, src/shader.comp:1064-1073).
+__device__ __forceinline__ float bk7_index(float lnm) {
+    float l = lnm * 1e-3f;
+    float l2 = l * l;
+    float n2 = 1.0f + (1.03961212f * l2) / (l2 - 6.00069867e-3f) + (0.231792344f * l2) / (l2 - 2.00179144e-2f) +
+               (1.01046945f * l2) / (l2 - 1.03560653e2f);
+    return sqrtf(n2);
+}
+
+__global__ void __launch_bounds__(kBlk) k_ext_raygen(ExtSceneDev S, ExtFrame F, ExtWave Wv) {
+    int n = F.pc * F.m;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)n);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int pix = F.p0 + i % F.pc;
+        int k = F.k0 + i / F.pc;
+        int gx = pix % F.W, gy = pix / F.W;
+        uint32_t sample = (uint32_t)(F.frame - F.spf + k);
+        uint32_t key = path_key(S.seed, (uint32_t)pix, sample);
+        float ju = rnd(key, kCamBounce, 0), jv = rnd(key, kCamBounce, 1);
+        float uvx = (2.0f * ((float)gx + ju) - (float)F.W) / (float)F.H;
+        float uvy = (2.0f * ((float)(F.H - gy) - jv) - (float)F.H) / (float)F.H;
+        float l0 = S.lmin + S.lrange * rnd(key, kCamBounce, 4);
+        f3 o, d;
+        if (S.camera == 1) {
+            float h = 0.5f * F.camSize;
+            o = F.camPos;
+            d = normalize(vmul(mk3(uvx * h, uvy * h, F.apDist), F.camM));
+        } else {
+            // The reference's camera (sensor, aperture disk, BK7 bi-convex lens) at the hero wavelength.
+            float sc = -F.camSize * 0.5f;
+            o = F.camPos + vmul(mk3(uvx * sc, uvy * sc, 0.0f), F.camM);
+            float ux = rnd(key, kCamBounce, 2), uy = rnd(key, kCamBounce, 3);
+            float phi = 2.0f * kPiF * uy, dr = sqrtf(ux);
+            float ap = 0.5f * F.apSize;
+            f3 onAp = F.camPos + vmul(mk3(ap * dr * __cosf(phi), ap * dr * __sinf(phi), F.apDist), F.camM);
+            d = normalize(onAp - o);
+            float l = l0;
+            for (int pass = 0; pass < 2; ++pass) {
+                // the two spherical caps (radius 2f) of the lens
+                float best = 1e6f;
+                f3 nb = mk3(0.0f, 0.0f, 0.0f);
+                int outside = 1;
+                for (int sidx = 0; sidx < 2; ++sidx) {
+                    bool first = sidx == 0;
+                    float off = F.lensTwoF - F.lensHalf;
+                    f3 lo = vmul(o - F.lensPos, F.lensM);
+                    if (first) lo.x += F.lensSlicePos - F.lensHalf - off;
+                    else lo.x -= F.lensSlicePos - F.lensHalf - off;
+                    f3 dl = vmul(d, F.lensM);
+                    float b = 2.0f * dot(dl, lo), c = dot(lo, lo) - F.lensTwoF * F.lensTwoF;
+                    float disc = b * b - 4.0f * c;
+                    if (disc < 0.0f) continue;
+                    float sq = sqrtf(disc);
+                    float t1 = (-b - sq) * 0.5f, t2 = (-b + sq) * 0.5f;
+                    if (first) {
+                        t1 = (fmaf(dl.x, t1, lo.x) > -off) ? 1e6f : t1;
+                        t2 = (fmaf(dl.x, t2, lo.x) > -off) ? 1e6f : t2;
+                    } else {
+                        t1 = (fmaf(dl.x, t1, lo.x) < off) ? 1e6f : t1;
+                        t2 = (fmaf(dl.x, t2, lo.x) < off) ? 1e6f : t2;
+                    }
+                    float t = (t1 > 0.0f) ? t1 : 1e6f;
+                    int out = 1;
+                    if (t2 < t) { t = t2; out = -1; }
+                    if (t < 1e-4f || !(t < best)) continue;
+                    best = t;
+                    nb = mvmul(F.lensM, normalize(fma3s(dl, t, lo) * (float)out));
+                    outside = out;
+                }
+                float n1 = 1.0f, n2 = 1.0f;
+                if (outside == 1) n2 = bk7_index(l);
+                else n1 = bk7_index(l);
+                float n12 = n1 / n2;
+                l = l * n12;
+                o = fma3s(d, best, o);
+                d = refract(d, nb, n12);
+            }
+        }
+        Wv.org[i] = make_float4(o.x, o.y, o.z, 0.0f);
+        Wv.dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
+        Wv.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, l0);
+        Wv.aux[i] = make_uint4(key, 2u, 0u, 0u);  // the camera ray counts as specular for emitter MIS
+        Wv.q[0][i] = i;
+    }
+}
+
+__device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& Wv, bool active, int p, uint32_t ref) {
+    int cls = active ? hit_class(S, ref) : -1;
+#pragma unroll
+    for (int c = 0; c < N_MATQ; ++c) {
+        int slot = wave_append(&Wv.counts[3 + c], cls == c);
+        if (cls == c) Wv.mq[c][slot] = p;
+    }
+}
+
+// Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
+// is also appended to its material queue (scenes without SDFs).
+template <int W, int STACK, typename SE, bool CLASSIFY>
+__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave Wv, int cur) {
+    __shared__ SE stack_mem[STACK * kBlk];
+    SE* stack = stack_mem + threadIdx.x;
+    const int n = Wv.counts[cur];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = idx < n;
+        int p = active ? Wv.q[cur][idx] : 0;
+        uint32_t ref = kMiss;
+        if (active) {
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            float t = 1e30f;
+            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack);
+            Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+        }
+        if (CLASSIFY) classify_append(S, Wv, active, p, ref);
+    }
+}
+
+// Sphere tracing of every SDF object of one kind against the current hit.
+template <int KIND>
+__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, int cur) {
+    const int n = Wv.counts[cur];
+    unsigned steps = 0;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        int p = Wv.q[cur][idx];
+        f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+        float2 h = Wv.hit[p];
+        float t = h.x;
+        uint32_t ref = __float_as_uint(h.y);
+        bool upd = false;
+        for (int s = 0; s < S.nsdf; ++s) {
+            if (S.sdf[s].kind != KIND) continue;
+            float th = ext_march<KIND>(S.sdf[s], o, d, t, S.sdf_steps_max, steps);
+            if (th > 0.0f && th < t) {
+                t = th;
+                ref = (REF_SDF << kRefShift) | (uint32_t)s;
+                upd = true;
+            }
+        }
+        if (upd) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+    }
+    wave_add_u64(&Wv.stats[2], steps);
+}
+
+__global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S, ExtWave Wv, int cur) {
+    const int n = Wv.counts[cur];
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = idx < n;
+        int p = active ? Wv.q[cur][idx] : 0;
+        classify_append(S, Wv, active, p, active ? __float_as_uint(Wv.hit[p].y) : kMiss);
+    }
+}
+
+// Russian roulette from depth rr_depth; returns false when the path ends.
+__device__ __forceinline__ bool roulette(const ExtSceneDev& S, uint32_t key, int bounce, f4& T) {
+    if (bounce + 1 < S.rr_depth) return true;
+    float m = fmaxf(fmaxf(T.x, T.y), fmaxf(T.z, T.w));
+    float q = fminf(m, 0.95f);
+    if (!(rnd(key, (uint32_t)bounce, 6) < q)) return false;
+    T = T * (1.0f / q);
+    return true;
+}
+
+// Escaped paths (environment) and emitter hits (weighted by MIS against NEE).
+__global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S, ExtWave Wv, int which) {
+    const int n = Wv.counts[3 + which];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + which], (unsigned long long)n);
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        int p = Wv.mq[which][idx];
+        f4 T = ld4(Wv.T, p);
+        float l[4];
+        wavelengths(S, Wv.acc[p].w, l);
+        if (which == Q_MISS) {
+            if (S.has_env) add_xyz(Wv, p, to_xyz(S, l, T * S.env));
+            continue;
+        }
+        f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+        float2 h = Wv.hit[p];
+        Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+        const ExtLight& L = S.light[sf.light];
+        uint4 aux = Wv.aux[p];
+        float w = 1.0f;
+        if (!(aux.y & 2u)) {
+            // pdf with which next-event estimation from the previous vertex picks this point
+            float pl;
+            if (L.type == REF_QUAD) {
+                float cl = fabsf(dot(L.n, d));
+                pl = (h.x * h.x) / fmaxf(cl * L.area, 1e-30f);
+            } else {
+                f3 dc = L.p - o;
+                float sin2 = (L.radius * L.radius) / dot(dc, dc);
+                float cmax = sqrtf(fmaxf(1.0f - sin2, 0.0f));
+                pl = 1.0f / (2.0f * kPiF * fmaxf(1.0f - cmax, 1e-12f));
+            }
+            pl *= 1.0f / (float)S.nlight;
+            float pb = __uint_as_float(aux.z);
+            w = pb * pb / (pb * pb + pl * pl);
+        }
+        f4 Le = f4{planck_norm(l[0], L.T, L.lum), 0.0f, 0.0f, 0.0f};
+        if (S.n_lambda > 1)
+            Le = f4{Le.x, planck_norm(l[1], L.T, L.lum), planck_norm(l[2], L.T, L.lum), planck_norm(l[3], L.T, L.lum)};
+        add_xyz(Wv, p, to_xyz(S, l, T * Le * w));
+    }
+}
+
+// Lambert vertices: next-event estimation (one shadow ray) + cosine sampling.
+__global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
+                                                            int bounce) {
+    const int nxt = cur ^ 1;
+    const int n = Wv.counts[3 + Q_LAMBERT];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_LAMBERT], (unsigned long long)n);
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = idx < n;
+        int p = 0;
+        bool cont = false, shadow = false;
+        f3 sP = mk3(0, 0, 0), sD = sP, sC = sP;
+        float smax = 0.0f;
+        if (active) {
+            p = Wv.mq[Q_LAMBERT][idx];
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            float2 h = Wv.hit[p];
+            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
+            const ExtMat& M = S.mat[sf.mat];
+            float l[4];
+            wavelengths(S, Wv.acc[p].w, l);
+            const bool multi = S.n_lambda > 1;
+            f4 rho = f4{reflectance(M, l[0]), 0.0f, 0.0f, 0.0f};
+            if (multi) rho = f4{rho.x, reflectance(M, l[1]), reflectance(M, l[2]), reflectance(M, l[3])};
+            f4 T = ld4(Wv.T, p);
+            uint4 aux = Wv.aux[p];
+            uint32_t key = aux.x;
+            f3 P = sf.p + N * offset_scale(sf.p);
+            if (S.nlight > 0) {
+                float ul = rnd(key, (uint32_t)bounce, 3);
+                int li = min((int)(ul * (float)S.nlight), S.nlight - 1);
+                const ExtLight& L = S.light[li];
+                float u4 = rnd(key, (uint32_t)bounce, 4), u5 = rnd(key, (uint32_t)bounce, 5);
+                f3 wi;
+                float dist, pl;
+                bool ok = true;
+                if (L.type == REF_QUAD) {
+                    f3 y = L.p + L.e1 * u4 + L.e2 * u5;
+                    f3 v = y - P;
+                    float d2 = dot(v, v);
+                    dist = sqrtf(d2);
+                    wi = v * (1.0f / dist);
+                    float cl = fabsf(dot(L.n, wi));
+                    ok = cl > 1e-7f;
+                    pl = d2 / fmaxf(cl * L.area, 1e-30f);
+                } else {
+                    f3 dc = L.p - P;
+                    float dc2 = dot(dc, dc);
+                    float sin2 = (L.radius * L.radius) / dc2;
+                    ok = sin2 < 1.0f;
+                    float cmax = sqrtf(fmaxf(1.0f - sin2, 0.0f));
+                    float ct = 1.0f - u4 * (1.0f - cmax);
+                    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+                    float phi = 2.0f * kPiF * u5;
+                    f3 ax = dc * rsqrtf(dc2), tb, bb;
+                    onb(ax, tb, bb);
+                    wi = tb * (st * __cosf(phi)) + bb * (st * __sinf(phi)) + ax * ct;
+                    float th = 0.0f;
+                    ok = ok && isect_sphere(make_float4(L.p.x, L.p.y, L.p.z, L.radius), P, wi, 1e30f, th);
+                    dist = th;
+                    pl = 1.0f / (2.0f * kPiF * fmaxf(1.0f - cmax, 1e-12f));
+                }
+                pl *= 1.0f / (float)S.nlight;
+                float cs = dot(N, wi);
+                if (ok && cs > 0.0f) {
+                    float pb = cs * (1.0f / kPiF);
+                    float w = pl * pl / (pl * pl + pb * pb);
+                    f4 Le = f4{planck_norm(l[0], L.T, L.lum), 0.0f, 0.0f, 0.0f};
+                    if (multi)
+                        Le = f4{Le.x, planck_norm(l[1], L.T, L.lum), planck_norm(l[2], L.T, L.lum),
+                                planck_norm(l[3], L.T, L.lum)};
+                    float g = cs * (1.0f / kPiF) / pl * w;
+                    sC = to_xyz(S, l, T * rho * Le * g);
+                    sP = P;
+                    sD = wi;
+                    smax = dist * (1.0f - 1e-4f);
+                    shadow = true;
+                }
+            }
+            // cosine-weighted hemisphere (Malley's method)
+            float u0 = rnd(key, (uint32_t)bounce, 0), u1 = rnd(key, (uint32_t)bounce, 1);
+            float rr = sqrtf(u0), phi = 2.0f * kPiF * u1;
+            float z = sqrtf(fmaxf(1.0f - u0, 0.0f));
+            f3 tb, bb;
+            onb(N, tb, bb);
+            f3 wo = tb * (rr * __cosf(phi)) + bb * (rr * __sinf(phi)) + N * z;
+            T = T * rho;
+            if (bounce + 1 < F.maxDepth && z > 0.0f && roulette(S, key, bounce, T)) {
+                cont = true;
+                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
+                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
+                Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), 0u);
+            }
+        }
+        int slot = wave_append(&Wv.counts[2], shadow);
+        if (shadow) {
+            Wv.s_org[slot] = make_float4(sP.x, sP.y, sP.z, smax);
+            Wv.s_dir[slot] = make_float4(sD.x, sD.y, sD.z, __int_as_float(p));
+            Wv.s_xyz[slot] = make_float4(sC.x, sC.y, sC.z, 0.0f);
+        }
+        int ns = wave_append(&Wv.counts[nxt], cont);
+        if (cont) Wv.q[nxt][ns] = p;
+    }
+}
+
+// Smooth dielectric: Fresnel-weighted choice of reflection or refraction at the
+// hero wavelength; a dispersive interface keeps only the hero.
+__global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
+                                                               int bounce) {
+    const int nxt = cur ^ 1;
+    const int n = Wv.counts[3 + Q_DIELECTRIC];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_DIELECTRIC], (unsigned long long)n);
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = idx < n;
+        int p = 0;
+        bool cont = false;
+        if (active) {
+            p = Wv.mq[Q_DIELECTRIC][idx];
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            float2 h = Wv.hit[p];
+            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            const ExtMat& M = S.mat[sf.mat];
+            uint4 aux = Wv.aux[p];
+            uint32_t key = aux.x;
+            float l0 = Wv.acc[p].w;
+            f3 N = sf.n;
+            float ci = -dot(d, N);
+            bool entering = ci > 0.0f;
+            if (!entering) { N = -N; ci = -ci; }
+            float eta = ior(M, l0);
+            float ei = entering ? 1.0f : eta, et = entering ? eta : 1.0f;
+            float er = ei / et;
+            float s2t = er * er * fmaxf(1.0f - ci * ci, 0.0f);
+            float Fr = 1.0f, ct = 0.0f;
+            if (s2t < 1.0f) {
+                ct = sqrtf(1.0f - s2t);
+                Fr = fresnel_dielectric(ci, ct, ei, et);
+            }
+            f3 wo, P;
+            float os = offset_scale(sf.p);
+            if (rnd(key, (uint32_t)bounce, 2) < Fr) {
+                wo = d + N * (2.0f * ci);
+                P = sf.p + N * os;
+            } else {
+                wo = normalize(d * er + N * (er * ci - ct));
+                P = sf.p - N * os;
+            }
+            f4 T = ld4(Wv.T, p);
+            uint32_t fl = aux.y;
+            if (M.dispersive && S.n_lambda > 1 && !(fl & 1u)) {
+                T = f4{T.x * (float)S.n_lambda, 0.0f, 0.0f, 0.0f};
+                fl |= 1u;
+            }
+            if (bounce + 1 < F.maxDepth && roulette(S, key, bounce, T)) {
+                cont = true;
+                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
+                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
+                Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, 0u);
+            }
+        }
+        int ns = wave_append(&Wv.counts[nxt], cont);
+        if (cont) Wv.q[nxt][ns] = p;
+    }
+}
+
+// Smooth conductor: mirror reflection weighted by the complex-index Fresnel term.
+__global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
+                                                              int bounce) {
+    const int nxt = cur ^ 1;
+    const int n = Wv.counts[3 + Q_CONDUCTOR];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_CONDUCTOR], (unsigned long long)n);
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {
+        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool active = idx < n;
+        int p = 0;
+        bool cont = false;
+        if (active) {
+            p = Wv.mq[Q_CONDUCTOR][idx];
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            float2 h = Wv.hit[p];
+            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            const ExtMat& M = S.mat[sf.mat];
+            f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
+            float ci = -dot(d, N);
+            float Fr = fresnel_conductor(ci, M.p[0], M.p[1]);
+            f4 T = ld4(Wv.T, p) * Fr;
+            f3 wo = d + N * (2.0f * ci);
+            f3 P = sf.p + N * offset_scale(sf.p);
+            uint4 aux = Wv.aux[p];
+            if (bounce + 1 < F.maxDepth && roulette(S, aux.x, bounce, T)) {
+                cont = true;
+                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
+                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
+                Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, 0u);
+            }
+        }
+        int ns = wave_append(&Wv.counts[nxt], cont);
+        if (cont) Wv.q[nxt][ns] = p;
+    }
+}
+
+// Shadow rays: any hit before tmax over the scene and the SDFs; unoccluded
+// contributions are added to the path.
+template <int W, int STACK, typename SE>
+__global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv) {
+    __shared__ SE stack_mem[STACK * kBlk];
+    SE* stack = stack_mem + threadIdx.x;
+    const int n = Wv.counts[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
+    unsigned steps = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+        f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
+        float t = so.w;
+        uint32_t ref = kMiss;
+        bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack);
+        for (int s = 0; !blocked && s < S.nsdf; ++s)
+            blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
+        if (!blocked) add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
+    }
+    wave_add_u64(&Wv.stats[2], steps);
+}
+
+// Per pixel, in sample order: add each path's XYZ to the running sum.
+__global__ void __launch_bounds__(kBlk) k_ext_accum(ExtFrame F, ExtWave Wv) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned nans = 0;
+    if (j < F.pc) {
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        if (F.k0 != 0) { ax = Wv.sum[j]; ay = Wv.sum[F.pc + j]; az = Wv.sum[2 * F.pc + j]; }
+        for (int s = 0; s < F.m; ++s) {
+            float4 a = Wv.acc[s * F.pc + j];
+            if (a.x != a.x || a.y != a.y || a.z != a.z) { ++nans; continue; }
+            ax += a.x; ay += a.y; az += a.z;
+        }
+        Wv.sum[j] = ax; Wv.sum[F.pc + j] = ay; Wv.sum[2 * F.pc + j] = az;
+    }
+    wave_add_u64(&Wv.stats[4], nans);
+}
+
+struct ExtFinish {
+    int p0, pc, spf, currentSamples, frame, ISO;
+    float apSize, FPS, persistence;
+};
+__global__ void __launch_bounds__(kBlk) k_ext_finish(ExtFinish F, const float* sum, float4* texels) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= F.pc) return;
+    float e = F.apSize * F.apSize * (float)F.ISO / (float)F.spf;
+    f3 out = mk3(sum[j], sum[F.pc + j], sum[2 * F.pc + j]) * e;
+    float4 tx = texels[F.p0 + j];
+    f3 in = mk3(tx.x, tx.y, tx.z);
+    if (F.currentSamples == F.spf && F.frame > F.spf) {
+        float wgt = ptm_pow(2.0f, -8.0f / (F.FPS * F.persistence));
+        out = (1.0f - wgt) * out + wgt * in;
+    } else {
+        int u = F.currentSamples / F.spf;
+        out = ((float)(u - 1) * in + out) / (float)u;
+    }
+    texels[F.p0 + j] = make_float4(out.x, out.y, out.z, 1.0f);
+}
+
+// ---------------------------------------------------------------------------------
+// Host side.
+struct ExtSceneHost {
+    DevAlloc mem;
+    ExtSceneDev dev{};
+    int stack_need = 0;
+    int width = 4;
+    bool small_nodes = true;  // every BVH has < 65536 nodes: 16-bit stack entries
+    std::vector<int> sdf_kinds;  // kinds present, one march launch per kind
+};
+
+struct ExtCtxState {
+    DevAlloc mem;
+    ExtWave wave{};
+    int cu = 0;  // compute units of the context's device (grid sizing)
+};
+
+static inline f3 v3(const float* a) { return mk3(a[0], a[1], a[2]); }
+
+static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
+    const pt::SceneModel& m = s->model;
+    if (!m.lenses.empty() || !m.cyclides.empty()) {
+        ctx->err = "extended mode has no lenses or cyclides (reference mode only)";
+        return PT_ERR_STATE;
+    }
+    std::vector<float4> sph, quad;
+    std::vector<int2> sph_ml, quad_ml, plane_ml;
+    std::vector<float> plane_y;
+    std::vector<ExtLight> lights;
+    auto light_of = [&](int lightID, int type, int prim) -> int {
+        if (lightID <= 0) return -1;
+        if ((size_t)lightID > m.lights.size()) return -2;
+        ExtLight L{};
+        L.type = type;
+        L.prim = prim;
+        L.T = m.lights[(size_t)lightID - 1].emission[0];
+        L.lum = m.lights[(size_t)lightID - 1].emission[1];
+        lights.push_back(L);
+        return (int)lights.size() - 1;
+    };
+    for (const pt::RSphere& o : m.spheres) {
+        int li = light_of(o.lightID, REF_SPHERE, (int)sph.size());
+        if (li == -2) { ctx->err = "sphere lightID out of range"; return PT_ERR_INVALID_ARG; }
+        if (li >= 0) { lights[(size_t)li].p = v3(o.pos); lights[(size_t)li].radius = o.radius; }
+        sph.push_back(make_float4(o.pos[0], o.pos[1], o.pos[2], o.radius));
+        sph_ml.push_back(make_int2(o.materialID - 1, li));
+    }
+    std::vector<pt::Aabb> qbox;
+    auto add_quad = [&](f3 Q, f3 e1, f3 e2, int materialID, int lightID) -> int {
+        f3 nn = cross(e1, e2);
+        float nn2 = dot(nn, nn);
+        if (!(nn2 > 0.0f)) return PT_OK;  // degenerate: skip
+        f3 w = nn * (1.0f / nn2);
+        f3 A = cross(e2, w), B = cross(w, e1);
+        float area = sqrtf(nn2);
+        int idx = (int)quad_ml.size();
+        int li = light_of(lightID, REF_QUAD, idx);
+        if (li == -2) { ctx->err = "quad lightID out of range"; return PT_ERR_INVALID_ARG; }
+        if (li >= 0) {
+            ExtLight& L = lights[(size_t)li];
+            L.p = Q; L.e1 = e1; L.e2 = e2; L.n = nn * (1.0f / area); L.area = area;
+        }
+        quad.push_back(make_float4(Q.x, Q.y, Q.z, dot(w, Q)));
+        quad.push_back(make_float4(w.x, w.y, w.z, area));
+        quad.push_back(make_float4(A.x, A.y, A.z, 0.0f));
+        quad.push_back(make_float4(B.x, B.y, B.z, 0.0f));
+        quad_ml.push_back(make_int2(materialID - 1, li));
+        pt::Aabb b;
+        f3 c[4] = {Q, Q + e1, Q + e2, Q + e1 + e2};
+        for (int k = 0; k < 3; ++k) {
+            b.lo[k] = 1e30f; b.hi[k] = -1e30f;
+            for (int v = 0; v < 4; ++v) { b.lo[k] = std::min(b.lo[k], comp(c[v], k)); b.hi[k] = std::max(b.hi[k], comp(c[v], k)); }
+            float pad = 1e-6f * std::max(1.0f, std::fabs(b.hi[k]));
+            b.lo[k] -= pad; b.hi[k] += pad;
+        }
+        qbox.push_back(b);
+        return PT_OK;
+    };
+    for (const pt::Quad& q : m.quads) {
+        int st = add_quad(v3(q.corner), v3(q.e1), v3(q.e2), q.materialID, q.lightID);
+        if (st != PT_OK) return st;
+    }
+    for (const pt::RBox& bx : m.boxes) {
+        // A rotated box as six outward-facing parallelograms (normal = e1 x e2);
+        // the box frame is the reference's RotationMatrix (src/shader.comp:337-364).
+        m3 M = rotation_deg(v3(bx.rotation));
+        f3 hs = v3(bx.size) * 0.5f, c = v3(bx.pos);
+        f3 ax = mvmul(M, mk3(1, 0, 0)), ay = mvmul(M, mk3(0, 1, 0)), az = mvmul(M, mk3(0, 0, 1));
+        f3 X = ax * (2.0f * hs.x), Y = ay * (2.0f * hs.y), Z = az * (2.0f * hs.z);
+        f3 o = c - ax * hs.x - ay * hs.y - az * hs.z;
+        f3 o2 = o + X + Y + Z;
+        f3 faces[6][3] = {{o, Z, Y}, {o, X, Z}, {o, Y, X}, {o2, -Y, -Z}, {o2, -Z, -X}, {o2, -X, -Y}};
+        for (auto& f : faces) {
+            int st = add_quad(f[0], f[1], f[2], bx.materialID, bx.lightID);
+            if (st != PT_OK) return st;
+        }
+    }
+    for (const pt::RPlane& pl : m.planes) {
+        if (pl.lightID > 0) {
+            ctx->err = "extended mode cannot sample an emissive infinite plane";
+            return PT_ERR_STATE;
+        }
+        plane_y.push_back(pl.pos[1]);
+        plane_ml.push_back(make_int2(pl.materialID - 1, -1));
+    }
+    std::vector<ExtSdf> sdfs;
+    std::vector<int> kinds;
+    for (const pt::RSdf& sd : m.sdfs) {
+        ExtSdf e{};
+        e.kind = sd.desc.kind;
+        e.iters = sd.desc.iterations;
+        e.bailout = sd.desc.bailout > 0.0f ? sd.desc.bailout : 16.0f;
+        e.mat = (int)std::floor(sd.desc.material[0]);
+        e.c = v3(sd.pos);
+        e.lo = v3(sd.pos) - v3(sd.size) * 0.5f;
+        e.hi = v3(sd.pos) + v3(sd.size) * 0.5f;
+        sdfs.push_back(e);
+        if (std::find(kinds.begin(), kinds.end(), e.kind) == kinds.end()) kinds.push_back(e.kind);
+    }
+    std::vector<ExtMat> mats;
+    for (const pt::Material& mm : m.materials) {
+        ExtMat e{};
+        e.type = mm.type;
+        switch (mm.type) {
+            case pt::MAT_LAMBERT_LOBE:
+                e.p[0] = mm.reflection[0]; e.p[1] = mm.reflection[1]; e.p[2] = mm.reflection[2];
+                break;
+            case pt::MAT_LAMBERT_GAUSS:
+                e.n = mm.n_gauss;
+                for (int k = 0; k < 9; ++k) e.p[k] = mm.gauss[k];
+                break;
+            case pt::MAT_DIELECTRIC:
+                e.n = mm.dispersion;
+                for (int k = 0; k < 6; ++k) e.p[k] = mm.disp[k];
+                e.dispersive = mm.dispersion == pt::DISP_SELLMEIER || mm.disp[1] != 0.0f;
+                break;
+            default:
+                e.p[0] = mm.cond_n; e.p[1] = mm.cond_k;
+                break;
+        }
+        mats.push_back(e);
+    }
+    auto bad_mat = [&](int mi) { return mi < 0 || mi >= (int)mats.size(); };
+    for (auto& ml : sph_ml) if (bad_mat(ml.x)) { ctx->err = "sphere materialID out of range"; return PT_ERR_INVALID_ARG; }
+    for (auto& ml : quad_ml) if (bad_mat(ml.x)) { ctx->err = "quad/box materialID out of range"; return PT_ERR_INVALID_ARG; }
+    for (auto& ml : plane_ml) if (bad_mat(ml.x)) { ctx->err = "plane materialID out of range"; return PT_ERR_INVALID_ARG; }
+    for (auto& e : sdfs) if (bad_mat(e.mat)) { ctx->err = "sdf material out of range"; return PT_ERR_INVALID_ARG; }
+
+    const int width = ctx->cfg.bvh_width ? ctx->cfg.bvh_width : 4;
+    // Sphere BVH with leaf-ordered sphere records.
+    std::vector<pt::Aabb> sbox(sph.size());
+    std::vector<uint32_t> sref(sph.size());
+    for (size_t i = 0; i < sph.size(); ++i) {
+        for (int k = 0; k < 3; ++k) {
+            float c = k == 0 ? sph[i].x : (k == 1 ? sph[i].y : sph[i].z);
+            sbox[i].lo[k] = c - sph[i].w;
+            sbox[i].hi[k] = c + sph[i].w;
+        }
+        sref[i] = (uint32_t)i;
+    }
+    pt::WideBvh sb = pt::build_wide_bvh(sbox, sref, width, 4);
+    std::vector<float4> sleaf(sb.prim_refs.size());
+    std::vector<int> sids(sb.prim_refs.size());
+    for (size_t k = 0; k < sb.prim_refs.size(); ++k) {
+        sleaf[k] = sph[sb.prim_refs[k]];
+        sids[k] = (int)sb.prim_refs[k];
+    }
+    const bool quad_bvh = quad_ml.size() > (size_t)kQuadBrute;
+    pt::WideBvh qb;
+    std::vector<float4> qleaf;
+    std::vector<int> qids;
+    if (quad_bvh) {
+        std::vector<uint32_t> qref(quad_ml.size());
+        for (size_t i = 0; i < qref.size(); ++i) qref[i] = (uint32_t)i;
+        qb = pt::build_wide_bvh(qbox, qref, width, 4);
+        for (uint32_t r : qb.prim_refs) {
+            for (int v = 0; v < 4; ++v) qleaf.push_back(quad[4 * (size_t)r + (size_t)v]);
+            qids.push_back((int)r);
+        }
+    }
+    int stack_need = std::max(sb.max_stack, quad_bvh ? qb.max_stack : 0);
+    if (stack_need > 64) {
+        ctx->err = "BVH too deep for the 64-entry traversal stack";
+        return PT_ERR_CAPACITY;
+    }
+    h->stack_need = stack_need;
+    h->width = width;
+    h->small_nodes = sb.n_nodes < 65536 && (!quad_bvh || qb.n_nodes < 65536);
+    h->sdf_kinds = kinds;
+
+    size_t bytes = 0;
+    auto sz = [&](size_t b) { bytes += ((b + 255) / 256) * 256; };
+    sz(sph.size() * 16); sz(sph_ml.size() * 8); sz(quad.size() * 16); sz(quad_ml.size() * 8);
+    sz(plane_y.size() * 4); sz(plane_ml.size() * 8); sz(sdfs.size() * sizeof(ExtSdf)); sz(mats.size() * sizeof(ExtMat));
+    sz(lights.size() * sizeof(ExtLight));
+    sz(sb.words.size() * 4); sz(sleaf.size() * 16); sz(sids.size() * 4);
+    sz(qb.words.size() * 4); sz(qleaf.size() * 16); sz(qids.size() * 4);
+    int st = dev_alloc(ctx, h->mem, bytes);
+    if (st != PT_OK) return st;
+    char* p = static_cast<char*>(h->mem.p);
+    bool fail = false;
+    auto put = [&](const void* src, size_t b) -> void* {
+        void* dst = p;
+        if (b && hipMemcpy(dst, src, b, hipMemcpyHostToDevice) != hipSuccess) fail = true;
+        p += ((b + 255) / 256) * 256;
+        return dst;
+    };
+    ExtSceneDev& D = h->dev;
+    D.sph = (const float4*)put(sph.data(), sph.size() * 16);
+    D.sph_ml = (const int2*)put(sph_ml.data(), sph_ml.size() * 8);
+    D.quad = (const float4*)put(quad.data(), quad.size() * 16);
+    D.quad_ml = (const int2*)put(quad_ml.data(), quad_ml.size() * 8);
+    D.plane_y = (const float*)put(plane_y.data(), plane_y.size() * 4);
+    D.plane_ml = (const int2*)put(plane_ml.data(), plane_ml.size() * 8);
+    D.sdf = (const ExtSdf*)put(sdfs.data(), sdfs.size() * sizeof(ExtSdf));
+    D.mat = (const ExtMat*)put(mats.data(), mats.size() * sizeof(ExtMat));
+    D.light = (const ExtLight*)put(lights.data(), lights.size() * sizeof(ExtLight));
+    D.sbvh.nodes = (const uint32_t*)put(sb.words.data(), sb.words.size() * 4);
+    D.sbvh.prims = (const float4*)put(sleaf.data(), sleaf.size() * 16);
+    D.sbvh.ids = (const int*)put(sids.data(), sids.size() * 4);
+    D.sbvh.count = (int)sleaf.size();
+    D.qbvh.nodes = (const uint32_t*)put(qb.words.data(), qb.words.size() * 4);
+    D.qbvh.prims = (const float4*)put(qleaf.data(), qleaf.size() * 16);
+    D.qbvh.ids = (const int*)put(qids.data(), qids.size() * 4);
+    D.qbvh.count = (int)qids.size();
+    if (fail) return hip_fail(ctx, hipGetLastError(), "extended scene upload");
+    D.nsph = (int)sph.size();
+    D.nquad = (int)quad_ml.size();
+    D.quad_bvh = quad_bvh ? 1 : 0;
+    D.nplane = (int)plane_y.size();
+    D.nsdf = (int)sdfs.size();
+    D.nmat = (int)mats.size();
+    D.nlight = (int)lights.size();
+    D.has_env = m.env_enabled ? 1 : 0;
+    D.env = m.env_radiance;
+    D.n_lambda = m.render.n_lambda;
+    D.lmin = m.render.lambda_min;
+    D.lrange = m.render.lambda_max - m.render.lambda_min;
+    D.rr_depth = m.render.rr_depth;
+    D.seed = m.render.seed;
+    D.camera = m.render.camera;
+    D.sdf_steps_max = ctx->cfg.sdf_max_steps > 0 ? ctx->cfg.sdf_max_steps : 256;
+    s->device_bytes += (int64_t)bytes;
+    return PT_OK;
+}
+
+int ext_scene_build(pt_ctx* ctx, pt_scene* s) {
+    if (s->ext) return PT_OK;
+    ExtSceneHost* h = new ExtSceneHost();
+    int st = upload_ext(ctx, s, h);
+    if (st != PT_OK) {
+        dev_free(ctx, h->mem);
+        delete h;
+        return st;
+    }
+    s->ext = h;
+    return PT_OK;
+}
+
+void ext_scene_release(pt_ctx* ctx, pt_scene* s) {
+    if (!s->ext) return;
+    ExtSceneHost* h = static_cast<ExtSceneHost*>(s->ext);
+    dev_free(ctx, h->mem);
+    delete h;
+    s->ext = nullptr;
+}
+
+template <typename T>
+static T* carve_e(char*& p, size_t n) {
+    T* r = reinterpret_cast<T*>(p);
+    p += ((n * sizeof(T) + 255) / 256) * 256;
+    return r;
+}
+
+static int ensure_ext_wave(pt_ctx* ctx, int cap) {
+    ExtCtxState* st = static_cast<ExtCtxState*>(ctx->ext_state);
+    if (st && st->wave.cap >= cap) return PT_OK;
+    if (!st) {
+        st = new ExtCtxState();
+        ctx->ext_state = st;
+    }
+    dev_free(ctx, st->mem);
+    size_t n = (size_t)cap;
+    // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
+    // shadow org/dir/xyz 48, sums 12
+    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 12;
+    int rc = dev_alloc(ctx, st->mem, n * per_path + 32 * 256 + 4096);
+    if (rc != PT_OK) return rc;
+    char* p = static_cast<char*>(st->mem.p);
+    ExtWave& W = st->wave;
+    W.org = carve_e<float4>(p, n); W.dir = carve_e<float4>(p, n);
+    W.T = carve_e<float4>(p, n); W.acc = carve_e<float4>(p, n);
+    W.aux = carve_e<uint4>(p, n); W.hit = carve_e<float2>(p, n);
+    W.q[0] = carve_e<int>(p, n); W.q[1] = carve_e<int>(p, n);
+    for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
+    W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
+    W.sum = carve_e<float>(p, 3 * n);
+    W.counts = carve_e<int>(p, 16);
+    W.stats = ctx->d_stats;
+    W.cap = cap;
+    return PT_OK;
+}
+
+void ext_wave_release(pt_ctx* ctx) {
+    ExtCtxState* st = static_cast<ExtCtxState*>(ctx->ext_state);
+    if (!st) return;
+    dev_free(ctx, st->mem);
+    delete st;
+    ctx->ext_state = nullptr;
+}
+
+template <int KIND>
+static void launch_march(pt_ctx* ctx, const char* name, const ExtSceneDev& S, ExtWave& Wv, int cur, int grid) {
+    launch(ctx, name, k_ext_march<KIND>, dim3(grid), dim3(kBlk), S, Wv, cur);
+}
+
+template <int W, int STACK, typename SE>
+static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
+                          int grid) {
+    const ExtSceneDev& S = H.dev;
+    launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, cur ^ 1);
+    if (S.nsdf > 0) {
+        launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false>, dim3(grid), dim3(kBlk), S, Wv, cur);
+        for (int kind : H.sdf_kinds) {
+            switch (kind) {
+                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB>(ctx, "k_ext_march_mandelbulb", S, Wv, cur, grid); break;
+                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER>(ctx, "k_ext_march_menger", S, Wv, cur, grid); break;
+                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB>(ctx, "k_ext_march_blob", S, Wv, cur, grid); break;
+                default: launch_march<PT_SDF_TERRAIN>(ctx, "k_ext_march_terrain", S, Wv, cur, grid); break;
+            }
+        }
+        launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
+    } else {
+        launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, true>, dim3(grid), dim3(kBlk), S, Wv, cur);
+    }
+    launch(ctx, "k_ext_shade_miss", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_MISS);
+    launch(ctx, "k_ext_shade_emit", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_EMIT);
+    launch(ctx, "k_ext_shade_lambert", k_ext_shade_lambert, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
+    launch(ctx, "k_ext_shade_dielectric", k_ext_shade_dielectric, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
+    launch(ctx, "k_ext_shade_conductor", k_ext_shade_conductor, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
+    launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
+}
+
+template <int W>
+static void launch_bounce_w(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
+                            int grid) {
+    if (H.small_nodes) {
+        if (H.stack_need <= 32) launch_bounce<W, 32, uint16_t>(ctx, H, F, Wv, cur, b, grid);
+        else launch_bounce<W, 64, uint16_t>(ctx, H, F, Wv, cur, b, grid);
+    } else {
+        if (H.stack_need <= 32) launch_bounce<W, 32, int>(ctx, H, F, Wv, cur, b, grid);
+        else launch_bounce<W, 64, int>(ctx, H, F, Wv, cur, b, grid);
+    }
+}
+
+int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+    int st = ext_scene_build(ctx, const_cast<pt_scene*>(s));
+    if (st != PT_OK) return st;
+    const ExtSceneHost& H = *static_cast<const ExtSceneHost*>(s->ext);
+    const int Wd = pc->resolution[0], Ht = pc->resolution[1], P = Wd * Ht;
+    int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
+    cap = std::max(cap, 1024);
+    st = ensure_ext_wave(ctx, cap);
+    if (st != PT_OK) return st;
+    ExtWave& Wv = static_cast<ExtCtxState*>(ctx->ext_state)->wave;
+
+    ExtFrame F{};
+    F.W = Wd; F.H = Ht; F.frame = pc->frame; F.spf = pc->samplesPerFrame; F.maxDepth = pc->pathLength;
+    F.camSize = pc->cameraSize; F.apSize = pc->apertureSize; F.apDist = pc->apertureDist;
+    F.camPos = mk3(pc->cameraPosX, pc->cameraPosY, pc->cameraPosZ);
+    F.camM = rotation_deg(mk3(pc->cameraAngle[0], pc->cameraAngle[1], 0.0f));
+    F.fwd = mk3(F.camM.c0.z, F.camM.c1.z, F.camM.c2.z);
+    F.lensPos = F.camPos + F.fwd * pc->lensDistance;
+    F.lensM = rotation_deg(mk3(0.0f, 90.0f - pc->cameraAngle[1], pc->cameraAngle[0]));
+    float r = pc->lensRadius, f = pc->lensFocalLength;
+    F.lensHalf = 2.0f * f - sqrtf(4.0f * f * f - r * r);
+    F.lensSlicePos = 0.5f * pc->lensThickness + F.lensHalf;
+    F.lensTwoF = 2.0f * f;
+
+    ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
+    if (cs->cu == 0) {
+        hipDeviceProp_t prop;
+        cs->cu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+                     ? prop.multiProcessorCount
+                     : 256;
+    }
+    const int grid_cap = cs->cu * 8;
+    const int pcmax = std::min(P, cap);
+    for (int p0 = 0; p0 < P; p0 += pcmax) {
+        const int pcur = std::min(pcmax, P - p0);
+        const int m = std::max(1, std::min(pc->samplesPerFrame, cap / pcur));
+        for (int k0 = 0; k0 < pc->samplesPerFrame; k0 += m) {
+            F.p0 = p0; F.pc = pcur; F.k0 = k0; F.m = std::min(m, pc->samplesPerFrame - k0);
+            const int npaths = F.pc * F.m;
+            const int grid = std::min((npaths + kBlk - 1) / kBlk, grid_cap);
+            launch(ctx, "k_ext_counts", k_ext_counts, dim3(1), dim3(64), Wv.counts, npaths);
+            launch(ctx, "k_ext_raygen", k_ext_raygen, dim3(grid), dim3(kBlk), H.dev, F, Wv);
+            for (int b = 0; b < pc->pathLength; ++b) {
+                const int cur = b & 1;
+                switch (H.width) {
+                    case 2: launch_bounce_w<2>(ctx, H, F, Wv, cur, b, grid); break;
+                    case 8: launch_bounce_w<8>(ctx, H, F, Wv, cur, b, grid); break;
+                    default: launch_bounce_w<4>(ctx, H, F, Wv, cur, b, grid); break;
+                }
+            }
+            launch(ctx, "k_ext_accum", k_ext_accum, dim3((F.pc + kBlk - 1) / kBlk), dim3(kBlk), F, Wv);
+        }
+        ExtFinish Fin{};
+        Fin.p0 = p0; Fin.pc = pcur; Fin.spf = pc->samplesPerFrame; Fin.currentSamples = pc->currentSamples;
+        Fin.frame = pc->frame; Fin.ISO = pc->ISO; Fin.apSize = pc->apertureSize; Fin.FPS = pc->FPS;
+        Fin.persistence = pc->persistence;
+        launch(ctx, "k_ext_finish", k_ext_finish, dim3((pcur + kBlk - 1) / kBlk), dim3(kBlk), Fin, (const float*)Wv.sum,
+               reinterpret_cast<float4*>(acc->d_texels));
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "extended dispatch launch");
+    return PT_OK;
+}
 
 }  // namespace ptk

@@ -1133,12 +1133,12 @@ static RefFrame make_frame(const pt_push_constants* pc) {
 }
 
 template <int KIND>
-static void launch_march(const RefSceneDev& S, RefWave& W, int cur, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_ref_march<KIND>, dim3(grid), dim3(kBlock), 0, st, S, W, cur);
+static void launch_march(pt_ctx* ctx, const RefSceneDev& S, RefWave& W, int cur, int grid) {
+    launch(ctx, "k_ref_march", k_ref_march<KIND>, dim3(grid), dim3(kBlock), S, W, cur);
 }
 template <int KIND>
-static void launch_shadow_march(const RefSceneDev& S, RefWave& W, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_ref_shadow_march<KIND>, dim3(grid), dim3(kBlock), 0, st, S, W);
+static void launch_shadow_march(pt_ctx* ctx, const RefSceneDev& S, RefWave& W, int grid) {
+    launch(ctx, "k_ref_shadow_march", k_ref_shadow_march<KIND>, dim3(grid), dim3(kBlock), S, W);
 }
 
 int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
@@ -1150,7 +1150,6 @@ int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
     int st = ensure_ref_wave(ctx, cap);
     if (st != PT_OK) return st;
     RefWave& Wv = ctx->ref_wave;
-    hipStream_t stream = ctx->stream;
     RefFrame F = make_frame(pc);
     const int pcmax = std::min(P, cap);
     for (int p0 = 0; p0 < P; p0 += pcmax) {
@@ -1163,34 +1162,34 @@ int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
             F.m = std::min(m, pc->samplesPerFrame - k0);
             const int npaths = F.pc * F.m;
             const int grid = (npaths + kBlock - 1) / kBlock;
-            hipLaunchKernelGGL(k_ref_counts, dim3(1), dim3(1), 0, stream, Wv.counts, npaths, 0, 0);
-            hipLaunchKernelGGL(k_ref_raygen, dim3(grid), dim3(kBlock), 0, stream, F, Wv);
+            launch(ctx, "k_ref_counts", k_ref_counts, dim3(1), dim3(1), Wv.counts, npaths, 0, 0);
+            launch(ctx, "k_ref_raygen", k_ref_raygen, dim3(grid), dim3(kBlock), F, Wv);
             for (int b = 0; b < pc->pathLength; ++b) {
                 const int cur = b & 1;
-                hipLaunchKernelGGL(k_ref_reset, dim3(1), dim3(1), 0, stream, Wv.counts, cur ^ 1);
-                hipLaunchKernelGGL(k_ref_intersect, dim3(grid), dim3(kBlock), 0, stream, S, Wv, cur);
+                launch(ctx, "k_ref_reset", k_ref_reset, dim3(1), dim3(1), Wv.counts, cur ^ 1);
+                launch(ctx, "k_ref_intersect", k_ref_intersect, dim3(grid), dim3(kBlock), S, Wv, cur);
                 if (S.nsdf > 0) {
                     switch (S.sdf_kind) {
-                        case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB>(S, Wv, cur, grid, stream); break;
-                        case PT_SDF_MENGER: launch_march<PT_SDF_MENGER>(S, Wv, cur, grid, stream); break;
-                        case PT_SDF_BLOB: launch_march<PT_SDF_BLOB>(S, Wv, cur, grid, stream); break;
-                        case PT_SDF_TERRAIN: launch_march<PT_SDF_TERRAIN>(S, Wv, cur, grid, stream); break;
-                        default: launch_march<0>(S, Wv, cur, grid, stream); break;
+                        case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB>(ctx, S, Wv, cur, grid); break;
+                        case PT_SDF_MENGER: launch_march<PT_SDF_MENGER>(ctx, S, Wv, cur, grid); break;
+                        case PT_SDF_BLOB: launch_march<PT_SDF_BLOB>(ctx, S, Wv, cur, grid); break;
+                        case PT_SDF_TERRAIN: launch_march<PT_SDF_TERRAIN>(ctx, S, Wv, cur, grid); break;
+                        default: launch_march<0>(ctx, S, Wv, cur, grid); break;
                     }
                 }
-                hipLaunchKernelGGL(k_ref_shade, dim3(grid), dim3(kBlock), 0, stream, S, F, Wv, cur, b);
-                hipLaunchKernelGGL(k_ref_shadow, dim3(grid), dim3(kBlock), 0, stream, S, Wv, S.nsdf > 0 ? 0 : 1);
+                launch(ctx, "k_ref_shade", k_ref_shade, dim3(grid), dim3(kBlock), S, F, Wv, cur, b);
+                launch(ctx, "k_ref_shadow", k_ref_shadow, dim3(grid), dim3(kBlock), S, Wv, S.nsdf > 0 ? 0 : 1);
                 if (S.nsdf > 0) {
                     switch (S.sdf_kind) {
-                        case PT_SDF_MANDELBULB: launch_shadow_march<PT_SDF_MANDELBULB>(S, Wv, grid, stream); break;
-                        case PT_SDF_MENGER: launch_shadow_march<PT_SDF_MENGER>(S, Wv, grid, stream); break;
-                        case PT_SDF_BLOB: launch_shadow_march<PT_SDF_BLOB>(S, Wv, grid, stream); break;
-                        case PT_SDF_TERRAIN: launch_shadow_march<PT_SDF_TERRAIN>(S, Wv, grid, stream); break;
-                        default: launch_shadow_march<0>(S, Wv, grid, stream); break;
+                        case PT_SDF_MANDELBULB: launch_shadow_march<PT_SDF_MANDELBULB>(ctx, S, Wv, grid); break;
+                        case PT_SDF_MENGER: launch_shadow_march<PT_SDF_MENGER>(ctx, S, Wv, grid); break;
+                        case PT_SDF_BLOB: launch_shadow_march<PT_SDF_BLOB>(ctx, S, Wv, grid); break;
+                        case PT_SDF_TERRAIN: launch_shadow_march<PT_SDF_TERRAIN>(ctx, S, Wv, grid); break;
+                        default: launch_shadow_march<0>(ctx, S, Wv, grid); break;
                     }
                 }
             }
-            hipLaunchKernelGGL(k_ref_accum, dim3((F.pc + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, S, F, Wv);
+            launch(ctx, "k_ref_accum", k_ref_accum, dim3((F.pc + kBlock - 1) / kBlock), dim3(kBlock), S, F, Wv);
         }
         RefFinish Fin{};
         Fin.p0 = p0;
@@ -1202,8 +1201,8 @@ int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
         Fin.apSize = pc->apertureSize;
         Fin.FPS = pc->FPS;
         Fin.persistence = pc->persistence;
-        hipLaunchKernelGGL(k_ref_finish, dim3((pcur + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, Fin, Wv.sum,
-                           reinterpret_cast<float4*>(acc->d_texels));
+        launch(ctx, "k_ref_finish", k_ref_finish, dim3((pcur + kBlock - 1) / kBlock), dim3(kBlock), Fin,
+               (const float*)Wv.sum, reinterpret_cast<float4*>(acc->d_texels));
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "reference dispatch launch");

@@ -48,8 +48,8 @@ int resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb) {
     DevAlloc buf;
     int st = dev_alloc(ctx, buf, (size_t)n * 3);
     if (st != PT_OK) return st;
-    hipLaunchKernelGGL(k_resolve, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
-                       reinterpret_cast<const float4*>(acc->d_texels), n, tonemap, static_cast<uint8_t*>(buf.p));
+    launch(ctx, "k_resolve", k_resolve, dim3((n + 255) / 256), dim3(256), reinterpret_cast<const float4*>(acc->d_texels),
+           n, tonemap, static_cast<uint8_t*>(buf.p));
     hipError_t e = hipMemcpyAsync(host_rgb, buf.p, (size_t)n * 3, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     dev_free(ctx, buf);
@@ -90,8 +90,8 @@ int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float*
     hipError_t e = hipMemcpyAsync(dx, x, n * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess && y) e = hipMemcpyAsync(dy, y, n * 4, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, fn, dx, dy, dout,
-                           (int)n);
+        launch(ctx, "k_math", k_math, dim3((unsigned)((n + 255) / 256)), dim3(256), fn, (const float*)dx,
+               (const float*)dy, dout, (int)n);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * 4, hipMemcpyDeviceToHost, ctx->stream);
@@ -103,8 +103,28 @@ int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float*
 
 __global__ void k_bump_dispatch(unsigned long long* stats) { stats[5] += 1ull; }
 
+int prof_kernel_id(pt_ctx* ctx, const char* name) {
+    for (size_t i = 0; i < ctx->prof_totals.size(); ++i)
+        if (ctx->prof_totals[i].name == name) return (int)i;
+    ProfTotal t;
+    t.name = name;
+    ctx->prof_totals.push_back(t);
+    return (int)ctx->prof_totals.size() - 1;
+}
+
+hipEvent_t prof_event(pt_ctx* ctx) {
+    if (!ctx->prof_pool.empty()) {
+        hipEvent_t e = ctx->prof_pool.back();
+        ctx->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
 int count_dispatch(pt_ctx* ctx) {
-    hipLaunchKernelGGL(k_bump_dispatch, dim3(1), dim3(1), 0, ctx->stream, ctx->d_stats);
+    launch(ctx, "k_bump_dispatch", k_bump_dispatch, dim3(1), dim3(1), ctx->d_stats);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "dispatch counter");
     return PT_OK;

@@ -101,13 +101,28 @@ struct DevAlloc {
     size_t bytes = 0;
 };
 
+struct ProfPending {
+    int kernel;
+    hipEvent_t start, stop;
+};
+struct ProfTotal {
+    std::string name;
+    uint64_t launches = 0;
+    double ms = 0.0;
+};
+
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     pt_config cfg{};
     std::string err;
     int64_t allocated = 0;
-    unsigned long long* d_stats = nullptr;  // 6 counters
+    unsigned long long* d_stats = nullptr;  // pt_counters fields, 16 slots
+    // kernel timing (pt_profile_enable)
+    bool prof = false;
+    std::vector<ProfPending> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<ProfTotal> prof_totals;
     // reference-mode wave
     DevAlloc ref_mem;
     ptk::RefWave ref_wave{};
@@ -160,5 +175,22 @@ void ext_wave_release(pt_ctx* ctx);
 int resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb);
 int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n);
 int count_dispatch(pt_ctx* ctx);
+
+// Kernel launch on the context's stream; with profiling on, bracketed by two
+// HIP events whose elapsed time is added to the kernel's total on read.
+int prof_kernel_id(pt_ctx* ctx, const char* name);
+hipEvent_t prof_event(pt_ctx* ctx);
+template <typename Kernel, typename... Args>
+inline void launch(pt_ctx* ctx, const char* name, Kernel kernel, dim3 grid, dim3 block, Args... args) {
+    if (!ctx->prof) {
+        hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+        return;
+    }
+    ProfPending pp{prof_kernel_id(ctx, name), prof_event(ctx), prof_event(ctx)};
+    (void)hipEventRecord(pp.start, ctx->stream);
+    hipLaunchKernelGGL(kernel, grid, block, 0, ctx->stream, args...);
+    (void)hipEventRecord(pp.stop, ctx->stream);
+    ctx->prof_pending.push_back(pp);
+}
 
 }  // namespace ptk

@@ -1,0 +1,170 @@
+"""Extended mode on the GPU: convergence to the FP64 oracle (oracle/ptext.c)
+within the Monte Carlo bound, closed-form radiometry, and determinism under
+re-batching.
+
+Statistical bar: on blocks of pixels, |GPU mean - oracle mean| <= 4 sigma for
+>= 99 % of blocks (sigma from the oracle's per-sample variance scaled to both
+sample counts), and the image means agree within 3 sigma."""
+import json
+
+import numpy as np
+import pytest
+
+import pathtracer_amd as pt
+from pathtracer_amd import scenes
+from oracle import ptext
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_render(ctx, text, W, H, spp, depth, spf=None, frames=None):
+    scene = ctx.scene_from_json(text)
+    spf = spf or spp
+    r = pt.Renderer(ctx, scene, W, H, spf=spf, path_length=depth, mode=pt.MODE_EXT)
+    for _ in range(frames or spp // spf):
+        r.render_frame()
+    return r.acc.read()[..., :3].astype(np.float64)
+
+
+def block(a, b):
+    h, w = a.shape[0] // b * b, a.shape[1] // b * b
+    return a[:h, :w].reshape(h // b, b, w // b, b, -1).mean(axis=(1, 3))
+
+
+def converge_check(gpu, gspp, ora, blk=4, z_max=4.0, frac=0.99):
+    mean_o, var_o = ora["mean"], ora["var_of_mean"]
+    ospp_ratio = 1.0  # var_of_mean is already per the oracle's sample count
+    # GPU variance of its mean, from the oracle's per-sample variance
+    var_g = var_o * (ora["spp"] / gspp)
+    gb, ob = block(gpu, blk), block(mean_o, blk)
+    vb = block(var_o + var_g, blk) / (blk * blk) * ospp_ratio
+    sig = np.sqrt(np.maximum(vb, 1e-30))
+    z = (gb - ob) / sig
+    ok = np.abs(z) <= z_max
+    live = vb > 1e-20
+    share = ok[live].mean() if live.any() else 1.0
+    # whole-image mean
+    n = gpu.size / 3
+    sig_img = np.sqrt((var_o + var_g).sum(axis=(0, 1))) / n
+    dimg = gpu.mean(axis=(0, 1)) - mean_o.mean(axis=(0, 1))
+    return share, dimg / np.maximum(sig_img, 1e-30), z
+
+
+@pytest.mark.parametrize("wavelengths", [1, 4])
+def test_cornell_converges_to_fp64_oracle(gpu_ctx, wavelengths):
+    text = scenes.cornell(wavelengths=wavelengths, seed=0)
+    W = H = 48
+    gspp, ospp = 2048, 512
+    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 8, spf=256)
+    ora = ptext.render(text, W, H, ospp, 8, seed=12345)
+    ora["spp"] = ospp
+    share, zimg, z = converge_check(gpu, gspp, ora)
+    print(f"\ncornell {wavelengths} wavelength(s): {share * 100:.2f}% of 4x4 blocks within 4 sigma, "
+          f"image-mean z = {np.round(zimg, 2)}")
+    assert share >= 0.99
+    assert np.all(np.abs(zimg) < 3.0)
+
+
+def test_random_spheres_converges_to_fp64_oracle(gpu_ctx):
+    text = scenes.random_spheres(n=4096, scene_seed=1, wavelengths=4, seed=0)
+    W, H = 48, 32
+    gspp, ospp = 2048, 512
+    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 12, spf=256)
+    ora = ptext.render(text, W, H, ospp, 12, seed=777)
+    ora["spp"] = ospp
+    share, zimg, _ = converge_check(gpu, gspp, ora)
+    print(f"\nspheres: {share * 100:.2f}% of blocks within 4 sigma, image-mean z = {np.round(zimg, 2)}")
+    assert share >= 0.99
+    assert np.all(np.abs(zimg) < 3.0)
+
+
+def test_sdf_scene_converges_to_fp64_oracle(gpu_ctx):
+    text = scenes.sdf_scene(wavelengths=4, seed=0)
+    W, H = 48, 27
+    gspp, ospp = 512, 256
+    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 8, spf=128)
+    ora = ptext.render(text, W, H, ospp, 8, seed=99)
+    ora["spp"] = ospp
+    share, zimg, _ = converge_check(gpu, gspp, ora, blk=3, z_max=5.0, frac=0.98)
+    print(f"\nsdf: {share * 100:.2f}% of blocks within 5 sigma, image-mean z = {np.round(zimg, 2)}")
+    assert share >= 0.98
+    assert np.all(np.abs(zimg) < 4.0)
+
+
+def _furnace(material, env=1.0, wavelengths=4):
+    return json.dumps({
+        "camera": {"numShots": 1, "position": [[0.0, 0.0, -3.0]], "angle": [[0.0, 0.0]], "ISO": 1, "size": 0.057,
+                   "apertureSize": 1.0, "apertureDistance": 0.1, "lensRadius": 0.01, "lensFocalLength": 0.03,
+                   "lensThickness": 0.0, "lensDistance": 0.05, "model": "pinhole"},
+        "sphere": [{"position": [0.0, 0.0, 0.0], "radius": 1.0, "materialID": 1, "lightID": 0}],
+        "material": [material],
+        "light": [],
+        "environment": {"radiance": env},
+        "render": {"wavelengths": wavelengths, "russianRouletteDepth": 64, "seed": 3},
+    })
+
+
+def flat_xyz(ctx, env=1.0):
+    """XYZ of the bare uniform environment (no object in view)."""
+    t = json.loads(_furnace({"type": "lambert", "gaussians": [[0.5, 550.0, 1e7]]}, env))
+    t["sphere"] = []
+    return gpu_render(ctx, json.dumps(t), 16, 16, 256, 4).mean(axis=(0, 1))
+
+
+def test_white_furnace_lambert(gpu_ctx):
+    """Convex Lambert sphere of albedo 0.5 under a uniform sky: every pixel on it
+    sees 0.5 x the sky (one bounce, the sphere cannot see itself)."""
+    sky = flat_xyz(gpu_ctx)
+    img = gpu_render(gpu_ctx, _furnace({"type": "lambert", "gaussians": [[0.5, 550.0, 1e7]]}), 32, 32, 256, 8)
+    centre = img[12:20, 12:20].mean(axis=(0, 1))
+    np.testing.assert_allclose(centre / sky, 0.5, rtol=0.02)
+
+
+def test_lossless_dielectric_keeps_energy(gpu_ctx):
+    """A smooth non-dispersive dielectric absorbs nothing: under a uniform sky
+    the sphere disappears (R + T = 1 along every path)."""
+    sky = flat_xyz(gpu_ctx)
+    img = gpu_render(gpu_ctx, _furnace({"type": "dielectric", "cauchy": [1.5, 0.0]}), 32, 32, 256, 64)
+    np.testing.assert_allclose(img.mean(axis=(0, 1)) / sky, 1.0, rtol=0.02)
+
+
+def test_conductor_normal_incidence(gpu_ctx):
+    """Mirror sphere: the centre pixel sees the sky scaled by the normal-incidence
+    reflectance ((n-1)^2 + k^2) / ((n+1)^2 + k^2)."""
+    n, k = 1.2, 4.0
+    f0 = ((n - 1) ** 2 + k ** 2) / ((n + 1) ** 2 + k ** 2)
+    sky = flat_xyz(gpu_ctx)
+    img = gpu_render(gpu_ctx, _furnace({"type": "conductor", "n": n, "k": k}), 33, 33, 64, 4)
+    np.testing.assert_allclose(img[16, 16] / sky, f0, rtol=5e-3)
+
+
+def test_rebatching_is_invisible(scene_texts):
+    """Counter-based RNG: the same pixels and samples give the same bits whatever
+    the wave capacity (paths per batch) and frame split."""
+    text = scenes.random_spheres(n=512, scene_seed=2, wavelengths=4, seed=5)
+    with pt.Context(0, wave_paths=1500) as small, pt.Context(0) as big:
+        a = gpu_render(small, text, 40, 30, 8, 12, spf=8)
+        b = gpu_render(big, text, 40, 30, 8, 12, spf=8)
+    assert np.array_equal(a, b)
+
+
+def test_bvh_widths_agree(scene_texts):
+    """BVH width changes traversal order only: identical closest hits, identical images."""
+    text = scenes.random_spheres(n=1024, scene_seed=4, wavelengths=4, seed=1)
+    imgs = []
+    for w in (2, 4, 8):
+        with pt.Context(0, bvh_width=w) as c:
+            imgs.append(gpu_render(c, text, 32, 24, 4, 12))
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[1], imgs[2])
+
+
+def test_counters_and_no_nans(gpu_ctx):
+    text = scenes.random_spheres(n=4096, scene_seed=1, wavelengths=4, seed=0)
+    gpu_ctx.reset_counters()
+    img = gpu_render(gpu_ctx, text, 64, 36, 4, 12)
+    c = gpu_ctx.counters()
+    assert not np.isnan(img).any() and c["nan_samples"] == 0
+    assert c["paths"] == 64 * 36 * 4
+    # every segment ends in exactly one class queue
+    assert c["segments"] == c["escaped"] + c["emitter_hits"] + c["lambert_hits"] + c["dielectric_hits"] + c["conductor_hits"]
+    assert c["dielectric_hits"] > 0 and c["conductor_hits"] > 0 and c["shadow_segments"] > 0
