@@ -400,10 +400,15 @@ static double fresnel_cond(double ci, double n, double k) {
     double complex rp = (eta * ci - ct) / (eta * ci + ct);
     return 0.5 * (creal(rs * conj(rs)) + creal(rp * conj(rp)));
 }
+/* Tangent frame of Duff et al. 2017 ("Building an orthonormal basis,
+ * revisited"), the frame the extended mode's sampling is specified in, so
+ * that equal random numbers give equal directions. */
 static void basis(d3 n, d3* t, d3* b) {
-    d3 a = fabs(n.x) > 0.9 ? D3(0, 1, 0) : D3(1, 0, 0);
-    *t = unit(cross(a, n));
-    *b = cross(n, *t);
+    double s = copysign(1.0, n.z);
+    double a = -1.0 / (s + n.z);
+    double c = n.x * n.y * a;
+    *t = D3(1.0 + s * n.x * n.x * a, s * c, -s * n.x);
+    *b = D3(c, s + n.y * n.y * a, -n.y);
 }
 
 static void surface(const Scene* s, const hitrec* h, d3 p, d3* n, int* mat, int* light) {

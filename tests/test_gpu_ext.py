@@ -50,12 +50,21 @@ def converge_check(gpu, gspp, ora, blk=4, z_max=4.0, frac=0.99):
     return share, dimg / np.maximum(sig_img, 1e-30), z
 
 
+def no_caustic_cornell(wavelengths):
+    """The Cornell box with the glass sphere made diffuse: no path reaches the
+    light only through specular bounces, so per-pixel sample variances are
+    finite and well estimated."""
+    s = scenes.cornell_dict(wavelengths, 0)
+    s["material"][4] = {"type": "lambert", "gaussians": [[0.6, 600.0, 80.0]]}
+    return json.dumps(s)
+
+
 @pytest.mark.parametrize("wavelengths", [1, 4])
 def test_cornell_converges_to_fp64_oracle(gpu_ctx, wavelengths):
-    text = scenes.cornell(wavelengths=wavelengths, seed=0)
+    text = no_caustic_cornell(wavelengths)
     W = H = 48
-    gspp, ospp = 2048, 512
-    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 8, spf=256)
+    gspp, ospp = 4096, 1024
+    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 8, spf=512)
     ora = ptext.render(text, W, H, ospp, 8, seed=12345)
     ora["spp"] = ospp
     share, zimg, z = converge_check(gpu, gspp, ora)
@@ -65,30 +74,45 @@ def test_cornell_converges_to_fp64_oracle(gpu_ctx, wavelengths):
     assert np.all(np.abs(zimg) < 3.0)
 
 
-def test_random_spheres_converges_to_fp64_oracle(gpu_ctx):
-    text = scenes.random_spheres(n=4096, scene_seed=1, wavelengths=4, seed=0)
-    W, H = 48, 32
-    gspp, ospp = 2048, 512
-    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 12, spf=256)
-    ora = ptext.render(text, W, H, ospp, 12, seed=777)
+def same_seed_agreement(ctx, text, W, H, spp, depth):
+    """GPU (FP32) and oracle (FP64) with the same seed draw the same numbers in
+    the same frames, so they trace the same paths; returns the share of pixels
+    within 1e-3 and 1e-2 relative, and the number of diverged pixels."""
+    g = gpu_render(ctx, text, W, H, spp, depth)
+    o = ptext.render(text, W, H, spp, depth)["mean"]
+    den = np.abs(o).sum(-1)
+    rel = np.where(den > 0, np.abs(g - o).sum(-1) / np.maximum(den, 1e-30), np.abs(g).sum(-1))
+    return float((rel <= 1e-3).mean()), float((rel <= 1e-2).mean()), int((rel > 1e-3).sum())
+
+
+@pytest.mark.parametrize("name,min3,min2", [("cornell", 0.999, 0.999), ("spheres", 0.95, 0.97), ("sdf", 0.90, 0.95)])
+def test_same_seed_paths_match_fp64_oracle(gpu_ctx, name, min3, min2):
+    text, depth, (W, H) = {
+        "cornell": (scenes.cornell(4), 8, (48, 48)),
+        "spheres": (scenes.random_spheres(4096, 1, 4), 12, (64, 36)),
+        "sdf": (scenes.sdf_scene(), 8, (64, 36)),
+    }[name]
+    f3, f2, diverged = same_seed_agreement(gpu_ctx, text, W, H, 16, depth)
+    print(f"\n{name}: {f3 * 100:.2f}% of pixels within 1e-3, {f2 * 100:.2f}% within 1e-2, "
+          f"{diverged} pixels with diverged paths")
+    assert f3 >= min3 and f2 >= min2
+
+
+@pytest.mark.parametrize("name", ["cornell_glass", "spheres", "sdf"])
+def test_image_mean_converges_to_fp64_oracle(gpu_ctx, name):
+    """Scenes with caustics (light reached only through specular bounces) have
+    heavy-tailed pixel estimates; their image means must still agree within 3 sigma."""
+    text, depth, (W, H), gspp, ospp = {
+        "cornell_glass": (scenes.cornell(4), 8, (48, 48), 4096, 1024),
+        "spheres": (scenes.random_spheres(4096, 1, 4), 12, (48, 32), 4096, 1024),
+        "sdf": (scenes.sdf_scene(), 8, (48, 27), 1024, 256),
+    }[name]
+    gpu = gpu_render(gpu_ctx, text, W, H, gspp, depth, spf=min(gspp, 512))
+    ora = ptext.render(text, W, H, ospp, depth, seed=4242)
     ora["spp"] = ospp
-    share, zimg, _ = converge_check(gpu, gspp, ora)
-    print(f"\nspheres: {share * 100:.2f}% of blocks within 4 sigma, image-mean z = {np.round(zimg, 2)}")
-    assert share >= 0.99
+    _, zimg, _ = converge_check(gpu, gspp, ora)
+    print(f"\n{name}: image-mean z = {np.round(zimg, 2)}")
     assert np.all(np.abs(zimg) < 3.0)
-
-
-def test_sdf_scene_converges_to_fp64_oracle(gpu_ctx):
-    text = scenes.sdf_scene(wavelengths=4, seed=0)
-    W, H = 48, 27
-    gspp, ospp = 512, 256
-    gpu = gpu_render(gpu_ctx, text, W, H, gspp, 8, spf=128)
-    ora = ptext.render(text, W, H, ospp, 8, seed=99)
-    ora["spp"] = ospp
-    share, zimg, _ = converge_check(gpu, gspp, ora, blk=3, z_max=5.0, frac=0.98)
-    print(f"\nsdf: {share * 100:.2f}% of blocks within 5 sigma, image-mean z = {np.round(zimg, 2)}")
-    assert share >= 0.98
-    assert np.all(np.abs(zimg) < 4.0)
 
 
 def _furnace(material, env=1.0, wavelengths=4):
@@ -133,9 +157,14 @@ def test_conductor_normal_incidence(gpu_ctx):
     reflectance ((n-1)^2 + k^2) / ((n+1)^2 + k^2)."""
     n, k = 1.2, 4.0
     f0 = ((n - 1) ** 2 + k ** 2) / ((n + 1) ** 2 + k ** 2)
-    sky = flat_xyz(gpu_ctx)
+    # A sky-only render of the same pixels draws the same wavelengths (the RNG is
+    # keyed on pixel and sample), so the per-pixel ratio is the reflectance.
+    t = json.loads(_furnace({"type": "conductor", "n": n, "k": k}))
+    t["sphere"] = []
+    sky = gpu_render(gpu_ctx, json.dumps(t), 33, 33, 64, 4)
     img = gpu_render(gpu_ctx, _furnace({"type": "conductor", "n": n, "k": k}), 33, 33, 64, 4)
-    np.testing.assert_allclose(img[16, 16] / sky, f0, rtol=5e-3)
+    # the centre 3 x 3 pixels see the sphere within ~6 degrees of normal incidence
+    np.testing.assert_allclose(img[15:18, 15:18] / sky[15:18, 15:18], f0, rtol=3e-3)
 
 
 def test_rebatching_is_invisible(scene_texts):
