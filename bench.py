@@ -55,10 +55,30 @@ def parse_args():
     ap.add_argument("--spheres", type=int, default=4096)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
+    ap.add_argument("--traversal", type=int, default=0, help="0 one ray per lane, 1 persistent lanes refilled from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run --steps device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe(mib: int, steps: int):
+    """Plain device copies of a known size: FETCH_SIZE / WRITE_SIZE per dispatch
+    of this kernel against mib MiB give the counters' scale factor."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    assert torch.equal(dst, src)
+    print(json.dumps({"copy_bytes": n * 4, "steps": steps, "GB/s_read+write": 2 * n * 4 * steps / dt / 1e9}))
 
 
 def traffic_from_profiles(kernel: str):
@@ -75,24 +95,26 @@ def traffic_from_profiles(kernel: str):
 
 def cpu_baseline(args, scene_text: str):
     """The FP64 oracle tracer (CPU restatement with its own BVH) on a bounded
-    sample of the same workload: full resolution, a number of samples per pixel
-    chosen to take about --cpu-seconds, all host threads."""
+    sample of the same workload: the full frame at a number of samples per
+    pixel chosen to take about --cpu-seconds, on this process's host cores
+    (at most 16, the share of one GPU)."""
     try:
         from oracle import ptext
     except Exception as e:  # oracle not built
         return {"value": None, "unit": "segments/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
-    threads = os.cpu_count() or 1
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    band = max(1, args.height // 16)
     probe = ptext.render(scene_text, args.width, args.height, spp=1, depth=args.depth, threads=threads,
-                         rows=(0, max(1, args.height // 16)))
-    rate = probe["segments"] / max(probe["seconds"], 1e-9)
-    rows = max(1, min(args.height, int(args.cpu_seconds * rate / max(1, probe["segments"]) * (args.height // 16))))
-    run = ptext.render(scene_text, args.width, args.height, spp=1, depth=args.depth, threads=threads, rows=(0, rows))
+                         rows=(0, band))
+    frame_s = probe["seconds"] * args.height / band
+    spp = max(1, int(round(args.cpu_seconds / max(frame_s, 1e-9))))
+    run = ptext.render(scene_text, args.width, args.height, spp=spp, depth=args.depth, threads=threads)
     return {
         "value": run["segments"] / run["seconds"],
         "unit": "segments/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 CPU oracle with BVH, {args.width}x{rows} pixels of the same frame x 1 spp, "
+        "sample": f"FP64 CPU oracle with its own BVH, the full {args.width}x{args.height} frame x {spp} spp, "
                   f"depth {args.depth}: {run['segments']} segments in {run['seconds']:.2f} s",
     }
 
@@ -103,21 +125,22 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU design is sample-split with one accumulator reduce, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe, args.steps)
+        return
     import pathtracer_amd as pt
     from pathtracer_amd import scenes
 
     text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
     W, H = args.width, args.height
-    ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width)
+    ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal)
     scene = ctx.scene_from_json(text)
     r = pt.Renderer(ctx, scene, W, H, spf=args.spp, path_length=args.depth, mode=pt.MODE_EXT)
     for _ in range(args.warmup):
         r.render_frame()
     ctx.sync()
+    # Timed region: exactly --steps dispatches, no instrumentation.
     ctx.reset_counters()
-    if not args.no_profile:
-        ctx.profile(True)
-        ctx.profile_read(reset=True)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -125,9 +148,24 @@ def main():
     ctx.sync()
     dt = time.perf_counter() - t0
     cnt = ctx.counters()
-    prof = ctx.profile_read(reset=True) if not args.no_profile else {}
-    ctx.profile(False)
     seg_rate = cnt["segments"] / dt
+
+    # The same --steps dispatches again with a HIP event pair around every
+    # kernel launch (on the library's stream): per-kernel average durations.
+    prof, pcnt, dt_prof = {}, cnt, None
+    if not args.no_profile:
+        ctx.reset_counters()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            r.render_frame()
+        ctx.sync()
+        dt_prof = time.perf_counter() - t1
+        pcnt = ctx.counters()
+        prof = ctx.profile_read(reset=True)
+        ctx.profile(False)
 
     roofline = None
     if prof:
@@ -135,7 +173,7 @@ def main():
         name, (launches, total_ms) = dom
         bpu, unit = BYTES_PER_UNIT.get(name, (None, None))
         if bpu is not None and launches > 0:
-            units = cnt.get(unit, 0)
+            units = pcnt.get(unit, 0)
             achieved = bpu * units / (total_ms * 1e-3) / 1e9
             traffic = traffic_from_profiles(name)
             roofline = {
@@ -176,6 +214,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if prof:
+        line["profiled_ms_per_step"] = dt_prof * 1e3 / args.steps
         line["kernels_ms"] = {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
     print(json.dumps(line))
     ctx.close()

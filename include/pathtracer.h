@@ -128,6 +128,8 @@ typedef struct pt_config {
     int32_t bvh_width;     /* 2, 4 or 8 */
     int32_t sdf_max_steps; /* sphere-tracing budget of the extended mode */
     int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
+    int32_t traversal;     /* extended mode: 0 = one ray per lane, 1 = persistent lanes refilled from the queue */
+    int32_t reserved;
 } pt_config;
 
 typedef struct pt_counters {

@@ -81,7 +81,8 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     ctx->device = device_ordinal;
     if (cfg) ctx->cfg = *cfg;
     if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && (ctx->cfg.block_size % 64) != 0) ||
-        (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8)) {
+        (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
+        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1) {
         delete ctx;
         return set_global(PT_ERR_INVALID_ARG, "pt_create: invalid pt_config");
     }

@@ -42,6 +42,14 @@ enum : uint32_t { REF_SPHERE = 0u, REF_QUAD = 1u, REF_PLANE = 2u, REF_SDF = 3u }
 constexpr uint32_t kRefShift = 29u, kRefMask = (1u << 29) - 1u, kMiss = 0xFFFFFFFFu;
 enum : int { Q_MISS = 0, Q_EMIT = 1, Q_LAMBERT = 2, Q_DIELECTRIC = 3, Q_CONDUCTOR = 4, N_MATQ = 5 };
 constexpr int kQuadBrute = 16;  // at most this many parallelograms are tested without a BVH
+// Queue lengths: [0..1] extension queues, [2] shadow queue, [3..7] material
+// queues, each counter on its own 256-byte line (device-scope atomics on one
+// line serialise across the chip).
+constexpr int kCntStride = 64;
+enum : int { CNT_SHADOW = 2, CNT_MAT = 3, N_CNT = 3 + N_MATQ };
+// Persistent kernels' queue cursors: kSeg segments per queue, one line each.
+constexpr int kSeg = 32;
+enum : int { CUR_ISECT = 0, CUR_SHADOW = 1 };
 
 struct ExtMat {
     int type;        // pt::MaterialType
@@ -118,7 +126,8 @@ struct ExtWave {
     float4* s_dir; // direction, w = path index bits
     float4* s_xyz; // unoccluded contribution
     float* sum;
-    int* counts;   // [0..1] active, [2] shadow, [3..7] material queues
+    int* counts;   // N_CNT queue lengths, kCntStride apart
+    int* cursor;   // 2 x kSeg persistent-kernel cursors, kCntStride apart
     unsigned long long* stats;
     int cap;
 };
@@ -250,85 +259,103 @@ __device__ __forceinline__ bool isect_quad(const float4* q, f3 o, f3 d, float tm
 // Wide-BVH traversal. `stack` is this lane's column of the block's LDS stack
 // (stride kBlk); STACK bounds the entries (checked on the host against the
 // tree's exact worst case) and SE is the entry type (16-bit when the tree has
-// fewer than 65536 nodes). Returns the leaf slot of the closest hit (or of
+// fewer than 65536 nodes).
+__device__ __forceinline__ void ray_inv(f3 o, f3 d, f3& inv, f3& oi) {
+    const float big = 1e30f;
+    inv = mk3(d.x != 0.0f ? 1.0f / d.x : copysignf(big, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(big, d.y),
+              d.z != 0.0f ? 1.0f / d.z : copysignf(big, d.z));
+    oi = o * inv;
+}
+
+// Visits the node on top of the stack: tests its W children's boxes, the
+// primitives of the leaf children (which can shrink tbest) and pushes the
+// internal children that the ray enters (nearest on top for closest hit).
+// `hit` becomes the leaf slot of the closest hit so far. With ANY, returns true
+// at the first hit.
+template <int W, typename SE, bool ANY, bool QUAD>
+__device__ __forceinline__ bool bvh_visit(const BvhDev& B, f3 o, f3 d, f3 inv, f3 oi, float& tbest, int& hit,
+                                          int& sp, SE* stack) {
+    const float big = 1e30f;
+    int node = (int)stack[(--sp) * kBlk];
+    const float4* nd = reinterpret_cast<const float4*>(B.nodes + (size_t)node * (8 * W));
+    float lo_x[W], lo_y[W], lo_z[W], hi_x[W], hi_y[W], hi_z[W];
+    int kid[W], cnt[W];
+    {
+        const float* f = reinterpret_cast<const float*>(nd);
+        const int* iv = reinterpret_cast<const int*>(nd);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            lo_x[j] = f[0 * W + j]; lo_y[j] = f[1 * W + j]; lo_z[j] = f[2 * W + j];
+            hi_x[j] = f[3 * W + j]; hi_y[j] = f[4 * W + j]; hi_z[j] = f[5 * W + j];
+            kid[j] = iv[6 * W + j]; cnt[j] = iv[7 * W + j];
+        }
+    }
+    float tn[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        float tx0 = fmaf(lo_x[j], inv.x, -oi.x), tx1 = fmaf(hi_x[j], inv.x, -oi.x);
+        float ty0 = fmaf(lo_y[j], inv.y, -oi.y), ty1 = fmaf(hi_y[j], inv.y, -oi.y);
+        float tz0 = fmaf(lo_z[j], inv.z, -oi.z), tz1 = fmaf(hi_z[j], inv.z, -oi.z);
+        float lo = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        float hi = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
+        tn[j] = (lo <= hi && cnt[j] >= 0) ? lo : big;
+    }
+    // Leaves first (they can shrink tbest), then internal children.
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (tn[j] < big && cnt[j] > 0) {
+            for (int k = kid[j]; k < kid[j] + cnt[j]; ++k) {
+                float t;
+                bool h = QUAD ? isect_quad(B.prims + 4 * k, o, d, tbest, t) : isect_sphere(B.prims[k], o, d, tbest, t);
+                if (h) {
+                    tbest = t;
+                    hit = k;
+                    if (ANY) return true;
+                }
+            }
+            tn[j] = big;
+        }
+    }
+    if (ANY) {
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            if (tn[j] < tbest && cnt[j] == 0) stack[(sp++) * kBlk] = (SE)kid[j];
+    } else {
+        // push far-to-near so the nearest child is visited next
+        float ds[W];
+        int ks[W];
+        int n = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (tn[j] < tbest && cnt[j] == 0) {
+                float t = tn[j];
+                int c = kid[j];
+                int p = n++;
+                while (p > 0 && ds[p - 1] < t) {
+                    ds[p] = ds[p - 1];
+                    ks[p] = ks[p - 1];
+                    --p;
+                }
+                ds[p] = t;
+                ks[p] = c;
+            }
+        }
+        for (int j = 0; j < n; ++j) stack[(sp++) * kBlk] = (SE)ks[j];
+    }
+    return false;
+}
+
+// Whole traversal of one ray. Returns the leaf slot of the closest hit (or of
 // any hit with ANY), -1 if none; tbest shrinks to the hit distance.
 template <int W, int STACK, typename SE, bool ANY, bool QUAD>
 __device__ __forceinline__ int bvh_trace(const BvhDev& B, f3 o, f3 d, float& tbest, SE* stack) {
-    const float big = 1e30f;
-    f3 inv = mk3(d.x != 0.0f ? 1.0f / d.x : copysignf(big, d.x), d.y != 0.0f ? 1.0f / d.y : copysignf(big, d.y),
-                 d.z != 0.0f ? 1.0f / d.z : copysignf(big, d.z));
-    f3 oi = o * inv;
+    f3 inv, oi;
+    ray_inv(o, d, inv, oi);
     int hit = -1;
     int sp = 1;
     stack[0] = (SE)0;
-    while (sp > 0) {
-        int node = (int)stack[(--sp) * kBlk];
-        const float4* nd = reinterpret_cast<const float4*>(B.nodes + (size_t)node * (8 * W));
-        float lo_x[W], lo_y[W], lo_z[W], hi_x[W], hi_y[W], hi_z[W];
-        int kid[W], cnt[W];
-        {
-            const float* f = reinterpret_cast<const float*>(nd);
-            const int* iv = reinterpret_cast<const int*>(nd);
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                lo_x[j] = f[0 * W + j]; lo_y[j] = f[1 * W + j]; lo_z[j] = f[2 * W + j];
-                hi_x[j] = f[3 * W + j]; hi_y[j] = f[4 * W + j]; hi_z[j] = f[5 * W + j];
-                kid[j] = iv[6 * W + j]; cnt[j] = iv[7 * W + j];
-            }
-        }
-        float tn[W];
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            float tx0 = fmaf(lo_x[j], inv.x, -oi.x), tx1 = fmaf(hi_x[j], inv.x, -oi.x);
-            float ty0 = fmaf(lo_y[j], inv.y, -oi.y), ty1 = fmaf(hi_y[j], inv.y, -oi.y);
-            float tz0 = fmaf(lo_z[j], inv.z, -oi.z), tz1 = fmaf(hi_z[j], inv.z, -oi.z);
-            float lo = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-            float hi = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
-            tn[j] = (lo <= hi && cnt[j] >= 0) ? lo : big;
-        }
-        // Leaves first (they can shrink tbest), then internal children.
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            if (tn[j] < big && cnt[j] > 0) {
-                for (int k = kid[j]; k < kid[j] + cnt[j]; ++k) {
-                    float t;
-                    bool h = QUAD ? isect_quad(B.prims + 4 * k, o, d, tbest, t) : isect_sphere(B.prims[k], o, d, tbest, t);
-                    if (h) {
-                        tbest = t;
-                        hit = k;
-                        if (ANY) return hit;
-                    }
-                }
-                tn[j] = big;
-            }
-        }
-        if (ANY) {
-#pragma unroll
-            for (int j = 0; j < W; ++j)
-                if (tn[j] < tbest && cnt[j] == 0) stack[(sp++) * kBlk] = (SE)kid[j];
-        } else {
-            // push far-to-near so the nearest child is visited next
-            float ds[W];
-            int ks[W];
-            int n = 0;
-#pragma unroll
-            for (int j = 0; j < W; ++j) {
-                if (tn[j] < tbest && cnt[j] == 0) {
-                    float t = tn[j];
-                    int c = kid[j];
-                    int p = n++;
-                    while (p > 0 && ds[p - 1] < t) {
-                        ds[p] = ds[p - 1];
-                        ks[p] = ks[p - 1];
-                        --p;
-                    }
-                    ds[p] = t;
-                    ks[p] = c;
-                }
-            }
-            for (int j = 0; j < n; ++j) stack[(sp++) * kBlk] = (SE)ks[j];
-        }
-    }
+    while (sp > 0)
+        if (bvh_visit<W, SE, ANY, QUAD>(B, o, d, inv, oi, tbest, hit, sp, stack)) return hit;
     return hit;
 }
 
@@ -518,8 +545,17 @@ __device__ __forceinline__ Surf surface(const ExtSceneDev& S, uint32_t ref, f3 o
 __device__ __forceinline__ int hit_class(const ExtSceneDev& S, uint32_t ref) {
     if (ref == kMiss) return Q_MISS;
     uint32_t idx = ref & kRefMask;
+    const uint32_t kind = ref >> kRefShift;
+    // A reference to no primitive of the scene is treated as a miss, so no
+    // later kernel indexes the scene tables with it.
+    const uint32_t limit = kind == REF_SPHERE ? (uint32_t)S.nsph
+                         : kind == REF_QUAD   ? (uint32_t)S.nquad
+                         : kind == REF_PLANE  ? (uint32_t)S.nplane
+                         : kind == REF_SDF    ? (uint32_t)S.nsdf
+                                              : 0u;
+    if (idx >= limit) return Q_MISS;
     int mat, light;
-    switch (ref >> kRefShift) {
+    switch (kind) {
         case REF_SPHERE: { int2 ml = S.sph_ml[idx]; mat = ml.x; light = ml.y; break; }
         case REF_QUAD: { int2 ml = S.quad_ml[idx]; mat = ml.x; light = ml.y; break; }
         case REF_PLANE: { int2 ml = S.plane_ml[idx]; mat = ml.x; light = ml.y; break; }
@@ -563,14 +599,18 @@ __device__ __forceinline__ void add_xyz(ExtWave& Wv, int p, f3 c) {
 
 // ---------------------------------------------------------------------------------
 // Kernels.
+__device__ __forceinline__ int* qlen(const ExtWave& W, int i) { return W.counts + i * kCntStride; }
+
 __global__ void k_ext_counts(int* counts, int n0) {
     int i = threadIdx.x;
-    if (i < 16) counts[i] = (i == 0) ? n0 : 0;
+    if (i < N_CNT) counts[i * kCntStride] = (i == 0) ? n0 : 0;
 }
-__global__ void k_ext_reset(int* counts, int nxt) {
+// Start of a bounce: empty the next extension queue, the shadow and material
+// queues, and the persistent cursors.
+__global__ void k_ext_reset(int* counts, int* cursor, int nxt) {
     int i = threadIdx.x;
-    if (i == 0) counts[nxt] = 0;
-    if (i >= 2 && i < 3 + N_MATQ) counts[i] = 0;
+    if (i == nxt || (i >= CNT_SHADOW && i < N_CNT)) counts[i * kCntStride] = 0;
+    if (i < 2 * kSeg) cursor[i * kCntStride] = 0;
 }
 
 // BK7 Sellmeier index (the reference camera lens, src/shader.comp:1064-1073).
@@ -660,13 +700,14 @@ __global__ void __launch_bounds__(kBlk) k_ext_raygen(ExtSceneDev S, ExtFrame F, 
     }
 }
 
+// Appends the path to its material queue; every thread of the block calls it.
 __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& Wv, bool active, int p, uint32_t ref) {
-    int cls = active ? hit_class(S, ref) : -1;
-#pragma unroll
-    for (int c = 0; c < N_MATQ; ++c) {
-        int slot = wave_append(&Wv.counts[3 + c], cls == c);
-        if (cls == c) Wv.mq[c][slot] = p;
-    }
+    const int cls = active ? hit_class(S, ref) : -1;
+    int* const ctr[N_MATQ] = {qlen(Wv, CNT_MAT + 0), qlen(Wv, CNT_MAT + 1), qlen(Wv, CNT_MAT + 2),
+                              qlen(Wv, CNT_MAT + 3), qlen(Wv, CNT_MAT + 4)};
+    int slot[N_MATQ];
+    block_append<kBlk, N_MATQ>(ctr, cls >= 0 ? 1u << cls : 0u, slot);
+    if (cls >= 0) Wv.mq[cls][slot[cls]] = p;
 }
 
 // Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
@@ -675,7 +716,7 @@ template <int W, int STACK, typename SE, bool CLASSIFY>
 __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave Wv, int cur) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
-    const int n = Wv.counts[cur];
+    const int n = *qlen(Wv, cur);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
@@ -694,10 +735,145 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave W
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Persistent traversal with lane refill. Each wave loops over the queue until it
+// is drained: lanes whose ray has finished take the next queue entries (once at
+// least kRefill lanes are idle, or none is busy), and every busy lane visits one
+// BVH node per iteration. The wave's lanes stay busy whatever the spread of
+// traversal lengths, which a one-ray-per-lane kernel loses to its longest ray.
+// Per ray, the nodes visited and the primitive tests are those of bvh_trace, so
+// hits are identical. The queue is split into min(kSeg, waves in the grid)
+// segments, each with its own cursor on its own line; a wave takes `chunk` entries per atomic from its home
+// segment (chunk = n / waves in the grid, clamped to [1, kChunkMax]) and hands
+// them to its lanes from registers. Consecutive waves have different home
+// segments, so the blocks of one XCD (blockIdx.x mod 8) share a few segments.
+constexpr int kRefill = 16;
+constexpr int kChunkMax = 64;
+
+// A wave's share of a segmented queue; wave-uniform.
+struct WaveChunk {
+    int lo = 0, hi = 0;          // entries taken but not handed out yet
+    int seg_lo = 0, seg_hi = 0;  // the home segment
+    int chunk = 1;
+    int* cur = nullptr;          // the home segment's cursor
+    bool exhausted = false;      // the home segment has no entries left to take
+    __device__ __forceinline__ bool drained() const { return exhausted && lo >= hi; }
+};
+
+__device__ __forceinline__ WaveChunk wave_chunk_init(int* cursors, int n) {
+    WaveChunk c;
+    const int nw = (int)(blockDim.x >> 6);
+    const int waves = (int)gridDim.x * nw;
+    // every segment needs at least one home wave: a grid of fewer than kSeg
+    // waves uses as many segments as it has waves
+    const int nseg = min(kSeg, waves);
+    const int seg = ((int)blockIdx.x * nw + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) % nseg;
+    const int len = (n + nseg - 1) / nseg;
+    c.seg_lo = min(seg * len, n);
+    c.seg_hi = min(c.seg_lo + len, n);
+    c.chunk = max(1, min(kChunkMax, n / max(1, waves)));
+    c.cur = cursors + seg * kCntStride;
+    c.exhausted = c.seg_lo >= c.seg_hi;
+    return c;
+}
+
+// Gives the idle lanes the next entries of the wave's segment; returns this
+// lane's entry or -1. Call from converged control flow.
+__device__ __forceinline__ int wave_refill(uint64_t idle, WaveChunk& c) {
+    const int lane = (int)__lane_id();
+    const int k = (int)__popcll(idle);
+    const int avail = c.hi - c.lo;
+    int nlo = 0, nhi = 0;
+    if (avail < k && !c.exhausted) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(c.cur, c.chunk);
+        base = __builtin_amdgcn_readfirstlane(base);  // lane 0's value, in a scalar register
+        nlo = min(c.seg_lo + base, c.seg_hi);
+        nhi = min(nlo + c.chunk, c.seg_hi);
+        if (nhi >= c.seg_hi) c.exhausted = true;
+    }
+    const int r = (int)lanes_below(idle);
+    int e = -1;
+    if ((idle >> lane) & 1ull) {
+        if (r < avail) e = c.lo + r;
+        else if (r - avail < nhi - nlo) e = nlo + (r - avail);
+    }
+    if (k <= avail) {
+        c.lo += k;
+    } else {
+        c.lo = nlo + min(k - avail, nhi - nlo);
+        c.hi = nhi;
+    }
+    return e;
+}
+
+// Persistent closest-hit kernel (parallelograms tested directly, spheres through
+// their BVH, then planes); see the persistent traversal notes above. Writes the
+// hits only: k_ext_classify appends the paths to the material queues (one
+// atomic per wave and class for 64 paths, where appending as lanes finish
+// would take one per finishing lane).
+template <int W, int STACK, typename SE>
+__global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur) {
+    __shared__ SE stack_mem[STACK * kBlk];
+    SE* stack = stack_mem + threadIdx.x;
+    const int n = *qlen(Wv, cur);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
+    const int* q = Wv.q[cur];
+    int p = -1, sp = 0, best = -1;
+    f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, inv = o, oi = o;
+    float t = 0.0f;
+    uint32_t ref = kMiss;
+    WaveChunk chunk = wave_chunk_init(Wv.cursor + CUR_ISECT * kSeg * kCntStride, n);
+    while (true) {
+        const uint64_t idle = __ballot(p < 0);
+        if (!chunk.drained() && (__popcll(idle) >= kRefill || ~idle == 0ull)) {
+            int e = wave_refill(idle, chunk);
+            if (e >= 0) {
+                p = q[e];
+                o = ld3(Wv.org, p);
+                d = ld3(Wv.dir, p);
+                t = 1e30f;
+                ref = kMiss;
+                best = -1;
+                for (int i = 0; i < S.nquad; ++i) {  // uniform index: scalar loads
+                    float tq;
+                    if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
+                        t = tq;
+                        ref = (REF_QUAD << kRefShift) | (uint32_t)i;
+                    }
+                }
+                ray_inv(o, d, inv, oi);
+                sp = 0;
+                if (S.nsph > 0) {
+                    stack[0] = (SE)0;
+                    sp = 1;
+                }
+            }
+        }
+        if (__ballot(p >= 0) == 0ull) {
+            if (chunk.drained()) break;
+            continue;
+        }
+        if (p >= 0 && sp > 0) bvh_visit<W, SE, false, false>(S.sbvh, o, d, inv, oi, t, best, sp, stack);
+        if (p >= 0 && sp == 0) {
+            if (best >= 0) ref = (REF_SPHERE << kRefShift) | (uint32_t)S.sbvh.ids[best];
+            for (int i = 0; i < S.nplane; ++i) {
+                float tp = (S.plane_y[i] - o.y) / d.y;
+                if (tp > kEps && tp < t) {
+                    t = tp;
+                    ref = (REF_PLANE << kRefShift) | (uint32_t)i;
+                }
+            }
+            Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+            p = -1;
+        }
+    }
+}
+
 // Sphere tracing of every SDF object of one kind against the current hit.
 template <int KIND>
 __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, int cur) {
-    const int n = Wv.counts[cur];
+    const int n = *qlen(Wv, cur);
     unsigned steps = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
@@ -721,7 +897,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, i
 }
 
 __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S, ExtWave Wv, int cur) {
-    const int n = Wv.counts[cur];
+    const int n = *qlen(Wv, cur);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
     for (int r = 0; r < rounds; ++r) {
@@ -744,7 +920,7 @@ __device__ __forceinline__ bool roulette(const ExtSceneDev& S, uint32_t key, int
 
 // Escaped paths (environment) and emitter hits (weighted by MIS against NEE).
 __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S, ExtWave Wv, int which) {
-    const int n = Wv.counts[3 + which];
+    const int n = *qlen(Wv, CNT_MAT + which);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + which], (unsigned long long)n);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.mq[which][idx];
@@ -788,7 +964,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S, ExtW
 __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
                                                             int bounce) {
     const int nxt = cur ^ 1;
-    const int n = Wv.counts[3 + Q_LAMBERT];
+    const int n = *qlen(Wv, CNT_MAT + Q_LAMBERT);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_LAMBERT], (unsigned long long)n);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
@@ -882,14 +1058,15 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
                 Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), 0u);
             }
         }
-        int slot = wave_append(&Wv.counts[2], shadow);
+        int* const ctr[2] = {qlen(Wv, CNT_SHADOW), qlen(Wv, nxt)};
+        int slot[2];
+        block_append<kBlk, 2>(ctr, (shadow ? 1u : 0u) | (cont ? 2u : 0u), slot);
         if (shadow) {
-            Wv.s_org[slot] = make_float4(sP.x, sP.y, sP.z, smax);
-            Wv.s_dir[slot] = make_float4(sD.x, sD.y, sD.z, __int_as_float(p));
-            Wv.s_xyz[slot] = make_float4(sC.x, sC.y, sC.z, 0.0f);
+            Wv.s_org[slot[0]] = make_float4(sP.x, sP.y, sP.z, smax);
+            Wv.s_dir[slot[0]] = make_float4(sD.x, sD.y, sD.z, __int_as_float(p));
+            Wv.s_xyz[slot[0]] = make_float4(sC.x, sC.y, sC.z, 0.0f);
         }
-        int ns = wave_append(&Wv.counts[nxt], cont);
-        if (cont) Wv.q[nxt][ns] = p;
+        if (cont) Wv.q[nxt][slot[1]] = p;
     }
 }
 
@@ -898,7 +1075,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
 __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
                                                                int bounce) {
     const int nxt = cur ^ 1;
-    const int n = Wv.counts[3 + Q_DIELECTRIC];
+    const int n = *qlen(Wv, CNT_MAT + Q_DIELECTRIC);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_DIELECTRIC], (unsigned long long)n);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
@@ -952,8 +1129,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, Ex
                 Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, 0u);
             }
         }
-        int ns = wave_append(&Wv.counts[nxt], cont);
-        if (cont) Wv.q[nxt][ns] = p;
+        int* const ctr[1] = {qlen(Wv, nxt)};
+        int slot[1];
+        block_append<kBlk, 1>(ctr, cont ? 1u : 0u, slot);
+        if (cont) Wv.q[nxt][slot[0]] = p;
     }
 }
 
@@ -961,7 +1140,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, Ex
 __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
                                                               int bounce) {
     const int nxt = cur ^ 1;
-    const int n = Wv.counts[3 + Q_CONDUCTOR];
+    const int n = *qlen(Wv, CNT_MAT + Q_CONDUCTOR);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_CONDUCTOR], (unsigned long long)n);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
@@ -991,8 +1170,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, Ext
                 Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, 0u);
             }
         }
-        int ns = wave_append(&Wv.counts[nxt], cont);
-        if (cont) Wv.q[nxt][ns] = p;
+        int* const ctr[1] = {qlen(Wv, nxt)};
+        int slot[1];
+        block_append<kBlk, 1>(ctr, cont ? 1u : 0u, slot);
+        if (cont) Wv.q[nxt][slot[0]] = p;
     }
 }
 
@@ -1002,7 +1183,7 @@ template <int W, int STACK, typename SE>
 __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
-    const int n = Wv.counts[2];
+    const int n = *qlen(Wv, CNT_SHADOW);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1014,6 +1195,70 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv) 
         for (int s = 0; !blocked && s < S.nsdf; ++s)
             blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
         if (!blocked) add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
+    }
+    wave_add_u64(&Wv.stats[2], steps);
+}
+
+// Persistent any-hit kernel for the shadow queue; see the persistent traversal
+// notes above. SDF objects are marched when the analytic scene leaves the ray
+// unblocked.
+template <int W, int STACK, typename SE>
+__global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave Wv) {
+    __shared__ SE stack_mem[STACK * kBlk];
+    SE* stack = stack_mem + threadIdx.x;
+    const int n = *qlen(Wv, CNT_SHADOW);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
+    unsigned steps = 0;
+    int i = -1, sp = 0, best = -1, path = 0;
+    bool blocked = false;
+    f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, inv = o, oi = o;
+    float t = 0.0f;
+    WaveChunk chunk = wave_chunk_init(Wv.cursor + CUR_SHADOW * kSeg * kCntStride, n);
+    while (true) {
+        const uint64_t idle = __ballot(i < 0);
+        if (!chunk.drained() && (__popcll(idle) >= kRefill || ~idle == 0ull)) {
+            int e = wave_refill(idle, chunk);
+            if (e >= 0) {
+                i = e;
+                float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+                o = mk3(so.x, so.y, so.z);
+                d = mk3(sd.x, sd.y, sd.z);
+                t = so.w;
+                path = __float_as_int(sd.w);
+                blocked = false;
+                for (int k = 0; k < S.nquad; ++k) {  // uniform index: scalar loads
+                    float tq;
+                    if (isect_quad(S.quad + 4 * k, o, d, t, tq)) {
+                        blocked = true;
+                        break;
+                    }
+                }
+                ray_inv(o, d, inv, oi);
+                sp = 0;
+                if (!blocked && S.nsph > 0) {
+                    stack[0] = (SE)0;
+                    sp = 1;
+                }
+            }
+        }
+        if (__ballot(i >= 0) == 0ull) {
+            if (chunk.drained()) break;
+            continue;
+        }
+        if (i >= 0 && sp > 0 && bvh_visit<W, SE, true, false>(S.sbvh, o, d, inv, oi, t, best, sp, stack)) {
+            blocked = true;
+            sp = 0;
+        }
+        if (i >= 0 && sp == 0) {
+            for (int k = 0; !blocked && k < S.nplane; ++k) {
+                float tp = (S.plane_y[k] - o.y) / d.y;
+                blocked = tp > kEps && tp < t;
+            }
+            for (int s = 0; !blocked && s < S.nsdf; ++s)
+                blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
+            if (!blocked) add_xyz(Wv, path, ld3(Wv.s_xyz, i));
+            i = -1;
+        }
     }
     wave_add_u64(&Wv.stats[2], steps);
 }
@@ -1342,7 +1587,8 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
     // shadow org/dir/xyz 48, sums 12
     size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 12;
-    int rc = dev_alloc(ctx, st->mem, n * per_path + 32 * 256 + 4096);
+    // + 256-byte rounding of 20 arrays + the counters and cursors, one line each
+    int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + 2 * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
     char* p = static_cast<char*>(st->mem.p);
     ExtWave& W = st->wave;
@@ -1353,7 +1599,8 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
     W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
     W.sum = carve_e<float>(p, 3 * n);
-    W.counts = carve_e<int>(p, 16);
+    W.counts = carve_e<int>(p, (size_t)N_CNT * kCntStride);
+    W.cursor = carve_e<int>(p, (size_t)2 * kSeg * kCntStride);
     W.stats = ctx->d_stats;
     W.cap = cap;
     return PT_OK;
@@ -1376,9 +1623,15 @@ template <int W, int STACK, typename SE>
 static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
                           int grid) {
     const ExtSceneDev& S = H.dev;
-    launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, cur ^ 1);
+    // Persistent traversal on request; it tests parallelograms directly, so
+    // scenes with a parallelogram BVH always take the one-ray-per-lane kernels.
+    const bool persistent = ctx->cfg.traversal == 1 && !S.quad_bvh;
+    launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1);
     if (S.nsdf > 0) {
-        launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false>, dim3(grid), dim3(kBlk), S, Wv, cur);
+        if (persistent)
+            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
+        else
+            launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false>, dim3(grid), dim3(kBlk), S, Wv, cur);
         for (int kind : H.sdf_kinds) {
             switch (kind) {
                 case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB>(ctx, "k_ext_march_mandelbulb", S, Wv, cur, grid); break;
@@ -1388,6 +1641,9 @@ static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F,
             }
         }
         launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
+    } else if (persistent) {
+        launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
+        launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
     } else {
         launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, true>, dim3(grid), dim3(kBlk), S, Wv, cur);
     }
@@ -1396,7 +1652,10 @@ static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F,
     launch(ctx, "k_ext_shade_lambert", k_ext_shade_lambert, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
     launch(ctx, "k_ext_shade_dielectric", k_ext_shade_dielectric, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
     launch(ctx, "k_ext_shade_conductor", k_ext_shade_conductor, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
-    launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
+    if (persistent)
+        launch(ctx, "k_ext_shadow", k_ext_shadow_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
+    else
+        launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
 }
 
 template <int W>

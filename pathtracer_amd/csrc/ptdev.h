@@ -113,6 +113,47 @@ __device__ __forceinline__ int wave_append(int* counter, bool pred) {
     uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
     return pred ? base + (int)below : -1;
 }
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Appends the calling thread to each of NQ queues whose bit is set in `want`,
+// with one device atomic per queue per block (per-wave ballots, block totals
+// in LDS): device-scope atomics on one address serialise across the chip, so
+// a block of BLK threads issues BLK/64 times fewer than per-wave appends.
+// slot[q] receives the thread's slot in queue q (-1 if not appended); slots
+// follow (wave, lane) order. Every thread of the block must call it, from
+// block-uniform control flow (it synchronises the block).
+template <int BLK, int NQ>
+__device__ __forceinline__ void block_append(int* const (&ctr)[NQ], uint32_t want, int (&slot)[NQ]) {
+    constexpr int NW = BLK / 64;
+    __shared__ int s_n[NQ][NW];
+    __shared__ int s_base[NQ];
+    const int lane = (int)__lane_id(), wv = (int)threadIdx.x >> 6;
+    uint64_t m[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        m[q] = __ballot((want >> q) & 1u);
+        if (lane == 0) s_n[q][wv] = (int)__popcll(m[q]);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < NQ) {
+        const int q = (int)threadIdx.x;
+        int tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) tot += s_n[q][w];
+        s_base[q] = tot ? atomicAdd(ctr[q], tot) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        int off = s_base[q];
+        for (int w = 0; w < wv; ++w) off += s_n[q][w];
+        slot[q] = ((want >> q) & 1u) ? off + (int)lanes_below(m[q]) : -1;
+    }
+    __syncthreads();  // s_n and s_base are rewritten by the next call
+}
+
 // Sum of `v` over the 64 lanes of the wave, added once to *dst. Call from
 // converged control flow with the whole wave active (lanes without work pass 0).
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {

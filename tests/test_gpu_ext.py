@@ -197,3 +197,22 @@ def test_counters_and_no_nans(gpu_ctx):
     # every segment ends in exactly one class queue
     assert c["segments"] == c["escaped"] + c["emitter_hits"] + c["lambert_hits"] + c["dielectric_hits"] + c["conductor_hits"]
     assert c["dielectric_hits"] > 0 and c["conductor_hits"] > 0 and c["shadow_segments"] > 0
+
+
+@pytest.mark.parametrize("name", ["spheres", "sdf", "cornell"])
+def test_persistent_traversal_is_invisible(scene_texts, name):
+    """Persistent lanes refilled from the queue visit the same nodes per ray as
+    one ray per lane: identical hits, identical images and counters."""
+    text, depth = {
+        "spheres": (scenes.random_spheres(n=4096, scene_seed=1, wavelengths=4, seed=7), 12),
+        "sdf": (scenes.sdf_scene(seed=7), 8),
+        "cornell": (scenes.cornell(4, seed=7), 8),
+    }[name]
+    imgs, cnts = [], []
+    for tr in (0, 1):
+        with pt.Context(0, traversal=tr) as c:
+            imgs.append(gpu_render(c, text, 64, 36, 4, depth))
+            cnts.append(c.counters())
+    assert np.array_equal(imgs[0], imgs[1])
+    for k in ("segments", "shadow_segments", "sdf_steps", "lambert_hits", "dielectric_hits", "conductor_hits"):
+        assert cnts[0][k] == cnts[1][k], k
