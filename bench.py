@@ -54,6 +54,7 @@ def parse_args():
     ap.add_argument("--depth", type=int, default=12)
     ap.add_argument("--spheres", type=int, default=4096)
     ap.add_argument("--bvh-width", type=int, default=0)
+    ap.add_argument("--bvh-leaf", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--traversal", type=int, default=0, help="0 one ray per lane, 1 persistent lanes refilled from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
@@ -133,7 +134,8 @@ def main():
 
     text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
     W, H = args.width, args.height
-    ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal)
+    ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal,
+                     bvh_leaf=args.bvh_leaf)
     scene = ctx.scene_from_json(text)
     r = pt.Renderer(ctx, scene, W, H, spf=args.spp, path_length=args.depth, mode=pt.MODE_EXT)
     for _ in range(args.warmup):

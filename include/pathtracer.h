@@ -125,11 +125,11 @@ typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
 typedef struct pt_config {
     int32_t wave_paths;    /* paths in flight per wave (queue capacity) */
     int32_t block_size;    /* threads per block of the queue kernels (multiple of 64) */
-    int32_t bvh_width;     /* 2, 4 or 8 */
+    int32_t bvh_width;     /* 2, 4 or 8 (0 = 2) */
     int32_t sdf_max_steps; /* sphere-tracing budget of the extended mode */
     int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
     int32_t traversal;     /* extended mode: 0 = one ray per lane, 1 = persistent lanes refilled from the queue */
-    int32_t reserved;
+    int32_t bvh_leaf;      /* most primitives per BVH leaf, 1..16 (0 = 1) */
 } pt_config;
 
 typedef struct pt_counters {
@@ -191,6 +191,17 @@ pt_status pt_sdf_recognize(const char* glsl, size_t len, pt_sdf_desc* out);
  * pointer may be NULL. */
 pt_status pt_pack_json(const char* json, size_t len, int shot, pt_packed_scene* ubo, pt_push_constants* pc,
                        pt_sdf_desc* sdfs, int cap, int* n_sdfs);
+
+/* Scene writer, UpdateToJSON + SaveScene (src/pathtracer.cpp:2724-2858,
+ * 3465-3470): the reference's keys in its order with every float rounded to
+ * the 1e-5 grid as the reference saves it (RoundDecimal, :935-943), 4-space
+ * indentation; the extended keys follow with their floats exact. Writes at most
+ * `cap` bytes including a terminating NUL into `out` (which may be NULL to ask
+ * for the size); *len receives the text length without the NUL. Returns
+ * PT_ERR_CAPACITY when `cap` is too small. */
+pt_status pt_scene_write_json(const pt_scene* scene, char* out, size_t cap, size_t* len);
+/* Host-only: parses `json` and writes it back as pt_scene_write_json does. */
+pt_status pt_json_rewrite(const char* json, size_t json_len, char* out, size_t cap, size_t* len);
 
 /* Accumulator: device RGBA32F, W*H texels, zeroed (the reference's texel buffer). */
 pt_status pt_accum_create(pt_ctx* ctx, int width, int height, pt_accum** out);

@@ -97,7 +97,7 @@ class Config(C.Structure):
         ("sdf_max_steps", C.c_int32),
         ("hbm_budget", C.c_int64),
         ("traversal", C.c_int32),
-        ("reserved", C.c_int32),
+        ("bvh_leaf", C.c_int32),
     ]
 
 
@@ -147,6 +147,8 @@ SIGNATURES = {
         [C.c_char_p, C.c_size_t, C.c_int, C.POINTER(PackedScene), C.POINTER(PushConstants), C.POINTER(SdfDesc),
          C.c_int, C.POINTER(C.c_int)],
     ),
+    "pt_scene_write_json": (C.c_int, [_P, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "pt_json_rewrite": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "pt_accum_create": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(_P)]),
     "pt_accum_reset": (C.c_int, [_P]),
     "pt_accum_read_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
@@ -223,6 +225,21 @@ def pack_json(text: str, shot: int = 1, pc: PushConstants | None = None):
     return ubo, pc, [arr[i] for i in range(n.value)]
 
 
+def _read_text(call) -> str:
+    n = C.c_size_t()
+    _check(call(None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    _check(call(buf, n.value + 1, C.byref(n)))
+    return buf.raw[: n.value].decode()
+
+
+def json_rewrite(text: str) -> str:
+    """Host-only: parse a scene and write it back as the reference's SaveScene
+    does (UpdateToJSON, src/pathtracer.cpp:2724-2858), extended keys exact."""
+    b = text.encode()
+    return _read_text(lambda out, cap, n: lib.pt_json_rewrite(b, len(b), out, cap, n))
+
+
 def math_eval_host(fn: int, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
     x = np.ascontiguousarray(x, dtype=np.float32)
     yy = np.ascontiguousarray(y, dtype=np.float32) if y is not None else None
@@ -235,8 +252,8 @@ class Context:
     """pt_ctx: one device, one stream."""
 
     def __init__(self, device: int = 0, wave_paths: int = 0, block_size: int = 0, bvh_width: int = 0,
-                 sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0):
-        cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, 0)
+                 sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0, bvh_leaf: int = 0):
+        cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, bvh_leaf)
         h = C.c_void_p()
         _check(lib.pt_create(C.byref(h), device, C.byref(cfg)))
         self.h = h
@@ -349,6 +366,9 @@ class Scene:
         b = C.c_int64()
         _check(lib.pt_scene_device_bytes(self.h, C.byref(b)))
         return b.value
+
+    def to_json(self) -> str:
+        return _read_text(lambda out, cap, n: lib.pt_scene_write_json(self.h, out, cap, n))
 
 
 class Accum:

@@ -82,7 +82,7 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     if (cfg) ctx->cfg = *cfg;
     if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && (ctx->cfg.block_size % 64) != 0) ||
         (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
-        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1) {
+        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16) {
         delete ctx;
         return set_global(PT_ERR_INVALID_ARG, "pt_create: invalid pt_config");
     }
@@ -243,6 +243,39 @@ pt_status pt_pack_json(const char* json, size_t len, int shot, pt_packed_scene* 
         return PT_OK;
     } catch (const std::exception& e) {
         return set_global(PT_ERR_JSON, std::string("pt_pack_json: ") + e.what());
+    }
+}
+
+static pt_status write_text(const std::string& text, char* out, size_t cap, size_t* len) {
+    if (len) *len = text.size();
+    if (!out) return PT_OK;
+    if (cap < text.size() + 1) return set_global(PT_ERR_CAPACITY, "scene JSON needs " + std::to_string(text.size() + 1) + " bytes");
+    std::memcpy(out, text.data(), text.size());
+    out[text.size()] = '\0';
+    return PT_OK;
+}
+
+pt_status pt_scene_write_json(const pt_scene* scene, char* out, size_t cap, size_t* len) {
+    if (!scene) return set_global(PT_ERR_INVALID_ARG, "pt_scene_write_json: NULL scene");
+    if (!scene->has_model)
+        return set_global(PT_ERR_STATE, "pt_scene_write_json: the scene came from a packed buffer and has no JSON model");
+    try {
+        return write_text(pt::write_scene_json(scene->model), out, cap, len);
+    } catch (const std::exception& e) {
+        return set_global(PT_ERR_JSON, std::string("pt_scene_write_json: ") + e.what());
+    }
+}
+
+pt_status pt_json_rewrite(const char* json, size_t json_len, char* out, size_t cap, size_t* len) {
+    if (!json) return set_global(PT_ERR_INVALID_ARG, "pt_json_rewrite: NULL json");
+    try {
+        pt::SceneModel m;
+        std::string err;
+        int st = pt::load_scene_json(json, json_len, m, err);
+        if (st != PT_OK) return set_global(st, "scene JSON: " + err);
+        return write_text(pt::write_scene_json(m), out, cap, len);
+    } catch (const std::exception& e) {
+        return set_global(PT_ERR_JSON, std::string("pt_json_rewrite: ") + e.what());
     }
 }
 

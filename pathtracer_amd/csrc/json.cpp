@@ -275,8 +275,14 @@ static void dump_rec(std::string& o, const Value& v, int indent, int level) {
         case Kind::Bool: o += v.b ? "true" : "false"; break;
         case Kind::Int: o += std::to_string(v.i); break;
         case Kind::Real: {
+            if (!std::isfinite(v.d)) { o += "null"; break; }  // as nlohmann::json does
+            // the shortest %g text that reads back as the same double (or float, for f32)
             char b[40];
-            std::snprintf(b, sizeof b, "%.17g", v.d);
+            for (int p = 1; p <= 17; ++p) {
+                std::snprintf(b, sizeof b, "%.*g", p, v.d);
+                double r = std::strtod(b, nullptr);
+                if (v.f32 ? (float)r == (float)v.d : r == v.d) break;
+            }
             std::string t = b;
             if (t.find_first_of(".eEn") == std::string::npos) t += ".0";
             o += t;

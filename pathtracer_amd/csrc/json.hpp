@@ -23,6 +23,7 @@ struct Value {
     bool b = false;
     int64_t i = 0;
     double d = 0.0;
+    bool f32 = false;  // Real holding a binary32 value: written as the shortest text that re-reads as that float
     std::string s;
     std::vector<Value> arr;
     std::vector<std::pair<std::string, Value>> obj;  // insertion order, like ordered_json

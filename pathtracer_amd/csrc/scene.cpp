@@ -593,4 +593,271 @@ int recognize_sdf(const std::string& glsl, pt_sdf_desc& d, std::string& err) {
     return PT_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// Scene writer: UpdateToJSON + SaveScene (src/pathtracer.cpp:2724-2858, 3465-3470).
+
+namespace {
+
+// RoundDecimal(x, 1e5) (src/pathtracer.cpp:935-943): round half away from zero
+// on the 1e-5 grid, through a truncating integer cast (64-bit here, where the
+// reference's 32-bit cast overflows above |x| = 21474.8).
+double round_decimal(double x) {
+    const double p = 1e5;
+    double n = x >= 0.0 ? (double)(int64_t)(x * p + 0.5) : (double)(int64_t)(x * p - 0.5);
+    return n / p;
+}
+Value jnum(float x) {  // a reference field: rounded as the reference saves it
+    Value v;
+    v.kind = ptjson::Kind::Real;
+    v.d = round_decimal((double)x);
+    return v;
+}
+Value jf32(float x) {  // an extended field: the float itself, shortest text
+    Value v;
+    v.kind = ptjson::Kind::Real;
+    v.d = (double)x;
+    v.f32 = true;
+    return v;
+}
+Value jint(int64_t x) {
+    Value v;
+    v.kind = ptjson::Kind::Int;
+    v.i = x;
+    return v;
+}
+Value jbool(bool x) {
+    Value v;
+    v.kind = ptjson::Kind::Bool;
+    v.b = x;
+    return v;
+}
+Value jstr(const std::string& x) {
+    Value v;
+    v.kind = ptjson::Kind::String;
+    v.s = x;
+    return v;
+}
+Value jarr(std::initializer_list<Value> xs) {
+    Value v;
+    v.kind = ptjson::Kind::Array;
+    v.arr.assign(xs.begin(), xs.end());
+    return v;
+}
+Value jobj() {
+    Value v;
+    v.kind = ptjson::Kind::Object;
+    return v;
+}
+Value jlist() {
+    Value v;
+    v.kind = ptjson::Kind::Array;
+    return v;
+}
+void put(Value& o, const char* key, Value v) { o.obj.emplace_back(key, std::move(v)); }
+Value vec3(const float* a) { return jarr({jnum(a[0]), jnum(a[1]), jnum(a[2])}); }
+Value vec3f(const float* a) { return jarr({jf32(a[0]), jf32(a[1]), jf32(a[2])}); }
+
+}  // namespace
+
+std::string write_scene_json(const SceneModel& s) {
+    Value root = jobj();
+    Value cam = jobj();
+    put(cam, "numShots", jint((int64_t)s.shots.size()));
+    Value pos = jlist(), ang = jlist();
+    for (const CameraShot& c : s.shots) {
+        pos.arr.push_back(vec3(c.pos));
+        ang.arr.push_back(jarr({jnum(c.angle[0]), jnum(c.angle[1])}));
+    }
+    if (!s.shots.empty()) {
+        put(cam, "position", pos);
+        put(cam, "angle", ang);
+    }
+    put(cam, "ISO", jint(s.cam.ISO));
+    put(cam, "size", jnum(s.cam.size));
+    put(cam, "apertureSize", jnum(s.cam.apertureSize));
+    put(cam, "apertureDistance", jnum(s.cam.apertureDist));
+    put(cam, "lensRadius", jnum(s.cam.lensRadius));
+    put(cam, "lensFocalLength", jnum(s.cam.lensFocalLength));
+    put(cam, "lensThickness", jnum(s.cam.lensThickness));
+    put(cam, "lensDistance", jnum(s.cam.lensDistance));
+    if (s.render.camera == 1) put(cam, "model", jstr("pinhole"));  // extended key
+    put(root, "camera", cam);
+
+    auto ids = [](Value& o, int m, int l) {
+        put(o, "materialID", jint(m));
+        put(o, "lightID", jint(l));
+    };
+    if (!s.spheres.empty()) {
+        Value a = jlist();
+        for (const RSphere& r : s.spheres) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            put(o, "radius", jnum(r.radius));
+            ids(o, r.materialID, r.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "sphere", a);
+    }
+    if (!s.planes.empty()) {
+        Value a = jlist();
+        for (const RPlane& r : s.planes) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            ids(o, r.materialID, r.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "plane", a);
+    }
+    if (!s.boxes.empty()) {
+        Value a = jlist();
+        for (const RBox& r : s.boxes) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            put(o, "rotation", vec3(r.rotation));
+            put(o, "size", vec3(r.size));
+            ids(o, r.materialID, r.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "box", a);
+    }
+    if (!s.lenses.empty()) {
+        Value a = jlist();
+        for (const RLens& r : s.lenses) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            put(o, "rotation", vec3(r.rotation));
+            put(o, "radius", jnum(r.radius));
+            put(o, "focalLength", jnum(r.focalLength));
+            put(o, "thickness", jnum(r.thickness));
+            put(o, "isConverging", jbool(r.isConverging));
+            ids(o, r.materialID, r.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "lens", a);
+    }
+    if (!s.cyclides.empty()) {
+        Value a = jlist();
+        for (const RCyclide& r : s.cyclides) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            put(o, "rotation", vec3(r.rotation));
+            put(o, "scale", vec3(r.scale));
+            put(o, "a", jnum(r.a));
+            put(o, "b", jnum(r.b));
+            put(o, "c", jnum(r.c));
+            put(o, "d", jnum(r.d));
+            put(o, "boundingRadius", jnum(r.brad));
+            ids(o, r.materialID, r.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "cyclide", a);
+    }
+    if (!s.sdfs.empty()) {
+        Value a = jlist();
+        for (const RSdf& r : s.sdfs) {
+            Value o = jobj();
+            put(o, "position", vec3(r.pos));
+            put(o, "boundingSize", vec3(r.size));
+            if (!r.glsl.empty()) {
+                put(o, "glsl", jstr(r.glsl));
+            } else {  // extended form
+                const pt_sdf_desc& d = r.desc;
+                const char* kinds[] = {"none", "mandelbulb", "menger", "blob", "terrain"};
+                put(o, "kind", jstr(kinds[d.kind >= 0 && d.kind <= 4 ? d.kind : 0]));
+                put(o, "iterations", jint(d.iterations));
+                if (d.kind == PT_SDF_MANDELBULB) put(o, "bailout", jf32(d.bailout));
+                if (d.material_kind == PT_SDFMAT_CONSTANT) put(o, "material", jf32(d.material[0]));
+                else if (d.material_kind == PT_SDFMAT_RADIAL)
+                    put(o, "material", jarr({jf32(d.material[0]), jf32(d.material[1]), jf32(d.material[2])}));
+                else
+                    put(o, "material", jarr({jf32(d.material[0]), jf32(d.material[1]), jf32(d.material[2]), jf32(d.material[3])}));
+            }
+            a.arr.push_back(o);
+        }
+        put(root, "sdf", a);
+    }
+    if (!s.materials.empty()) {
+        Value a = jlist();
+        for (const Material& m : s.materials) {
+            Value o = jobj();
+            if (m.type == MAT_LAMBERT_LOBE || m.reflection[0] != 0.0f || m.reflection[1] != 0.0f || m.reflection[2] != 0.0f) {
+                Value r = jobj();
+                put(r, "peakWavelength", jnum(m.reflection[0]));
+                put(r, "sigma", jnum(m.reflection[1]));
+                put(r, "isInvert", jbool(m.reflection[2] != 0.0f));
+                put(o, "reflection", r);
+            }
+            switch (m.type) {  // extended keys
+                case MAT_LAMBERT_GAUSS: {
+                    put(o, "type", jstr("lambert"));
+                    Value g = jlist();
+                    for (int i = 0; i < m.n_gauss; ++i) g.arr.push_back(vec3f(&m.gauss[3 * i]));
+                    put(o, "gaussians", g);
+                    break;
+                }
+                case MAT_DIELECTRIC:
+                    put(o, "type", jstr("dielectric"));
+                    if (m.dispersion == DISP_SELLMEIER) {
+                        Value c = jlist();
+                        for (int k = 0; k < 6; ++k) c.arr.push_back(jf32(m.disp[k]));
+                        put(o, "sellmeier", c);
+                    } else {
+                        put(o, "cauchy", jarr({jf32(m.disp[0]), jf32(m.disp[1])}));
+                    }
+                    break;
+                case MAT_CONDUCTOR:
+                    put(o, "type", jstr("conductor"));
+                    put(o, "n", jf32(m.cond_n));
+                    put(o, "k", jf32(m.cond_k));
+                    break;
+                default: break;
+            }
+            a.arr.push_back(o);
+        }
+        put(root, "material", a);
+    }
+    if (!s.lights.empty()) {
+        Value a = jlist();
+        for (const Light& l : s.lights) {
+            Value o = jobj(), e = jobj();
+            put(e, "temperature", jnum(l.emission[0]));
+            put(e, "luminosity", jnum(l.emission[1]));
+            put(o, "emission", e);
+            a.arr.push_back(o);
+        }
+        put(root, "light", a);
+    }
+    // extended top-level keys
+    if (!s.quads.empty()) {
+        Value a = jlist();
+        for (const Quad& q : s.quads) {
+            Value o = jobj();
+            put(o, "position", vec3f(q.corner));
+            put(o, "edge1", vec3f(q.e1));
+            put(o, "edge2", vec3f(q.e2));
+            ids(o, q.materialID, q.lightID);
+            a.arr.push_back(o);
+        }
+        put(root, "quad", a);
+    }
+    if (s.env_enabled) {
+        Value e = jobj();
+        put(e, "radiance", jf32(s.env_radiance));
+        put(root, "environment", e);
+    }
+    const RenderSettings def;
+    const RenderSettings& r = s.render;
+    if (r.n_lambda != def.n_lambda || r.lambda_min != def.lambda_min || r.lambda_max != def.lambda_max ||
+        r.rr_depth != def.rr_depth || r.seed != def.seed) {
+        Value o = jobj();
+        put(o, "wavelengths", jint(r.n_lambda));
+        put(o, "lambdaMin", jf32(r.lambda_min));
+        put(o, "lambdaMax", jf32(r.lambda_max));
+        put(o, "russianRouletteDepth", jint(r.rr_depth));
+        put(o, "seed", jint((int64_t)r.seed));
+        put(root, "render", o);
+    }
+    return ptjson::dump(root, 4);
+}
+
 }  // namespace pt

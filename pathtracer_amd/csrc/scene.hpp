@@ -92,6 +92,12 @@ const float* cie_table();
 // Camera part of UpdateFromJSON + UpdatePushConstant (src/pathtracer.cpp:2577-2606, 3813-3834).
 int camera_to_push(const SceneModel& s, int shot, pt_push_constants& pc, std::string& err);
 
+// UpdateToJSON + SaveScene (src/pathtracer.cpp:2724-2858, 3465-3470): the
+// reference's keys in its order, each float rounded to the 1e-5 grid as the
+// reference saves it; extended keys follow with their floats exact (the
+// shortest text that reads back as the same binary32 value).
+std::string write_scene_json(const SceneModel& s);
+
 // InsertSDF's input recognised as a built-in kind (src/pathtracer.cpp:2004-2054).
 int recognize_sdf(const std::string& glsl, pt_sdf_desc& desc, std::string& err);
 

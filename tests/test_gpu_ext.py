@@ -177,14 +177,16 @@ def test_rebatching_is_invisible(scene_texts):
     assert np.array_equal(a, b)
 
 
-def test_bvh_widths_agree(scene_texts):
-    """BVH width changes traversal order only: identical closest hits, identical images."""
+def test_bvh_shapes_agree(scene_texts):
+    """BVH width and leaf size change the traversal order only: identical closest
+    hits, identical images."""
     text = scenes.random_spheres(n=1024, scene_seed=4, wavelengths=4, seed=1)
     imgs = []
-    for w in (2, 4, 8):
-        with pt.Context(0, bvh_width=w) as c:
+    for w, leaf in ((2, 4), (2, 1), (4, 4), (8, 8), (2, 16)):
+        with pt.Context(0, bvh_width=w, bvh_leaf=leaf) as c:
             imgs.append(gpu_render(c, text, 32, 24, 4, 12))
-    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[1], imgs[2])
+    for im in imgs[1:]:
+        assert np.array_equal(imgs[0], im)
 
 
 def test_counters_and_no_nans(gpu_ctx):

@@ -133,3 +133,15 @@ def test_ref_rejects_bad_dispatch(gpu_ctx, scene_texts):
     with pytest.raises(pt.PTError) as e:
         gpu_ctx.dispatch(big, acc, pc, pt.MODE_REF)
     assert e.value.status == pt.PT_ERR_CAPACITY
+
+
+def test_scene_to_json_matches_host_rewrite(gpu_ctx, scene_texts):
+    """pt_scene_write_json on a loaded scene writes what pt_json_rewrite writes
+    for the same text; a scene built from a packed buffer has no JSON model."""
+    for text in list(scene_texts.values())[:4]:
+        sc = gpu_ctx.scene_from_json(text)
+        assert sc.to_json() == pt.json_rewrite(text)
+    ubo, _, sdfs = pt.pack_json(list(scene_texts.values())[0])
+    packed = gpu_ctx.scene_from_packed(ubo, sdfs)
+    with pytest.raises(pt.PTError):
+        packed.to_json()
