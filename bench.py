@@ -27,19 +27,22 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic HBM bytes per unit of work of each kernel: the SoA records the
 # kernel must read and write once (path state, hit, queue entries), excluding
-# the BVH and scene tables that stay in L2.
-#   unit of k_ext_intersect = one extension ray (segment):
+# the BVH and scene tables that stay in L2. A kernel's bytes per launch are the
+# sum over its terms of (bytes per unit x units traced in the timed region)
+# divided by its launches.
+#   extension ray (segment), in k_ext_trace / k_ext_intersect:
 #     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8
 #     + primitive material/light lookup 8 + material-queue entry 4      = 48 B
-#   unit of k_ext_shadow = one shadow ray:
+#   shadow ray, in k_ext_trace / k_ext_shadow:
 #     record (origin 12, direction 12, tmax 4, XYZ 12, path 4) 44 + path XYZ read+write 24 = 68 B
-#   unit of k_ext_shade_lambert = one Lambert vertex:
+#   Lambert vertex, in k_ext_shade_lambert:
 #     queue 4 + ray 24 + hit 8 + throughput 16 + wavelength 4 + key 4 + flags 4
 #     + writes: ray 24, throughput 16, pdf 4, flags 4, shadow record 44, next-queue 4 = 160 B
 BYTES_PER_UNIT = {
-    "k_ext_intersect": (48, "segments"),
-    "k_ext_shadow": (68, "shadow_segments"),
-    "k_ext_shade_lambert": (160, "lambert_hits"),
+    "k_ext_trace": [(48, "segments"), (68, "shadow_segments")],
+    "k_ext_intersect": [(48, "segments")],
+    "k_ext_shadow": [(68, "shadow_segments")],
+    "k_ext_shade_lambert": [(160, "lambert_hits")],
 }
 
 
@@ -173,10 +176,10 @@ def main():
     if prof:
         dom = max(prof.items(), key=lambda kv: kv[1][1])
         name, (launches, total_ms) = dom
-        bpu, unit = BYTES_PER_UNIT.get(name, (None, None))
-        if bpu is not None and launches > 0:
-            units = pcnt.get(unit, 0)
-            achieved = bpu * units / (total_ms * 1e-3) / 1e9
+        terms = BYTES_PER_UNIT.get(name)
+        if terms and launches > 0:
+            total_bytes = sum(bpu * pcnt.get(unit, 0) for bpu, unit in terms)
+            achieved = total_bytes / (total_ms * 1e-3) / 1e9
             traffic = traffic_from_profiles(name)
             roofline = {
                 "bound": "hbm",
@@ -187,7 +190,7 @@ def main():
                 "traffic": traffic,
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
-                "algorithmic_bytes_per_launch": round(bpu * units / launches),
+                "algorithmic_bytes_per_launch": round(total_bytes / launches),
             }
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, text)
     line = {
@@ -211,6 +214,8 @@ def main():
             "segments": cnt["segments"],
             "shadow_segments": cnt["shadow_segments"],
             "segments_per_path": cnt["segments"] / max(1, cnt["paths"]),
+            "bvh_nodes_per_segment": cnt["bvh_nodes"] / max(1, cnt["segments"]),
+            "bvh_nodes_per_shadow_ray": cnt["shadow_bvh_nodes"] / max(1, cnt["shadow_segments"]),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

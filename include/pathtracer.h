@@ -145,6 +145,8 @@ typedef struct pt_counters {
     uint64_t lambert_hits;
     uint64_t dielectric_hits;
     uint64_t conductor_hits;
+    uint64_t bvh_nodes;       /* BVH nodes visited by extension rays (extended mode) */
+    uint64_t shadow_bvh_nodes;/* BVH nodes visited by shadow rays (extended mode) */
 } pt_counters;
 
 typedef struct pt_ctx pt_ctx;

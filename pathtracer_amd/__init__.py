@@ -118,6 +118,8 @@ class Counters(C.Structure):
         ("lambert_hits", C.c_uint64),
         ("dielectric_hits", C.c_uint64),
         ("conductor_hits", C.c_uint64),
+        ("bvh_nodes", C.c_uint64),
+        ("shadow_bvh_nodes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -88,8 +88,9 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     }
     hipError_t e = hipSetDevice(device_ordinal);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, 16 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(ctx->d_stats, 0, 16 * sizeof(unsigned long long));
+    const size_t stat_bytes = (size_t)ptd::kStatLines * ptd::kStatStride * sizeof(unsigned long long);
+    if (e == hipSuccess) e = hipMalloc(&ctx->d_stats, stat_bytes);
+    if (e == hipSuccess) e = hipMemset(ctx->d_stats, 0, stat_bytes);
     if (e != hipSuccess) {
         std::string msg = std::string("pt_create: ") + hipGetErrorString(e);
         if (ctx->d_stats) (void)hipFree(ctx->d_stats);
@@ -386,10 +387,14 @@ pt_status pt_sync(pt_ctx* ctx) {
 pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     if (!ctx || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_counters_get: NULL argument");
     (void)hipSetDevice(ctx->device);
-    unsigned long long h[16];
-    hipError_t e = hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+    std::vector<unsigned long long> lines((size_t)ptd::kStatLines * ptd::kStatStride);
+    hipError_t e = hipMemcpyAsync(lines.data(), ctx->d_stats, lines.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_get");
+    unsigned long long h[ptd::kStatSlots] = {};
+    for (int l = 0; l < ptd::kStatLines; ++l)
+        for (int k = 0; k < ptd::kStatSlots; ++k) h[k] += lines[(size_t)l * ptd::kStatStride + (size_t)k];
     out->segments = h[0];
     out->shadow_segments = h[1];
     out->sdf_steps = h[2];
@@ -401,13 +406,16 @@ pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     out->lambert_hits = h[8];
     out->dielectric_hits = h[9];
     out->conductor_hits = h[10];
+    out->bvh_nodes = h[11];
+    out->shadow_bvh_nodes = h[12];
     return PT_OK;
 }
 
 pt_status pt_counters_reset(pt_ctx* ctx) {
     if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_counters_reset: NULL context");
     (void)hipSetDevice(ctx->device);
-    hipError_t e = hipMemsetAsync(ctx->d_stats, 0, 16 * sizeof(unsigned long long), ctx->stream);
+    hipError_t e = hipMemsetAsync(ctx->d_stats, 0, (size_t)ptd::kStatLines * ptd::kStatStride * sizeof(unsigned long long),
+                                  ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_counters_reset");
     return PT_OK;

@@ -46,7 +46,11 @@ constexpr int kQuadBrute = 16;  // at most this many parallelograms are tested w
 // queues, each counter on its own 256-byte line (device-scope atomics on one
 // line serialise across the chip).
 constexpr int kCntStride = 64;
-enum : int { CNT_SHADOW = 2, CNT_MAT = 3, N_CNT = 3 + N_MATQ };
+// The shadow queue's length alternates between two counters by bounce parity:
+// bounce b appends to CNT_SHADOW[b & 1], which the next bounce's trace reads
+// while bounce b + 1 appends to the other one.
+enum : int { CNT_SHADOW0 = 2, CNT_MAT = 3, CNT_SHADOW1 = 3 + N_MATQ, N_CNT = 4 + N_MATQ };
+constexpr int shadow_cnt(int b) { return (b & 1) ? CNT_SHADOW1 : CNT_SHADOW0; }
 // Persistent kernels' queue cursors: kSeg segments per queue, one line each.
 constexpr int kSeg = 32;
 enum : int { CUR_ISECT = 0, CUR_SHADOW = 1 };
@@ -267,16 +271,17 @@ __device__ __forceinline__ void ray_inv(f3 o, f3 d, f3& inv, f3& oi) {
     oi = o * inv;
 }
 
-// Visits the node on top of the stack: tests its W children's boxes, the
-// primitives of the leaf children (which can shrink tbest) and pushes the
-// internal children that the ray enters (nearest on top for closest hit).
-// `hit` becomes the leaf slot of the closest hit so far. With ANY, returns true
-// at the first hit and pushes children unordered.
+// Visits `node`: tests its W children's boxes, the primitives of the leaf
+// children (which can shrink tbest), and picks the next node: the nearest
+// internal child the ray enters (kept in a register, the others pushed far to
+// near), else the top of the stack, else -1 when the traversal is over. Only
+// pushes and pops touch the LDS stack, so most visits' dependent chain is one
+// node fetch. `hit` becomes the leaf slot of the closest hit so far. With ANY,
+// returns true at the first hit, and children are taken in node order.
 template <int W, typename SE, bool ANY, bool QUAD>
-__device__ __forceinline__ bool bvh_visit(const BvhDev& B, f3 o, f3 d, f3 inv, f3 oi, float& tbest, int& hit,
-                                          int& sp, SE* stack) {
+__device__ __forceinline__ bool bvh_step(const BvhDev& B, f3 o, f3 d, f3 inv, f3 oi, float& tbest, int& hit,
+                                         int& node, int& sp, SE* stack) {
     const float big = 1e30f;
-    int node = (int)stack[(--sp) * kBlk];
     const float4* nd = reinterpret_cast<const float4*>(B.nodes + (size_t)node * (8 * W));
     float lo_x[W], lo_y[W], lo_z[W], hi_x[W], hi_y[W], hi_z[W];
     int kid[W], cnt[W];
@@ -316,12 +321,17 @@ __device__ __forceinline__ bool bvh_visit(const BvhDev& B, f3 o, f3 d, f3 inv, f
             tn[j] = big;
         }
     }
+    int next = -1;
     if (ANY) {
 #pragma unroll
-        for (int j = 0; j < W; ++j)
-            if (tn[j] < tbest && cnt[j] == 0) stack[(sp++) * kBlk] = (SE)kid[j];
+        for (int j = 0; j < W; ++j) {
+            if (tn[j] < tbest && cnt[j] == 0) {
+                if (next < 0) next = kid[j];
+                else stack[(sp++) * kBlk] = (SE)kid[j];
+            }
+        }
     } else {
-        // push far-to-near so the nearest child is visited next
+        // entered internal children sorted far to near: ks[n - 1] is the nearest
         float ds[W];
         int ks[W];
         int n = 0;
@@ -340,31 +350,38 @@ __device__ __forceinline__ bool bvh_visit(const BvhDev& B, f3 o, f3 d, f3 inv, f
                 ks[p] = c;
             }
         }
-        for (int j = 0; j < n; ++j) stack[(sp++) * kBlk] = (SE)ks[j];
+        if (n > 0) {
+            for (int j = 0; j < n - 1; ++j) stack[(sp++) * kBlk] = (SE)ks[j];
+            next = ks[n - 1];
+        }
     }
+    if (next < 0 && sp > 0) next = (int)stack[(--sp) * kBlk];
+    node = next;
     return false;
 }
 
 // Whole traversal of one ray. Returns the leaf slot of the closest hit (or of
 // any hit with ANY), -1 if none; tbest shrinks to the hit distance.
+// `visits` counts the nodes visited.
 template <int W, int STACK, typename SE, bool ANY, bool QUAD>
-__device__ __forceinline__ int bvh_trace(const BvhDev& B, f3 o, f3 d, float& tbest, SE* stack) {
+__device__ __forceinline__ int bvh_trace(const BvhDev& B, f3 o, f3 d, float& tbest, SE* stack, unsigned& visits) {
     f3 inv, oi;
     ray_inv(o, d, inv, oi);
-    int hit = -1;
-    int sp = 1;
-    stack[0] = (SE)0;
-    while (sp > 0)
-        if (bvh_visit<W, SE, ANY, QUAD>(B, o, d, inv, oi, tbest, hit, sp, stack)) return hit;
+    int hit = -1, sp = 0, node = 0;
+    while (node >= 0) {
+        ++visits;
+        if (bvh_step<W, SE, ANY, QUAD>(B, o, d, inv, oi, tbest, hit, node, sp, stack)) return hit;
+    }
     return hit;
 }
 
 // Closest (or any, with ANY) hit over parallelograms, spheres and planes.
 template <int W, int STACK, typename SE, bool ANY>
-__device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref, SE* stack) {
+__device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref, SE* stack,
+                                            unsigned& visits) {
     bool hit = false;
     if (S.quad_bvh) {
-        int k = bvh_trace<W, STACK, SE, ANY, true>(S.qbvh, o, d, t, stack);
+        int k = bvh_trace<W, STACK, SE, ANY, true>(S.qbvh, o, d, t, stack, visits);
         if (k >= 0) {
             ref = (REF_QUAD << kRefShift) | (uint32_t)S.qbvh.ids[k];
             hit = true;
@@ -382,7 +399,7 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
         }
     }
     if (S.nsph > 0) {
-        int k = bvh_trace<W, STACK, SE, ANY, false>(S.sbvh, o, d, t, stack);
+        int k = bvh_trace<W, STACK, SE, ANY, false>(S.sbvh, o, d, t, stack, visits);
         if (k >= 0) {
             ref = (REF_SPHERE << kRefShift) | (uint32_t)S.sbvh.ids[k];
             hit = true;
@@ -605,11 +622,12 @@ __global__ void k_ext_counts(int* counts, int n0) {
     int i = threadIdx.x;
     if (i < N_CNT) counts[i * kCntStride] = (i == 0) ? n0 : 0;
 }
-// Start of a bounce: empty the next extension queue, the shadow and material
-// queues, and the persistent cursors.
-__global__ void k_ext_reset(int* counts, int* cursor, int nxt) {
+// Start of bounce b: empty the next extension queue, the material queues, the
+// shadow queue bounce b appends to (read by bounce b - 1 at the latest), and
+// the persistent cursors.
+__global__ void k_ext_reset(int* counts, int* cursor, int nxt, int shq) {
     int i = threadIdx.x;
-    if (i == nxt || (i >= CNT_SHADOW && i < N_CNT)) counts[i * kCntStride] = 0;
+    if (i == nxt || i == shq || (i >= CNT_MAT && i < CNT_MAT + N_MATQ)) counts[i * kCntStride] = 0;
     if (i < 2 * kSeg) cursor[i * kCntStride] = 0;
 }
 
@@ -720,6 +738,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave W
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
+    unsigned visits = 0;
     for (int r = 0; r < rounds; ++r) {
         int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = idx < n;
@@ -728,11 +747,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave W
         if (active) {
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
             float t = 1e30f;
-            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack);
+            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, visits);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
         }
         if (CLASSIFY) classify_append(S, Wv, active, p, ref);
     }
+    wave_add_u64(stat_slot(Wv.stats, 11), visits);
 }
 
 // ---------------------------------------------------------------------------------
@@ -819,7 +839,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
     const int n = *qlen(Wv, cur);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
     const int* q = Wv.q[cur];
-    int p = -1, sp = 0, best = -1;
+    int p = -1, sp = 0, node = -1, best = -1;
+    unsigned visits = 0;
     f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, inv = o, oi = o;
     float t = 0.0f;
     uint32_t ref = kMiss;
@@ -844,18 +865,18 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
                 }
                 ray_inv(o, d, inv, oi);
                 sp = 0;
-                if (S.nsph > 0) {
-                    stack[0] = (SE)0;
-                    sp = 1;
-                }
+                node = S.nsph > 0 ? 0 : -1;
             }
         }
         if (__ballot(p >= 0) == 0ull) {
             if (chunk.drained()) break;
             continue;
         }
-        if (p >= 0 && sp > 0) bvh_visit<W, SE, false, false>(S.sbvh, o, d, inv, oi, t, best, sp, stack);
-        if (p >= 0 && sp == 0) {
+        if (p >= 0 && node >= 0) {
+            ++visits;
+            bvh_step<W, SE, false, false>(S.sbvh, o, d, inv, oi, t, best, node, sp, stack);
+        }
+        if (p >= 0 && node < 0) {
             if (best >= 0) ref = (REF_SPHERE << kRefShift) | (uint32_t)S.sbvh.ids[best];
             for (int i = 0; i < S.nplane; ++i) {
                 float tp = (S.plane_y[i] - o.y) / d.y;
@@ -868,6 +889,54 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
             p = -1;
         }
     }
+    wave_add_u64(stat_slot(Wv.stats, 11), visits);
+}
+
+// Extension rays of queue `cur` (closest hit, then the material queues) and,
+// in the same launch, the shadow rays of the previous bounce (queue length at
+// counter `shq`; any hit, then the unoccluded contribution). Both only read the
+// scene and write different path fields, and a path's shadow contribution is
+// still added before the next bounce's shading, so the sums are unchanged. A
+// bounce launch lasts as long as its slowest ray's chain of dependent node
+// fetches; one launch instead of two pays that once per bounce. Scenes without
+// SDF objects only (their shadow rays also march the SDFs).
+template <int W, int STACK, typename SE>
+__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S, ExtWave Wv, int cur, int shq) {
+    __shared__ SE stack_mem[STACK * kBlk];
+    SE* stack = stack_mem + threadIdx.x;
+    const int ne = *qlen(Wv, cur), ns = *qlen(Wv, shq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&Wv.stats[0], (unsigned long long)ne);
+        atomicAdd(&Wv.stats[1], (unsigned long long)ns);
+    }
+    const int n = ne + ns;
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    unsigned ev = 0, sv = 0;  // nodes visited by extension and shadow rays
+    for (int r = 0; r < rounds; ++r) {
+        const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool ext = idx < ne;
+        int p = 0;
+        uint32_t ref = kMiss;
+        if (ext) {
+            p = Wv.q[cur][idx];
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            float t = 1e30f;
+            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, ev);
+            Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+        } else if (idx < n) {
+            const int i = idx - ne;
+            float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+            f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
+            float t = so.w;
+            uint32_t sref = kMiss;
+            if (!scene_trace<W, STACK, SE, true>(S, o, d, t, sref, stack, sv))
+                add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
+        }
+        classify_append(S, Wv, ext, p, ref);
+    }
+    wave_add_u64(stat_slot(Wv.stats, 11), ev);
+    wave_add_u64(stat_slot(Wv.stats, 12), sv);
 }
 
 // Sphere tracing of every SDF object of one kind against the current hit.
@@ -893,7 +962,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, i
         }
         if (upd) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
     }
-    wave_add_u64(&Wv.stats[2], steps);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
 }
 
 __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S, ExtWave Wv, int cur) {
@@ -1058,7 +1127,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
                 Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), 0u);
             }
         }
-        int* const ctr[2] = {qlen(Wv, CNT_SHADOW), qlen(Wv, nxt)};
+        int* const ctr[2] = {qlen(Wv, shadow_cnt(bounce)), qlen(Wv, nxt)};
         int slot[2];
         block_append<kBlk, 2>(ctr, (shadow ? 1u : 0u) | (cont ? 2u : 0u), slot);
         if (shadow) {
@@ -1180,36 +1249,38 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, Ext
 // Shadow rays: any hit before tmax over the scene and the SDFs; unoccluded
 // contributions are added to the path.
 template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv) {
+__global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
-    const int n = *qlen(Wv, CNT_SHADOW);
+    const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
-    unsigned steps = 0;
+    unsigned steps = 0, visits = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
         f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
         float t = so.w;
         uint32_t ref = kMiss;
-        bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack);
+        bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack, visits);
         for (int s = 0; !blocked && s < S.nsdf; ++s)
             blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
         if (!blocked) add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
     }
-    wave_add_u64(&Wv.stats[2], steps);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 12), visits);
 }
 
 // Persistent any-hit kernel for the shadow queue; see the persistent traversal
 // notes above. SDF objects are marched when the analytic scene leaves the ray
 // unblocked.
 template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave Wv) {
+__global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave Wv, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
-    const int n = *qlen(Wv, CNT_SHADOW);
+    const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0;
-    int i = -1, sp = 0, best = -1, path = 0;
+    int i = -1, sp = 0, node = -1, best = -1, path = 0;
+    unsigned visits = 0;
     bool blocked = false;
     f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, inv = o, oi = o;
     float t = 0.0f;
@@ -1235,21 +1306,21 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
                 }
                 ray_inv(o, d, inv, oi);
                 sp = 0;
-                if (!blocked && S.nsph > 0) {
-                    stack[0] = (SE)0;
-                    sp = 1;
-                }
+                node = !blocked && S.nsph > 0 ? 0 : -1;
             }
         }
         if (__ballot(i >= 0) == 0ull) {
             if (chunk.drained()) break;
             continue;
         }
-        if (i >= 0 && sp > 0 && bvh_visit<W, SE, true, false>(S.sbvh, o, d, inv, oi, t, best, sp, stack)) {
-            blocked = true;
-            sp = 0;
+        if (i >= 0 && node >= 0) {
+            ++visits;
+            if (bvh_step<W, SE, true, false>(S.sbvh, o, d, inv, oi, t, best, node, sp, stack)) {
+                blocked = true;
+                node = -1;
+            }
         }
-        if (i >= 0 && sp == 0) {
+        if (i >= 0 && node < 0) {
             for (int k = 0; !blocked && k < S.nplane; ++k) {
                 float tp = (S.plane_y[k] - o.y) / d.y;
                 blocked = tp > kEps && tp < t;
@@ -1260,7 +1331,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
             i = -1;
         }
     }
-    wave_add_u64(&Wv.stats[2], steps);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 12), visits);
 }
 
 // Per pixel, in sample order: add each path's XYZ to the running sum.
@@ -1277,7 +1349,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_accum(ExtFrame F, ExtWave Wv) {
         }
         Wv.sum[j] = ax; Wv.sum[F.pc + j] = ay; Wv.sum[2 * F.pc + j] = az;
     }
-    wave_add_u64(&Wv.stats[4], nans);
+    wave_add_u64(stat_slot(Wv.stats, 4), nans);
 }
 
 struct ExtFinish {
@@ -1622,6 +1694,11 @@ static void launch_march(pt_ctx* ctx, const char* name, const ExtSceneDev& S, Ex
     launch(ctx, name, k_ext_march<KIND>, dim3(grid), dim3(kBlk), S, Wv, cur);
 }
 
+// One bounce b (extension queue `cur`). Scenes without SDF objects trace the
+// extension rays together with bounce b - 1's shadow rays (k_ext_trace); the
+// last bounce's shadow rays are traced by launch_tail. Scenes with SDF objects,
+// and persistent traversal, keep separate intersect / march / classify /
+// shadow kernels.
 template <int W, int STACK, typename SE>
 static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
                           int grid) {
@@ -1629,8 +1706,11 @@ static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F,
     // Persistent traversal on request; it tests parallelograms directly, so
     // scenes with a parallelogram BVH always take the one-ray-per-lane kernels.
     const bool persistent = ctx->cfg.traversal == 1 && !S.quad_bvh;
-    launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1);
-    if (S.nsdf > 0) {
+    const bool fused = !persistent && S.nsdf == 0;
+    launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1, shadow_cnt(b));
+    if (fused) {
+        launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, shadow_cnt(b - 1));
+    } else if (S.nsdf > 0) {
         if (persistent)
             launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         else
@@ -1644,11 +1724,9 @@ static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F,
             }
         }
         launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
-    } else if (persistent) {
+    } else {
         launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
-    } else {
-        launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, true>, dim3(grid), dim3(kBlk), S, Wv, cur);
     }
     launch(ctx, "k_ext_shade_miss", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_MISS);
     launch(ctx, "k_ext_shade_emit", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_EMIT);
@@ -1658,20 +1736,40 @@ static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F,
     // Shadow rays push children unordered: near-first ordering measured slower
     // on the 4096-sphere scene (most shadow rays are occluded early anyway).
     if (persistent)
-        launch(ctx, "k_ext_shadow", k_ext_shadow_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
-    else
-        launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv);
+        launch(ctx, "k_ext_shadow", k_ext_shadow_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, shadow_cnt(b));
+    else if (!fused)
+        launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, shadow_cnt(b));
+}
+
+// After the last bounce (`nb` bounces, extension queue nb & 1 empty): the
+// shadow rays the last bounce emitted, when the bounces are fused.
+template <int W, int STACK, typename SE>
+static void launch_tail(pt_ctx* ctx, const ExtSceneHost& H, ExtWave& Wv, int nb, int grid) {
+    const ExtSceneDev& S = H.dev;
+    const bool persistent = ctx->cfg.traversal == 1 && !S.quad_bvh;
+    if (persistent || S.nsdf > 0 || nb <= 0) return;
+    launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, nb & 1, shadow_cnt(nb - 1));
 }
 
 template <int W>
 static void launch_bounce_w(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
-                            int grid) {
+                            int grid, bool tail) {
     if (H.small_nodes) {
-        if (H.stack_need <= 32) launch_bounce<W, 32, uint16_t>(ctx, H, F, Wv, cur, b, grid);
-        else launch_bounce<W, 64, uint16_t>(ctx, H, F, Wv, cur, b, grid);
+        if (H.stack_need <= 32) {
+            if (tail) launch_tail<W, 32, uint16_t>(ctx, H, Wv, b, grid);
+            else launch_bounce<W, 32, uint16_t>(ctx, H, F, Wv, cur, b, grid);
+        } else {
+            if (tail) launch_tail<W, 64, uint16_t>(ctx, H, Wv, b, grid);
+            else launch_bounce<W, 64, uint16_t>(ctx, H, F, Wv, cur, b, grid);
+        }
     } else {
-        if (H.stack_need <= 32) launch_bounce<W, 32, int>(ctx, H, F, Wv, cur, b, grid);
-        else launch_bounce<W, 64, int>(ctx, H, F, Wv, cur, b, grid);
+        if (H.stack_need <= 32) {
+            if (tail) launch_tail<W, 32, int>(ctx, H, Wv, b, grid);
+            else launch_bounce<W, 32, int>(ctx, H, F, Wv, cur, b, grid);
+        } else {
+            if (tail) launch_tail<W, 64, int>(ctx, H, Wv, b, grid);
+            else launch_bounce<W, 64, int>(ctx, H, F, Wv, cur, b, grid);
+        }
     }
 }
 
@@ -1717,12 +1815,13 @@ int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
             const int grid = std::min((npaths + kBlk - 1) / kBlk, grid_cap);
             launch(ctx, "k_ext_counts", k_ext_counts, dim3(1), dim3(64), Wv.counts, npaths);
             launch(ctx, "k_ext_raygen", k_ext_raygen, dim3(grid), dim3(kBlk), H.dev, F, Wv);
-            for (int b = 0; b < pc->pathLength; ++b) {
+            for (int b = 0; b <= pc->pathLength; ++b) {
                 const int cur = b & 1;
+                const bool tail = b == pc->pathLength;  // b = the number of bounces traced
                 switch (H.width) {
-                    case 2: launch_bounce_w<2>(ctx, H, F, Wv, cur, b, grid); break;
-                    case 8: launch_bounce_w<8>(ctx, H, F, Wv, cur, b, grid); break;
-                    default: launch_bounce_w<4>(ctx, H, F, Wv, cur, b, grid); break;
+                    case 2: launch_bounce_w<2>(ctx, H, F, Wv, cur, b, grid, tail); break;
+                    case 8: launch_bounce_w<8>(ctx, H, F, Wv, cur, b, grid, tail); break;
+                    default: launch_bounce_w<4>(ctx, H, F, Wv, cur, b, grid, tail); break;
                 }
             }
             launch(ctx, "k_ext_accum", k_ext_accum, dim3((F.pc + kBlk - 1) / kBlk), dim3(kBlk), F, Wv);

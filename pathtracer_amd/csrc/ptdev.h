@@ -154,6 +154,15 @@ __device__ __forceinline__ void block_append(int* const (&ctr)[NQ], uint32_t wan
     __syncthreads();  // s_n and s_base are rewritten by the next call
 }
 
+// Device counters (pt_counters): kStatLines copies of kStatSlots 64-bit slots,
+// one 256-byte line per copy; a block adds into copy blockIdx.x % kStatLines and
+// the host sums the copies, so per-wave additions from a whole grid do not
+// serialise on one address.
+constexpr int kStatSlots = 16, kStatLines = 64, kStatStride = 32;
+__device__ __forceinline__ unsigned long long* stat_slot(unsigned long long* stats, int slot) {
+    return stats + (size_t)(blockIdx.x % kStatLines) * kStatStride + slot;
+}
+
 // Sum of `v` over the 64 lanes of the wave, added once to *dst. Call from
 // converged control flow with the whole wave active (lanes without work pass 0).
 __device__ __forceinline__ void wave_add_u64(unsigned long long* dst, unsigned long long v) {

@@ -753,7 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_ref_march(RefSceneDev S, RefWave Wv,
             Wv.hlight[p] = h.light;
         }
     }
-    wave_add_u64(&Wv.stats[2], steps);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
 }
 
 // TraceRay after the hit (src/shader.comp:1352-1391) with SampleLightSource
@@ -889,7 +889,7 @@ __global__ void __launch_bounds__(kBlock) k_ref_shadow_march(RefSceneDev S, RefW
         if (sphere_trace<KIND>(S, o, d, h, steps)) id = -1;
         if (id == Wv.s_lobj[i]) add_contrib(Wv, i);
     }
-    wave_add_u64(&Wv.stats[2], steps);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
 }
 
 // Per pixel, in sample order: XYZ of each path (src/shader.comp:1477-1487)
@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(kBlock) k_ref_accum(RefSceneDev S, RefFrame F,
         Wv.sum[F.pc + j] = acc.y;
         Wv.sum[2 * F.pc + j] = acc.z;
     }
-    wave_add_u64(&Wv.stats[4], nans);
+    wave_add_u64(stat_slot(Wv.stats, 4), nans);
 }
 
 struct RefFinish {

@@ -199,6 +199,8 @@ def test_counters_and_no_nans(gpu_ctx):
     # every segment ends in exactly one class queue
     assert c["segments"] == c["escaped"] + c["emitter_hits"] + c["lambert_hits"] + c["dielectric_hits"] + c["conductor_hits"]
     assert c["dielectric_hits"] > 0 and c["conductor_hits"] > 0 and c["shadow_segments"] > 0
+    # every ray into the sphere BVH visits its root at least
+    assert c["bvh_nodes"] >= c["segments"] and c["shadow_bvh_nodes"] > 0
 
 
 @pytest.mark.parametrize("name", ["spheres", "sdf", "cornell"])
@@ -216,5 +218,6 @@ def test_persistent_traversal_is_invisible(scene_texts, name):
             imgs.append(gpu_render(c, text, 64, 36, 4, depth))
             cnts.append(c.counters())
     assert np.array_equal(imgs[0], imgs[1])
-    for k in ("segments", "shadow_segments", "sdf_steps", "lambert_hits", "dielectric_hits", "conductor_hits"):
+    for k in ("segments", "shadow_segments", "sdf_steps", "lambert_hits", "dielectric_hits", "conductor_hits",
+              "bvh_nodes", "shadow_bvh_nodes"):
         assert cnts[0][k] == cnts[1][k], k
