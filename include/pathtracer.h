@@ -248,6 +248,74 @@ pt_status pt_resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* hos
 pt_status pt_math_eval_host(int fn, const float* x, const float* y, float* out, size_t n);
 pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n);
 
+/* ---- Serving --------------------------------------------------------------
+ * No counterpart in the reference, which renders one scene interactively until
+ * the user stops (src/pathtracer.cpp:4005-4086). A server traces a stream of
+ * render requests (extended mode) on one context: each request runs until
+ * its relative RMSE estimate, sqrt(sum of per-pixel variance of the mean) /
+ * sqrt(sum of mean^2) over pixels and XYZ channels, reaches its target or its
+ * sample budget is spent. Scenes, BVHs and per-pixel moments of admitted
+ * requests stay in device memory; admission stops at the server's HBM budget
+ * and request count. One step traces one quantum of samples of one request,
+ * chosen by the policy: min_spp samples per pixel first, then the estimated
+ * remaining ones (from error ~ 1/sqrt(samples)), at most a full wave. Times are seconds of the
+ * server's monotonic clock (pt_server_now). A server is used by one host
+ * thread at a time. */
+typedef struct pt_server pt_server;
+#define PT_SERVE_SRPT 0 /* shortest estimated remaining GPU time first */
+#define PT_SERVE_FIFO 1 /* arrival order, one request to completion at a time */
+#define PT_SERVE_IDLE 1 /* pt_server_step status: no request to trace */
+#define PT_REQ_WAITING 0
+#define PT_REQ_ACTIVE 1
+#define PT_REQ_DONE 2
+#define PT_REQ_FAILED 3
+
+typedef struct pt_server_config {
+    int64_t hbm_budget;    /* bytes of admitted scenes + moment buffers (0 = 64 GiB) */
+    int32_t max_active;    /* requests admitted at once (0 = 16) */
+    int32_t policy;        /* PT_SERVE_SRPT or PT_SERVE_FIFO */
+    int32_t min_spp;       /* samples per pixel before a request may be declared converged (0 = 8) */
+    int32_t reserved;
+} pt_server_config;
+
+typedef struct pt_request_desc {
+    const char* json;      /* extended-mode scene JSON; camera shot 1 is rendered */
+    size_t len;
+    int32_t width, height;
+    int32_t spp_max;       /* sample budget per pixel */
+    int32_t path_length;   /* maximum segments per path */
+    float target_rel;      /* relative RMSE at which the request is done, e.g. 0.01 */
+    int32_t reserved;
+} pt_request_desc;
+
+typedef struct pt_request_status {
+    int32_t id, state;     /* PT_REQ_* */
+    int32_t spp;           /* samples per pixel traced */
+    int32_t converged;     /* 1 when rel_error <= target_rel after at least min_spp samples */
+    int32_t width, height;
+    double rel_error;      /* last estimate, -1 before two samples */
+    double t_arrive, t_admit, t_done;
+    double gpu_seconds;    /* wall time of the steps that traced it */
+    uint64_t segments;
+} pt_request_status;
+
+pt_status pt_server_create(pt_ctx* ctx, const pt_server_config* cfg, pt_server** out);
+void pt_server_destroy(pt_server* server);
+const char* pt_server_last_error(const pt_server* server);
+double pt_server_now(const pt_server* server);
+/* Queues a request that arrived at t_arrive (server clock); *id receives its id. */
+pt_status pt_server_submit(pt_server* server, const pt_request_desc* req, double t_arrive, int32_t* id);
+/* Admits what fits, then traces one quantum of the chosen request and updates
+ * its estimate; *stepped receives that request's status. Returns PT_OK,
+ * PT_SERVE_IDLE when nothing is admitted, or an error. */
+pt_status pt_server_step(pt_server* server, pt_request_status* stepped);
+pt_status pt_server_status(pt_server* server, int32_t id, pt_request_status* out);
+/* The request's mean XYZ image as RGBA32F texels in the units of pt_accum. */
+pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t n_floats);
+/* Frees a finished or waiting request's device memory and forgets it. */
+pt_status pt_server_release(pt_server* server, int32_t id);
+pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* active, int64_t* bytes_in_use);
+
 #ifdef __cplusplus
 }
 #endif

@@ -129,6 +129,52 @@ class Counters(C.Structure):
 assert C.sizeof(PackedScene) == 16388
 assert C.sizeof(PushConstants) == 88
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_int64),
+        ("max_active", C.c_int32),
+        ("policy", C.c_int32),
+        ("min_spp", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class RequestDesc(C.Structure):
+    _fields_ = [
+        ("json", C.c_char_p),
+        ("len", C.c_size_t),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("spp_max", C.c_int32),
+        ("path_length", C.c_int32),
+        ("target_rel", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("id", C.c_int32),
+        ("state", C.c_int32),
+        ("spp", C.c_int32),
+        ("converged", C.c_int32),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("rel_error", C.c_double),
+        ("t_arrive", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+        ("gpu_seconds", C.c_double),
+        ("segments", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+SERVE_SRPT, SERVE_FIFO, SERVE_IDLE = 0, 1, 1
+REQ_WAITING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
 # Every symbol include/pathtracer.h declares, with (restype, argtypes).
 _P = C.c_void_p
 SIGNATURES = {
@@ -163,6 +209,16 @@ SIGNATURES = {
     "pt_profile_enable": (C.c_int, [_P, C.c_int]),
     "pt_profile_read": (C.c_int, [_P, C.POINTER(KernelStat), C.c_int, C.POINTER(C.c_int), C.c_int]),
     "pt_resolve_srgb8": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_uint8), C.c_size_t]),
+    "pt_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "pt_server_destroy": (None, [_P]),
+    "pt_server_last_error": (C.c_char_p, [_P]),
+    "pt_server_now": (C.c_double, [_P]),
+    "pt_server_submit": (C.c_int, [_P, C.POINTER(RequestDesc), C.c_double, C.POINTER(C.c_int32)]),
+    "pt_server_step": (C.c_int, [_P, C.POINTER(RequestStatus)]),
+    "pt_server_status": (C.c_int, [_P, C.c_int32, C.POINTER(RequestStatus)]),
+    "pt_server_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
+    "pt_server_release": (C.c_int, [_P, C.c_int32]),
+    "pt_server_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "pt_math_eval_host": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
     "pt_math_eval_device": (
         C.c_int,
@@ -465,3 +521,69 @@ class Renderer:
         with open(path, "wb") as f:
             f.write(b"P6\n%d\n%d\n255\n" % (self.acc.w, self.acc.hgt))
             f.write(rgb.tobytes())
+
+
+class Server:
+    """pt_server: render requests traced on one context until each reaches its
+    relative-RMSE target (shortest estimated remaining time first, or FIFO)."""
+
+    def __init__(self, ctx: "Context", policy: int = SERVE_SRPT, max_active: int = 0, hbm_budget: int = 0,
+                 min_spp: int = 0):
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget, max_active, policy, min_spp, 0)
+        h = C.c_void_p()
+        _check(lib.pt_server_create(ctx.h, C.byref(cfg), C.byref(h)), ctx.h)
+        self.h = h
+
+    def _ck(self, st: int) -> int:
+        if st < 0:
+            raise PTError(st, (lib.pt_server_last_error(self.h) or b"").decode(errors="replace"))
+        return st
+
+    def close(self) -> None:
+        if self.h:
+            lib.pt_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def now(self) -> float:
+        return lib.pt_server_now(self.h)
+
+    def submit(self, text: str, width: int, height: int, spp_max: int, path_length: int, target: float = 0.01,
+               t_arrive: float | None = None) -> int:
+        b = text.encode()
+        d = RequestDesc(b, len(b), width, height, spp_max, path_length, target, 0)
+        rid = C.c_int32()
+        self._ck(lib.pt_server_submit(self.h, C.byref(d), self.now() if t_arrive is None else t_arrive,
+                                      C.byref(rid)))
+        return rid.value
+
+    def step(self) -> RequestStatus | None:
+        """One quantum; None when nothing is admitted."""
+        st = RequestStatus()
+        rc = self._ck(lib.pt_server_step(self.h, C.byref(st)))
+        return None if rc == SERVE_IDLE else st
+
+    def status(self, rid: int) -> RequestStatus:
+        st = RequestStatus()
+        self._ck(lib.pt_server_status(self.h, rid, C.byref(st)))
+        return st
+
+    def read(self, rid: int) -> np.ndarray:
+        st = self.status(rid)
+        out = np.empty((st.height, st.width, 4), np.float32)
+        self._ck(lib.pt_server_read(self.h, rid, _fptr(out), out.size))
+        return out
+
+    def release(self, rid: int) -> None:
+        self._ck(lib.pt_server_release(self.h, rid))
+
+    def counts(self) -> tuple[int, int, int]:
+        w, a, b = C.c_int32(), C.c_int32(), C.c_int64()
+        self._ck(lib.pt_server_counts(self.h, C.byref(w), C.byref(a), C.byref(b)))
+        return w.value, a.value, b.value

@@ -1373,6 +1373,69 @@ __global__ void __launch_bounds__(kBlk) k_ext_finish(ExtFinish F, const float* s
     texels[F.p0 + j] = make_float4(out.x, out.y, out.z, 1.0f);
 }
 
+// Serving sink: per pixel, in sample order, the first and second moments of
+// the per-sample XYZ over a request's samples (sum[c * P + pixel],
+// sumsq[c * P + pixel]); samples with a NaN are dropped and counted.
+__global__ void __launch_bounds__(kBlk) k_ext_moments(ExtFrame F, ExtWave Wv, int P, float* sum, float* sumsq,
+                                                      int* nsamp) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned nans = 0;
+    if (j < F.pc) {
+        const int px = F.p0 + j;
+        float ax = sum[px], ay = sum[P + px], az = sum[2 * P + px];
+        float qx = sumsq[px], qy = sumsq[P + px], qz = sumsq[2 * P + px];
+        int n = nsamp[px];
+        for (int s = 0; s < F.m; ++s) {
+            float4 a = Wv.acc[s * F.pc + j];
+            if (a.x != a.x || a.y != a.y || a.z != a.z) { ++nans; continue; }
+            ax += a.x; ay += a.y; az += a.z;
+            qx += a.x * a.x; qy += a.y * a.y; qz += a.z * a.z;
+            ++n;
+        }
+        sum[px] = ax; sum[P + px] = ay; sum[2 * P + px] = az;
+        sumsq[px] = qx; sumsq[P + px] = qy; sumsq[2 * P + px] = qz;
+        nsamp[px] = n;
+    }
+    wave_add_u64(stat_slot(Wv.stats, 4), nans);
+}
+
+// Relative RMSE estimate of a request's mean image: over pixels and XYZ
+// channels, sum of (sample variance / n) and sum of mean^2. Per block, then
+// one double atomic per block into line blockIdx.x % 64 of `out` (2 doubles
+// per 256-byte line); the host adds the lines.
+__global__ void __launch_bounds__(kBlk) k_ext_rel_error(int P, const float* sum, const float* sumsq, const int* nsamp,
+                                                        double* out) {
+    __shared__ double s_v[kBlk / 64], s_m[kBlk / 64];
+    double v = 0.0, m = 0.0;
+    for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < P; px += gridDim.x * blockDim.x) {
+        const int n = nsamp[px];
+        if (n < 2) continue;
+        for (int c = 0; c < 3; ++c) {
+            const double s1 = sum[c * P + px], s2 = sumsq[c * P + px];
+            const double mean = s1 / n;
+            const double var = fmax(s2 - s1 * mean, 0.0) / (double)(n - 1);
+            v += var / n;
+            m += mean * mean;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off);
+        m += __shfl_down(m, off);
+    }
+    if (__lane_id() == 0) {
+        s_v[threadIdx.x >> 6] = v;
+        s_m[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tv = 0.0, tm = 0.0;
+        for (int w = 0; w < kBlk / 64; ++w) { tv += s_v[w]; tm += s_m[w]; }
+        double* line = out + (size_t)(blockIdx.x % 64) * 32;
+        atomicAdd(&line[0], tv);
+        atomicAdd(&line[1], tm);
+    }
+}
+
 // ---------------------------------------------------------------------------------
 // Host side.
 struct ExtSceneHost {
@@ -1773,7 +1836,18 @@ static void launch_bounce_w(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& 
     }
 }
 
-int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+// Where a frame's per-path XYZ goes: the reference's texel accumulate
+// (running mean / persistence blend of the push constants), or a serving
+// request's per-pixel moments.
+struct ExtSink {
+    pt_accum* acc = nullptr;
+    const pt_push_constants* pc = nullptr;
+    float* sum = nullptr;
+    float* sumsq = nullptr;
+    int* nsamp = nullptr;
+};
+
+static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, const ExtSink& sink) {
     int st = ext_scene_build(ctx, const_cast<pt_scene*>(s));
     if (st != PT_OK) return st;
     const ExtSceneHost& H = *static_cast<const ExtSceneHost*>(s->ext);
@@ -1824,17 +1898,50 @@ int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
                     default: launch_bounce_w<4>(ctx, H, F, Wv, cur, b, grid, tail); break;
                 }
             }
-            launch(ctx, "k_ext_accum", k_ext_accum, dim3((F.pc + kBlk - 1) / kBlk), dim3(kBlk), F, Wv);
+            const dim3 pg((F.pc + kBlk - 1) / kBlk);
+            if (sink.acc)
+                launch(ctx, "k_ext_accum", k_ext_accum, pg, dim3(kBlk), F, Wv);
+            else
+                launch(ctx, "k_ext_moments", k_ext_moments, pg, dim3(kBlk), F, Wv, P, sink.sum, sink.sumsq, sink.nsamp);
         }
-        ExtFinish Fin{};
-        Fin.p0 = p0; Fin.pc = pcur; Fin.spf = pc->samplesPerFrame; Fin.currentSamples = pc->currentSamples;
-        Fin.frame = pc->frame; Fin.ISO = pc->ISO; Fin.apSize = pc->apertureSize; Fin.FPS = pc->FPS;
-        Fin.persistence = pc->persistence;
-        launch(ctx, "k_ext_finish", k_ext_finish, dim3((pcur + kBlk - 1) / kBlk), dim3(kBlk), Fin, (const float*)Wv.sum,
-               reinterpret_cast<float4*>(acc->d_texels));
+        if (sink.acc) {
+            ExtFinish Fin{};
+            Fin.p0 = p0; Fin.pc = pcur; Fin.spf = pc->samplesPerFrame; Fin.currentSamples = pc->currentSamples;
+            Fin.frame = pc->frame; Fin.ISO = pc->ISO; Fin.apSize = pc->apertureSize; Fin.FPS = pc->FPS;
+            Fin.persistence = pc->persistence;
+            launch(ctx, "k_ext_finish", k_ext_finish, dim3((pcur + kBlk - 1) / kBlk), dim3(kBlk), Fin,
+                   (const float*)Wv.sum, reinterpret_cast<float4*>(sink.acc->d_texels));
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "extended dispatch launch");
+    return PT_OK;
+}
+
+int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+    ExtSink sink;
+    sink.acc = acc;
+    sink.pc = pc;
+    return ext_run(ctx, s, pc, sink);
+}
+
+int ext_dispatch_moments(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* sum, float* sumsq,
+                         int* nsamp) {
+    ExtSink sink;
+    sink.pc = pc;
+    sink.sum = sum;
+    sink.sumsq = sumsq;
+    sink.nsamp = nsamp;
+    return ext_run(ctx, s, pc, sink);
+}
+
+int ext_rel_error(pt_ctx* ctx, int P, const float* sum, const float* sumsq, const int* nsamp, double* d_out) {
+    ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
+    const int cu = cs && cs->cu ? cs->cu : 256;
+    const int grid = std::max(1, std::min((P + kBlk - 1) / kBlk, cu * 4));
+    launch(ctx, "k_ext_rel_error", k_ext_rel_error, dim3(grid), dim3(kBlk), P, sum, sumsq, nsamp, d_out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "relative error launch");
     return PT_OK;
 }
 

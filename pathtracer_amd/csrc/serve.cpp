@@ -1,0 +1,370 @@
+// serve.cpp — render requests served on one context (pt_server_*).
+//
+// The reference renders one scene interactively and accumulates until the user
+// stops (src/pathtracer.cpp:4005-4086). A server instead takes a stream of
+// requests (scene JSON, resolution, sample budget, relative-error target) and
+// traces each until its estimated relative RMSE reaches the target:
+//   - every admitted request keeps its scene, BVH and per-pixel moment buffers
+//     resident in HBM; admission stops at the server's HBM budget and request
+//     count;
+//   - a step traces one quantum of samples of one request: min_spp samples per
+//     pixel first, then the estimated remaining samples, at most as many as
+//     fill the context's wave (a small image traces many samples per pixel in
+//     one wave, so it still fills the GPU);
+//   - the request is chosen by the policy: shortest estimated remaining GPU time
+//     first (PT_SERVE_SRPT), or arrival order, one request to completion at a
+//     time (PT_SERVE_FIFO, the one-job-at-a-time baseline);
+//   - after each quantum the relative RMSE is estimated on the GPU from the
+//     per-pixel second moments (k_ext_rel_error): sqrt(sum var/n) / sqrt(sum mean^2)
+//     over pixels and XYZ channels.
+// The extended tracer's random numbers are keyed on (scene seed, pixel, sample,
+// bounce, dimension), so a request's image does not depend on how its samples
+// were split into quanta or interleaved with other requests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pathtracer.h"
+#include "tracer.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    int id = 0;
+    std::string json;
+    int W = 0, H = 0, P = 0;
+    int spp_max = 0, path_length = 0;
+    float target = 0.01f;
+    pt_scene* scene = nullptr;
+    pt_push_constants pc{};
+    DevAlloc mem;  // sum (3P floats), sumsq (3P floats), nsamp (P ints)
+    float* sum = nullptr;
+    float* sumsq = nullptr;
+    int* nsamp = nullptr;
+    int64_t bytes = 0;  // scene + moment buffers, counted against the server budget
+    int spp = 0;
+    double rel = INFINITY;
+    int state = PT_REQ_WAITING;
+    double t_arrive = 0.0, t_admit = -1.0, t_done = -1.0, gpu_s = 0.0;
+    uint64_t segments = 0;
+};
+
+}  // namespace
+
+struct pt_server {
+    pt_ctx* ctx = nullptr;
+    pt_server_config cfg{};
+    Clock::time_point t0;
+    int next_id = 0;
+    std::map<int, std::unique_ptr<Request>> reqs;
+    std::deque<int> waiting;  // arrival order
+    std::vector<int> active;
+    int64_t bytes_in_use = 0;
+    double sec_per_path = 0.0;  // running estimate of GPU seconds per traced path
+    double* d_rel = nullptr;    // 64 lines x 32 doubles
+    std::string err;
+};
+
+namespace {
+
+constexpr int kRelLines = 64, kRelStride = 32;
+
+double now_s(const pt_server* s) { return std::chrono::duration<double>(Clock::now() - s->t0).count(); }
+
+int fail(pt_server* s, int st, const std::string& msg) {
+    s->err = msg;
+    return st;
+}
+
+// Bytes a request will hold once admitted: the moment buffers plus an estimate
+// of its scene tables and BVH (a few hundred bytes per primitive).
+int64_t estimate_bytes(const Request& r) {
+    int64_t prims = 0;
+    for (size_t i = r.json.find("\"radius\""); i != std::string::npos; i = r.json.find("\"radius\"", i + 1)) ++prims;
+    for (size_t i = r.json.find("\"edge1\""); i != std::string::npos; i = r.json.find("\"edge1\"", i + 1)) ++prims;
+    return (int64_t)r.P * 28 + prims * 320 + (1 << 20);
+}
+
+void release_request(pt_server* s, Request& r) {
+    if (r.scene) pt_scene_release(r.scene);
+    r.scene = nullptr;
+    ptk::dev_free(s->ctx, r.mem);
+    s->bytes_in_use -= r.bytes;
+    r.bytes = 0;
+}
+
+int admit(pt_server* s, Request& r, double now) {
+    pt_ctx* ctx = s->ctx;
+    int st = pt_scene_from_json(ctx, r.json.data(), r.json.size(), &r.scene);
+    if (st != PT_OK) return fail(s, st, std::string("request scene: ") + pt_last_error(ctx));
+    st = pt_scene_camera(r.scene, 1, &r.pc);
+    if (st != PT_OK) return fail(s, st, std::string("request camera: ") + pt_last_error(ctx));
+    r.pc.resolution[0] = r.W;
+    r.pc.resolution[1] = r.H;
+    r.pc.pathLength = r.path_length;
+    const size_t P = (size_t)r.P;
+    st = ptk::dev_alloc(ctx, r.mem, P * 28 + 3 * 256);
+    if (st != PT_OK) return fail(s, st, ctx->err);
+    char* p = static_cast<char*>(r.mem.p);
+    auto carve = [&](size_t n) {
+        char* q = p;
+        p += (n + 255) / 256 * 256;
+        return q;
+    };
+    r.sum = reinterpret_cast<float*>(carve(3 * P * 4));
+    r.sumsq = reinterpret_cast<float*>(carve(3 * P * 4));
+    r.nsamp = reinterpret_cast<int*>(carve(P * 4));
+    hipError_t e = hipMemsetAsync(r.mem.p, 0, r.mem.bytes, ctx->stream);
+    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "request buffers"), ctx->err);
+    // build the extended scene (BVH upload) now, so its bytes are known
+    st = ptk::ext_scene_build(ctx, r.scene);
+    if (st != PT_OK) return fail(s, st, ctx->err);
+    int64_t sb = 0;
+    (void)pt_scene_device_bytes(r.scene, &sb);
+    r.bytes = (int64_t)r.mem.bytes + sb;
+    s->bytes_in_use += r.bytes;
+    r.state = PT_REQ_ACTIVE;
+    r.t_admit = now;
+    return PT_OK;
+}
+
+// Samples still needed for the target, from rel ~ 1/sqrt(spp), and at least
+// up to the minimum before a convergence decision.
+int remaining_spp(const Request& r, int min_spp) {
+    const int left = r.spp_max - r.spp;
+    if (r.spp < 2 || !std::isfinite(r.rel)) return left;
+    double need = (double)r.spp * ((r.rel / r.target) * (r.rel / r.target) - 1.0);
+    need = std::max(need, (double)(min_spp - r.spp));
+    return (int)std::min<double>(left, std::max(1.0, std::ceil(need)));
+}
+
+double remaining_cost(const pt_server* s, const Request& r) {
+    return (double)remaining_spp(r, s->cfg.min_spp) * r.P * (s->sec_per_path > 0.0 ? s->sec_per_path : 1e-9);
+}
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_server_create(pt_ctx* ctx, const pt_server_config* cfg, pt_server** out) {
+    if (!ctx || !out) return PT_ERR_INVALID_ARG;
+    *out = nullptr;
+    pt_server* s = new (std::nothrow) pt_server();
+    if (!s) return PT_ERR_INVALID_ARG;
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
+    if (s->cfg.max_active <= 0) s->cfg.max_active = 16;
+    if (s->cfg.min_spp <= 0) s->cfg.min_spp = 8;
+    if (s->cfg.policy != PT_SERVE_FIFO) s->cfg.policy = PT_SERVE_SRPT;
+    s->t0 = Clock::now();
+    (void)hipSetDevice(ctx->device);
+    if (hipMalloc(&s->d_rel, kRelLines * kRelStride * sizeof(double)) != hipSuccess) {
+        delete s;
+        return ptk::hip_fail(ctx, hipGetLastError(), "pt_server_create");
+    }
+    *out = s;
+    return PT_OK;
+}
+
+void pt_server_destroy(pt_server* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (auto& kv : s->reqs) release_request(s, *kv.second);
+    if (s->d_rel) (void)hipFree(s->d_rel);
+    delete s;
+}
+
+const char* pt_server_last_error(const pt_server* s) { return s ? s->err.c_str() : ""; }
+
+double pt_server_now(const pt_server* s) { return s ? now_s(s) : 0.0; }
+
+pt_status pt_server_submit(pt_server* s, const pt_request_desc* d, double t_arrive, int32_t* id) {
+    if (!s || !d || !d->json || !id) return PT_ERR_INVALID_ARG;
+    if (d->width <= 0 || d->height <= 0 || (int64_t)d->width * d->height > (int64_t)1 << 26 || d->spp_max <= 0 ||
+        d->path_length <= 0 || !(d->target_rel > 0.0f))
+        return fail(s, PT_ERR_INVALID_ARG, "pt_server_submit: bad request");
+    auto r = std::make_unique<Request>();
+    r->id = s->next_id++;
+    r->json.assign(d->json, d->len);
+    r->W = d->width;
+    r->H = d->height;
+    r->P = d->width * d->height;
+    r->spp_max = d->spp_max;
+    r->path_length = d->path_length;
+    r->target = d->target_rel;
+    r->t_arrive = t_arrive;
+    *id = r->id;
+    s->waiting.push_back(r->id);
+    s->reqs[r->id] = std::move(r);
+    return PT_OK;
+}
+
+pt_status pt_server_step(pt_server* s, pt_request_status* stepped) {
+    if (!s) return PT_ERR_INVALID_ARG;
+    pt_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (stepped) {
+        std::memset(stepped, 0, sizeof *stepped);
+        stepped->id = -1;
+    }
+    // admission in arrival order, within the request count and HBM budget
+    while (!s->waiting.empty() && (int)s->active.size() < s->cfg.max_active) {
+        Request& r = *s->reqs[s->waiting.front()];
+        if (!s->active.empty() && s->bytes_in_use + estimate_bytes(r) > s->cfg.hbm_budget) break;
+        int st = admit(s, r, now_s(s));
+        if (st != PT_OK) {
+            s->waiting.pop_front();
+            r.state = PT_REQ_FAILED;
+            r.t_done = now_s(s);
+            release_request(s, r);
+            if (stepped) (void)pt_server_status(s, r.id, stepped);
+            return st;
+        }
+        s->waiting.pop_front();
+        s->active.push_back(r.id);
+    }
+    if (s->active.empty()) return PT_SERVE_IDLE;
+
+    // choose the request
+    int pick = s->active.front();
+    if (s->cfg.policy == PT_SERVE_SRPT) {
+        double best = INFINITY;
+        for (int id : s->active) {
+            double c = remaining_cost(s, *s->reqs[id]);
+            if (c < best) {
+                best = c;
+                pick = id;
+            }
+        }
+    }
+    Request& r = *s->reqs[pick];
+
+    // quantum: min_spp samples first, then the estimated remaining samples (5 %
+    // margin against the noise of the estimate), at most a full wave
+    const int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
+    int want = r.spp == 0 ? s->cfg.min_spp : (int)std::ceil(1.05 * remaining_spp(r, s->cfg.min_spp));
+    want = std::min(want, r.spp_max - r.spp);
+    const int q = std::max(1, std::min(cap / std::max(1, r.P), want));
+
+    pt_counters c0{}, c1{};
+    (void)pt_counters_get(ctx, &c0);
+    const auto t_start = Clock::now();
+    r.pc.samplesPerFrame = q;
+    r.pc.frame = r.spp + q;
+    r.pc.currentSamples = r.spp + q;
+    int st = ptk::ext_dispatch_moments(ctx, r.scene, &r.pc, r.sum, r.sumsq, r.nsamp);
+    if (st != PT_OK) return fail(s, st, ctx->err);
+    hipError_t e = hipMemsetAsync(s->d_rel, 0, kRelLines * kRelStride * sizeof(double), ctx->stream);
+    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
+    st = ptk::ext_rel_error(ctx, r.P, r.sum, r.sumsq, r.nsamp, s->d_rel);
+    if (st != PT_OK) return fail(s, st, ctx->err);
+    std::vector<double> lines(kRelLines * kRelStride);
+    e = hipMemcpyAsync(lines.data(), s->d_rel, lines.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
+    const double dt = std::chrono::duration<double>(Clock::now() - t_start).count();
+    (void)pt_counters_get(ctx, &c1);
+    double v = 0.0, m = 0.0;
+    for (int l = 0; l < kRelLines; ++l) {
+        v += lines[(size_t)l * kRelStride];
+        m += lines[(size_t)l * kRelStride + 1];
+    }
+    r.spp += q;
+    r.rel = m > 0.0 ? std::sqrt(v / m) : INFINITY;
+    r.gpu_s += dt;
+    r.segments += c1.segments - c0.segments;
+    const double paths = (double)q * r.P;
+    const double spp_cost = dt / paths;
+    s->sec_per_path = s->sec_per_path > 0.0 ? 0.8 * s->sec_per_path + 0.2 * spp_cost : spp_cost;
+    if ((r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max) {
+        r.state = PT_REQ_DONE;
+        r.t_done = now_s(s);
+        s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
+    }
+    if (stepped) (void)pt_server_status(s, r.id, stepped);
+    return PT_OK;
+}
+
+pt_status pt_server_status(pt_server* s, int32_t id, pt_request_status* out) {
+    if (!s || !out) return PT_ERR_INVALID_ARG;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(s, PT_ERR_INVALID_ARG, "pt_server_status: unknown request");
+    const Request& r = *it->second;
+    std::memset(out, 0, sizeof *out);
+    out->id = r.id;
+    out->state = r.state;
+    out->spp = r.spp;
+    out->converged = r.spp >= s->cfg.min_spp && r.rel <= r.target;
+    out->rel_error = std::isfinite(r.rel) ? r.rel : -1.0;
+    out->t_arrive = r.t_arrive;
+    out->t_admit = r.t_admit;
+    out->t_done = r.t_done;
+    out->gpu_seconds = r.gpu_s;
+    out->segments = r.segments;
+    out->width = r.W;
+    out->height = r.H;
+    return PT_OK;
+}
+
+pt_status pt_server_read(pt_server* s, int32_t id, float* host_rgba, size_t n_floats) {
+    if (!s || !host_rgba) return PT_ERR_INVALID_ARG;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(s, PT_ERR_INVALID_ARG, "pt_server_read: unknown request");
+    Request& r = *it->second;
+    if (!r.mem.p) return fail(s, PT_ERR_STATE, "pt_server_read: request not admitted or released");
+    if (n_floats < (size_t)r.P * 4) return fail(s, PT_ERR_CAPACITY, "pt_server_read: buffer too small");
+    const size_t P = (size_t)r.P;
+    std::vector<float> sum(3 * P);
+    std::vector<int> n(P);
+    pt_ctx* ctx = s->ctx;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(sum.data(), r.sum, 3 * P * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(n.data(), r.nsamp, P * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_read"), ctx->err);
+    // the texel units of k_ext_finish: mean per-sample XYZ x apertureSize^2 x ISO
+    const float ex = r.pc.apertureSize * r.pc.apertureSize * (float)r.pc.ISO;
+    for (size_t i = 0; i < P; ++i) {
+        const float inv = n[i] > 0 ? ex / (float)n[i] : 0.0f;
+        host_rgba[4 * i + 0] = sum[i] * inv;
+        host_rgba[4 * i + 1] = sum[P + i] * inv;
+        host_rgba[4 * i + 2] = sum[2 * P + i] * inv;
+        host_rgba[4 * i + 3] = 1.0f;
+    }
+    return PT_OK;
+}
+
+pt_status pt_server_release(pt_server* s, int32_t id) {
+    if (!s) return PT_ERR_INVALID_ARG;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(s, PT_ERR_INVALID_ARG, "pt_server_release: unknown request");
+    Request& r = *it->second;
+    if (r.state == PT_REQ_ACTIVE) return fail(s, PT_ERR_STATE, "pt_server_release: request still active");
+    if (r.state == PT_REQ_WAITING) s->waiting.erase(std::find(s->waiting.begin(), s->waiting.end(), id));
+    (void)hipStreamSynchronize(s->ctx->stream);
+    release_request(s, r);
+    s->reqs.erase(it);
+    return PT_OK;
+}
+
+pt_status pt_server_counts(const pt_server* s, int32_t* waiting, int32_t* active, int64_t* bytes_in_use) {
+    if (!s) return PT_ERR_INVALID_ARG;
+    if (waiting) *waiting = (int32_t)s->waiting.size();
+    if (active) *active = (int32_t)s->active.size();
+    if (bytes_in_use) *bytes_in_use = s->bytes_in_use;
+    return PT_OK;
+}
+
+}  // extern "C"

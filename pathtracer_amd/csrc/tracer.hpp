@@ -169,6 +169,13 @@ void ref_wave_release(pt_ctx* ctx);
 int ext_scene_build(pt_ctx* ctx, pt_scene* s);
 void ext_scene_release(pt_ctx* ctx, pt_scene* s);
 int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc);
+// The samples [frame - spf, frame) of every pixel added to a request's
+// per-pixel moments (sum, sumsq: 3 x P floats, channel-major; nsamp: P ints).
+int ext_dispatch_moments(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* sum, float* sumsq,
+                         int* nsamp);
+// Adds (sum over pixels and channels of variance of the mean, sum of mean^2)
+// into d_out: 64 lines of 32 doubles, slots 0 and 1 of each line.
+int ext_rel_error(pt_ctx* ctx, int P, const float* sum, const float* sumsq, const int* nsamp, double* d_out);
 void ext_wave_release(pt_ctx* ctx);
 
 // Display resolve and diagnostics (resolve.hip).

@@ -11,7 +11,7 @@ import pathtracer_amd as pt
 from tests.conftest import ROOT
 
 HEADER = (ROOT / "include" / "pathtracer.h").read_text()
-DECLARED = sorted(set(re.findall(r"^\s*(?:pt_status|void|const char\*)\s+(pt_\w+)\s*\(", HEADER, re.M)))
+DECLARED = sorted(set(re.findall(r"^\s*(?:pt_status|void|double|const char\*)\s+(pt_\w+)\s*\(", HEADER, re.M)))
 
 
 def test_header_declarations_found():
@@ -36,6 +36,10 @@ def test_struct_layouts():
     assert pt.PushConstants.cameraAngle.offset == 32
     assert pt.PushConstants.tonemap.offset == 84
     assert pt.PackedScene.cie.offset == 4 * (7 + 1024 + 768 + 783 + 128 + 64)
+    # serving structs (no reference counterpart): C layouts on x86-64
+    assert C.sizeof(pt.ServerConfig) == 24
+    assert C.sizeof(pt.RequestDesc) == 40 and pt.RequestDesc.target_rel.offset == 32
+    assert C.sizeof(pt.RequestStatus) == 72 and pt.RequestStatus.rel_error.offset == 24
 
 
 def test_argument_validation_without_device():
@@ -54,3 +58,27 @@ def test_create_reports_missing_device_or_succeeds():
     else:
         assert st == pt.PT_ERR_NO_DEVICE
         assert b"no HIP device" in pt.lib.pt_last_error_global()
+
+
+def test_struct_layouts_match_a_c_compiler(tmp_path):
+    """sizeof/offsetof from gcc on the header itself agree with the ctypes mirror."""
+    src = tmp_path / "layout.c"
+    src.write_text("""
+#include <stddef.h>
+#include <stdio.h>
+#include "pathtracer.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n",
+           sizeof(pt_packed_scene), sizeof(pt_push_constants), sizeof(pt_sdf_desc), sizeof(pt_config),
+           sizeof(pt_counters), sizeof(pt_kernel_stat), sizeof(pt_server_config), sizeof(pt_request_desc),
+           sizeof(pt_request_status), offsetof(pt_request_status, rel_error));
+    return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [C.sizeof(pt.PackedScene), C.sizeof(pt.PushConstants), C.sizeof(pt.SdfDesc), C.sizeof(pt.Config),
+            C.sizeof(pt.Counters), C.sizeof(pt.KernelStat), C.sizeof(pt.ServerConfig), C.sizeof(pt.RequestDesc),
+            C.sizeof(pt.RequestStatus), pt.RequestStatus.rel_error.offset]
+    assert got == want

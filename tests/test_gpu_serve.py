@@ -1,0 +1,126 @@
+"""Serving (pt_server_*) on the GPU: convergence detection, independence of
+how samples are split and interleaved, the relative-RMSE estimate against a
+converged render, scheduling order and HBM admission."""
+import json
+
+import numpy as np
+import pytest
+
+import pathtracer_amd as pt
+from pathtracer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def no_caustic_cornell(seed=0):
+    s = scenes.cornell_dict(4, seed)
+    s["material"][4] = {"type": "lambert", "gaussians": [[0.6, 600.0, 80.0]]}
+    return json.dumps(s)
+
+
+def run_until_done(srv, ids, max_steps=10000):
+    done = set()
+    for _ in range(max_steps):
+        st = srv.step()
+        if st is None:
+            break
+        if st.state == pt.REQ_DONE:
+            done.add(st.id)
+        if done >= set(ids):
+            break
+    return {i: srv.status(i) for i in ids}
+
+
+def test_request_converges_and_is_independent_of_quanta_and_mix(gpu_ctx):
+    """The per-pixel sums are added in sample order whatever the quantum size
+    and whatever else is traced in between: identical images."""
+    text = scenes.random_spheres(n=256, scene_seed=3, wavelengths=4, seed=11)
+    imgs = []
+    for wave, others in ((1 << 22, 0), (20000, 0), (1 << 22, 2)):
+        with pt.Context(0, wave_paths=wave) as ctx, pt.Server(ctx) as srv:
+            rid = srv.submit(text, 48, 32, spp_max=64, path_length=8, target=1e-6)  # runs to spp_max
+            extra = [srv.submit(scenes.cornell(4, seed=k), 24, 24, spp_max=16, path_length=6, target=1e-6)
+                     for k in range(others)]
+            st = run_until_done(srv, [rid] + extra)
+            assert st[rid].state == pt.REQ_DONE and st[rid].spp == 64 and not st[rid].converged
+            imgs.append(srv.read(rid))
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+    assert np.isfinite(imgs[0]).all() and imgs[0][..., 1].mean() > 0
+
+
+def test_mean_matches_the_renderer(gpu_ctx):
+    """Served mean == the running-mean texels of pt_dispatch with the same samples."""
+    text = no_caustic_cornell(seed=4)
+    with pt.Server(gpu_ctx) as srv:
+        rid = srv.submit(text, 40, 40, spp_max=32, path_length=8, target=1e-6)
+        run_until_done(srv, [rid])
+        served = srv.read(rid)[..., :3].astype(np.float64)
+    sc = gpu_ctx.scene_from_json(text)
+    r = pt.Renderer(gpu_ctx, sc, 40, 40, spf=32, path_length=8, mode=pt.MODE_EXT)
+    r.render_frame()
+    ref = r.acc.read()[..., :3].astype(np.float64)
+    np.testing.assert_allclose(served, ref, rtol=2e-5, atol=1e-7)
+
+
+def test_relative_error_estimate_tracks_the_converged_render(gpu_ctx):
+    """Stopping at a 3 % estimate gives an image whose actual relative RMSE
+    against a 64x longer render is close to 3 %."""
+    text = no_caustic_cornell(seed=6)
+    W = H = 48
+    with pt.Server(gpu_ctx) as srv:
+        a = srv.submit(text, W, H, spp_max=4096, path_length=8, target=0.03)
+        st = run_until_done(srv, [a])[a]
+        # converged without overshooting far past the target
+        assert st.converged and 0.025 <= st.rel_error <= 0.03
+        img = srv.read(a)[..., :3].astype(np.float64)
+        b = srv.submit(text, W, H, spp_max=64 * st.spp, path_length=8, target=1e-6)
+        run_until_done(srv, [b])
+        ref = srv.read(b)[..., :3].astype(np.float64)
+    # the reference shares the first st.spp samples: compare against its other samples only
+    n = 64 * st.spp
+    ref_other = (ref * n - img * st.spp) / (n - st.spp)
+    actual = np.sqrt(((img - ref_other) ** 2).sum()) / np.sqrt((ref_other ** 2).sum())
+    print(f"\nspp {st.spp}: estimate {st.rel_error:.4f}, actual {actual:.4f}")
+    assert 0.6 * st.rel_error <= actual <= 1.5 * st.rel_error
+
+
+def test_srpt_finishes_small_requests_first(gpu_ctx):
+    big = scenes.random_spheres(n=1024, scene_seed=2, wavelengths=4, seed=1)
+    small = no_caustic_cornell(seed=2)
+    order = {}
+    for policy in (pt.SERVE_FIFO, pt.SERVE_SRPT):
+        with pt.Server(gpu_ctx, policy=policy) as srv:
+            ids = [srv.submit(big, 256, 256, spp_max=256, path_length=8, target=1e-6)]
+            ids += [srv.submit(small, 32, 32, spp_max=64, path_length=8, target=1e-6) for _ in range(3)]
+            st = run_until_done(srv, ids)
+            order[policy] = sorted(ids, key=lambda i: st[i].t_done)
+            big_id = ids[0]
+    assert order[pt.SERVE_FIFO][0] == big_id  # one request at a time in arrival order
+    assert order[pt.SERVE_SRPT][-1] == big_id  # the small ones overtake it
+
+
+def test_hbm_budget_limits_admission_and_release_frees(gpu_ctx):
+    text = scenes.random_spheres(n=64, scene_seed=5, wavelengths=4, seed=2)
+    # a 4096-path wave traces one sample per pixel of a 64 x 64 request per step
+    with pt.Context(0, wave_paths=4096) as ctx, pt.Server(ctx, hbm_budget=1) as srv:  # one request fits at a time
+        ids = [srv.submit(text, 64, 64, spp_max=8, path_length=6, target=1e-6) for _ in range(3)]
+        st = srv.step()
+        assert st.id == ids[0] and st.spp == 1
+        waiting, active, used = srv.counts()
+        assert active == 1 and waiting == 2 and used > 0
+        run_until_done(srv, ids)
+        for i in ids:
+            assert srv.status(i).spp == 8
+            srv.release(i)
+        assert srv.counts() == (0, 0, 0)
+
+
+def test_bad_requests_are_rejected(gpu_ctx):
+    with pt.Server(gpu_ctx) as srv:
+        with pytest.raises(pt.PTError):
+            srv.submit(scenes.cornell(4), 0, 16, spp_max=4, path_length=4)
+        rid = srv.submit('{"camera": 1}', 16, 16, spp_max=4, path_length=4)
+        with pytest.raises(pt.PTError):
+            srv.step()
+        assert srv.status(rid).state == pt.REQ_FAILED
+        assert srv.step() is None
