@@ -63,6 +63,11 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--serve", type=int, default=0, metavar="N",
+                    help="serving workload instead: requests 0..N-1 of the serving stream (256 = the full workload)")
+    ap.add_argument("--serve-target", type=float, default=0.01, help="relative RMSE at which a request is done")
+    ap.add_argument("--serve-load", type=float, default=0.85,
+                    help="utilisation of the one-request-at-a-time tracer that sets the Poisson arrival rate")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run --steps device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -123,6 +128,118 @@ def cpu_baseline(args, scene_text: str):
     }
 
 
+def percentile(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def serve_bench(args):
+    """Serving workload: requests 0..N-1 of scenes.serving_request.
+
+    1. Each request's service time alone: all requests traced one at a time to
+       completion (FIFO server, all queued at once).
+    2. Poisson arrivals (seeded) at the rate that loads that one-at-a-time
+       tracer to --serve-load.
+    3. The same arrivals served in real time by the SRPT server; per request the
+       time from arrival to its relative-RMSE target (or to its sample budget).
+    4. The one-at-a-time (FIFO) response times for the same arrivals, from the
+       service times of step 1 (a FIFO queue is exact given service times).
+    Prints one JSON line."""
+    import numpy as np
+    import pathtracer_amd as pt
+    from pathtracer_amd import scenes
+
+    reqs = [scenes.serving_request(i) for i in range(args.serve)]
+    ctx = pt.Context(0)
+    t_start = time.perf_counter()
+
+    def submit(srv, r, t):
+        return srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target, t_arrive=t)
+
+    # 1. service times alone
+    service = {}
+    with pt.Server(ctx, policy=pt.SERVE_FIFO, max_active=1) as srv:
+        ids = {submit(srv, r, 0.0): r["id"] for r in reqs}
+        while len(service) < len(reqs):
+            sts = srv.step()
+            if sts is None:
+                break
+            for st in sts:
+                if st.state in (pt.REQ_DONE, pt.REQ_FAILED):
+                    service[ids[st.id]] = st.as_dict()
+                    srv.release(st.id)
+    s_alone = np.array([service[r["id"]]["gpu_seconds"] for r in reqs])
+    alone_converged = sum(service[r["id"]]["converged"] for r in reqs)
+
+    # 2. arrivals
+    rate = args.serve_load / s_alone.mean()
+    gaps = np.random.default_rng(20261202).exponential(1.0 / rate, len(reqs))
+    offsets = np.cumsum(gaps) - gaps[0]
+
+    # 3. SRPT in real time
+    done = {}
+    segments = 0
+    with pt.Server(ctx, policy=pt.SERVE_SRPT) as srv:
+        base = srv.now()
+        arrive = base + offsets
+        ids, nxt = {}, 0
+        while len(done) < len(reqs):
+            now = srv.now()
+            while nxt < len(reqs) and arrive[nxt] <= now:
+                ids[submit(srv, reqs[nxt], float(arrive[nxt]))] = reqs[nxt]["id"]
+                nxt += 1
+            sts = srv.step()
+            if sts is None:
+                if nxt < len(reqs):
+                    time.sleep(max(0.0, min(0.002, arrive[nxt] - srv.now())))
+                continue
+            for st in sts:
+                if st.state in (pt.REQ_DONE, pt.REQ_FAILED):
+                    d = st.as_dict()
+                    d["response"] = d["t_done"] - d["t_arrive"]
+                    done[ids[st.id]] = d
+                    segments += d["segments"]
+                    srv.release(st.id)
+        makespan = srv.now() - base
+
+    # 4. one request at a time, same arrivals
+    fifo_resp, c = [], 0.0
+    for k in range(len(reqs)):
+        c = max(offsets[k], c) + s_alone[k]
+        fifo_resp.append(c - offsets[k])
+    fifo_resp = np.array(fifo_resp)
+    conv = [r["id"] for r in reqs if done[r["id"]]["converged"]]
+    resp = np.array([done[r["id"]]["response"] for r in reqs])
+    resp_conv = np.array([done[i]["response"] for i in conv])
+    fifo_conv = np.array([fifo_resp[i] for i in range(len(reqs)) if service[reqs[i]["id"]]["converged"]])
+    line = {
+        "metric": METRIC,
+        "value": percentile(resp_conv, 50),
+        "unit": "s (p50 time-to-RMSE<=1%, SRPT server)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "data": "synthetic",
+        "dtype": "fp32",
+        "config": {"workload": f"serving stream requests 0..{len(reqs) - 1}: 50% Cornell+N random spheres "
+                               "(N log-uniform 1..4096, depth 12) / 50% mandelbulb+menger (parameters x U[0.9,1.1], "
+                               "depth 8); resolution U{256^2, 512^2, 1024x768, 1920x1080}; "
+                               "spp budget log-uniform 16..1024; Poisson arrivals",
+                   "requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
+                   "arrival_rate_per_s": rate},
+        "srpt": {"p50_time_to_rmse_s": percentile(resp_conv, 50), "p99_time_to_rmse_s": percentile(resp_conv, 99),
+                 "converged": len(conv), "p50_response_s": percentile(resp, 50),
+                 "p99_response_s": percentile(resp, 99), "makespan_s": makespan,
+                 "segments_per_s": segments / makespan},
+        "one_at_a_time": {"p50_time_to_rmse_s": percentile(fifo_conv, 50),
+                          "p99_time_to_rmse_s": percentile(fifo_conv, 99), "converged": int(alone_converged),
+                          "p50_response_s": percentile(fifo_resp, 50), "p99_response_s": percentile(fifo_resp, 99),
+                          "mean_service_s": float(s_alone.mean())},
+        "bench_seconds": time.perf_counter() - t_start,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -131,6 +248,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe, args.steps)
+        return
+    if args.serve:
+        serve_bench(args)
         return
     import pathtracer_amd as pt
     from pathtracer_amd import scenes

@@ -256,9 +256,11 @@ pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* 
  * sqrt(sum of mean^2) over pixels and XYZ channels, reaches its target or its
  * sample budget is spent. Scenes, BVHs and per-pixel moments of admitted
  * requests stay in device memory; admission stops at the server's HBM budget
- * and request count. One step traces one quantum of samples of one request,
- * chosen by the policy: min_spp samples per pixel first, then the estimated
- * remaining ones (from error ~ 1/sqrt(samples)), at most a full wave. Times are seconds of the
+ * and request count. One step traces one wave (continuous batching): every
+ * active request that fits, in policy order, contributes a quantum of samples
+ * per pixel, min_spp first, then its estimated remaining ones (from error ~
+ * 1/sqrt(samples)); each request's samples are summed in sample order, so its
+ * result is the same alone and in the mix. Times are seconds of the
  * server's monotonic clock (pt_server_now). A server is used by one host
  * thread at a time. */
 typedef struct pt_server pt_server;
@@ -295,7 +297,7 @@ typedef struct pt_request_status {
     int32_t width, height;
     double rel_error;      /* last estimate, -1 before two samples */
     double t_arrive, t_admit, t_done;
-    double gpu_seconds;    /* wall time of the steps that traced it */
+    double gpu_seconds;    /* wall time of its admission (scene parse, BVH build) and of the steps that traced it */
     uint64_t segments;
 } pt_request_status;
 
@@ -305,10 +307,11 @@ const char* pt_server_last_error(const pt_server* server);
 double pt_server_now(const pt_server* server);
 /* Queues a request that arrived at t_arrive (server clock); *id receives its id. */
 pt_status pt_server_submit(pt_server* server, const pt_request_desc* req, double t_arrive, int32_t* id);
-/* Admits what fits, then traces one quantum of the chosen request and updates
- * its estimate; *stepped receives that request's status. Returns PT_OK,
- * PT_SERVE_IDLE when nothing is admitted, or an error. */
-pt_status pt_server_step(pt_server* server, pt_request_status* stepped);
+/* Admits what fits, then traces one wave holding a quantum of samples of each
+ * active request that fits (in policy order) and updates their estimates;
+ * up to `cap` statuses of the traced requests go to stepped[], their number to
+ * *n_stepped. Returns PT_OK, PT_SERVE_IDLE when nothing is admitted, or an error. */
+pt_status pt_server_step(pt_server* server, pt_request_status* stepped, int32_t cap, int32_t* n_stepped);
 pt_status pt_server_status(pt_server* server, int32_t id, pt_request_status* out);
 /* The request's mean XYZ image as RGBA32F texels in the units of pt_accum. */
 pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t n_floats);

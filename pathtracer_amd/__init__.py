@@ -214,7 +214,7 @@ SIGNATURES = {
     "pt_server_last_error": (C.c_char_p, [_P]),
     "pt_server_now": (C.c_double, [_P]),
     "pt_server_submit": (C.c_int, [_P, C.POINTER(RequestDesc), C.c_double, C.POINTER(C.c_int32)]),
-    "pt_server_step": (C.c_int, [_P, C.POINTER(RequestStatus)]),
+    "pt_server_step": (C.c_int, [_P, C.POINTER(RequestStatus), C.c_int32, C.POINTER(C.c_int32)]),
     "pt_server_status": (C.c_int, [_P, C.c_int32, C.POINTER(RequestStatus)]),
     "pt_server_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
     "pt_server_release": (C.c_int, [_P, C.c_int32]),
@@ -563,11 +563,12 @@ class Server:
                                       C.byref(rid)))
         return rid.value
 
-    def step(self) -> RequestStatus | None:
-        """One quantum; None when nothing is admitted."""
-        st = RequestStatus()
-        rc = self._ck(lib.pt_server_step(self.h, C.byref(st)))
-        return None if rc == SERVE_IDLE else st
+    def step(self) -> list[RequestStatus] | None:
+        """One wave; the statuses of the requests it traced, None when nothing is admitted."""
+        arr = (RequestStatus * 64)()
+        n = C.c_int32()
+        rc = self._ck(lib.pt_server_step(self.h, arr, 64, C.byref(n)))
+        return None if rc == SERVE_IDLE else [arr[i] for i in range(n.value)]
 
     def status(self, rid: int) -> RequestStatus:
         st = RequestStatus()

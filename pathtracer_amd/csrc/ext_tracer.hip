@@ -136,6 +136,22 @@ struct ExtWave {
     int cap;
 };
 
+// Continuous batching: one wave holds the paths of several requests. A path's
+// request slot is aux.w (a shadow record's is s_xyz.w); the slot indexes the
+// batch's scene and frame tables. Kernels instantiated with MULTI = false read
+// the one scene and frame of their arguments instead.
+struct ExtBatch {
+    const ExtSceneDev* scenes = nullptr;
+    const ExtFrame* frames = nullptr;
+};
+
+// The scene / frame of request slot rid: the batch's tables with MULTI, the
+// kernel's own arguments otherwise.
+template <bool MULTI>
+__device__ __forceinline__ const ExtSceneDev& scene_of(const ExtSceneDev& S0, const ExtBatch& B, uint32_t rid) {
+    return MULTI ? B.scenes[rid] : S0;
+}
+
 static __constant__ float c_cie[PT_CIE_SIZE] = {
 #include "cie1931_2deg.inc"
 };
@@ -640,80 +656,114 @@ __device__ __forceinline__ float bk7_index(float lnm) {
     return sqrtf(n2);
 }
 
+// Camera ray of sample k of pixel `pix` of frame F (scene S) into wave slot i,
+// request slot rid.
+__device__ __forceinline__ void gen_ray(const ExtSceneDev& S, const ExtFrame& F, int pix, int k, ExtWave& Wv, int i,
+                                        uint32_t rid) {
+    int gx = pix % F.W, gy = pix / F.W;
+    uint32_t sample = (uint32_t)(F.frame - F.spf + k);
+    uint32_t key = path_key(S.seed, (uint32_t)pix, sample);
+    float ju = rnd(key, kCamBounce, 0), jv = rnd(key, kCamBounce, 1);
+    float uvx = (2.0f * ((float)gx + ju) - (float)F.W) / (float)F.H;
+    float uvy = (2.0f * ((float)(F.H - gy) - jv) - (float)F.H) / (float)F.H;
+    float l0 = S.lmin + S.lrange * rnd(key, kCamBounce, 4);
+    f3 o, d;
+    if (S.camera == 1) {
+        float h = 0.5f * F.camSize;
+        o = F.camPos;
+        d = normalize(vmul(mk3(uvx * h, uvy * h, F.apDist), F.camM));
+    } else {
+        // The reference's camera (sensor, aperture disk, BK7 bi-convex lens) at the hero wavelength.
+        float sc = -F.camSize * 0.5f;
+        o = F.camPos + vmul(mk3(uvx * sc, uvy * sc, 0.0f), F.camM);
+        float ux = rnd(key, kCamBounce, 2), uy = rnd(key, kCamBounce, 3);
+        float phi = 2.0f * kPiF * uy, dr = sqrtf(ux);
+        float ap = 0.5f * F.apSize;
+        f3 onAp = F.camPos + vmul(mk3(ap * dr * __cosf(phi), ap * dr * __sinf(phi), F.apDist), F.camM);
+        d = normalize(onAp - o);
+        float l = l0;
+        for (int pass = 0; pass < 2; ++pass) {
+            // the two spherical caps (radius 2f) of the lens
+            float best = 1e6f;
+            f3 nb = mk3(0.0f, 0.0f, 0.0f);
+            int outside = 1;
+            for (int sidx = 0; sidx < 2; ++sidx) {
+                bool first = sidx == 0;
+                float off = F.lensTwoF - F.lensHalf;
+                f3 lo = vmul(o - F.lensPos, F.lensM);
+                if (first) lo.x += F.lensSlicePos - F.lensHalf - off;
+                else lo.x -= F.lensSlicePos - F.lensHalf - off;
+                f3 dl = vmul(d, F.lensM);
+                float b = 2.0f * dot(dl, lo), c = dot(lo, lo) - F.lensTwoF * F.lensTwoF;
+                float disc = b * b - 4.0f * c;
+                if (disc < 0.0f) continue;
+                float sq = sqrtf(disc);
+                float t1 = (-b - sq) * 0.5f, t2 = (-b + sq) * 0.5f;
+                if (first) {
+                    t1 = (fmaf(dl.x, t1, lo.x) > -off) ? 1e6f : t1;
+                    t2 = (fmaf(dl.x, t2, lo.x) > -off) ? 1e6f : t2;
+                } else {
+                    t1 = (fmaf(dl.x, t1, lo.x) < off) ? 1e6f : t1;
+                    t2 = (fmaf(dl.x, t2, lo.x) < off) ? 1e6f : t2;
+                }
+                float t = (t1 > 0.0f) ? t1 : 1e6f;
+                int out = 1;
+                if (t2 < t) { t = t2; out = -1; }
+                if (t < 1e-4f || !(t < best)) continue;
+                best = t;
+                nb = mvmul(F.lensM, normalize(fma3s(dl, t, lo) * (float)out));
+                outside = out;
+            }
+            float n1 = 1.0f, n2 = 1.0f;
+            if (outside == 1) n2 = bk7_index(l);
+            else n1 = bk7_index(l);
+            float n12 = n1 / n2;
+            l = l * n12;
+            o = fma3s(d, best, o);
+            d = refract(d, nb, n12);
+        }
+    }
+    Wv.org[i] = make_float4(o.x, o.y, o.z, 0.0f);
+    Wv.dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
+    Wv.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, l0);
+    Wv.aux[i] = make_uint4(key, 2u, 0u, rid);  // the camera ray counts as specular for emitter MIS
+}
+
 __global__ void __launch_bounds__(kBlk) k_ext_raygen(ExtSceneDev S, ExtFrame F, ExtWave Wv) {
     int n = F.pc * F.m;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)n);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        int pix = F.p0 + i % F.pc;
-        int k = F.k0 + i / F.pc;
-        int gx = pix % F.W, gy = pix / F.W;
-        uint32_t sample = (uint32_t)(F.frame - F.spf + k);
-        uint32_t key = path_key(S.seed, (uint32_t)pix, sample);
-        float ju = rnd(key, kCamBounce, 0), jv = rnd(key, kCamBounce, 1);
-        float uvx = (2.0f * ((float)gx + ju) - (float)F.W) / (float)F.H;
-        float uvy = (2.0f * ((float)(F.H - gy) - jv) - (float)F.H) / (float)F.H;
-        float l0 = S.lmin + S.lrange * rnd(key, kCamBounce, 4);
-        f3 o, d;
-        if (S.camera == 1) {
-            float h = 0.5f * F.camSize;
-            o = F.camPos;
-            d = normalize(vmul(mk3(uvx * h, uvy * h, F.apDist), F.camM));
-        } else {
-            // The reference's camera (sensor, aperture disk, BK7 bi-convex lens) at the hero wavelength.
-            float sc = -F.camSize * 0.5f;
-            o = F.camPos + vmul(mk3(uvx * sc, uvy * sc, 0.0f), F.camM);
-            float ux = rnd(key, kCamBounce, 2), uy = rnd(key, kCamBounce, 3);
-            float phi = 2.0f * kPiF * uy, dr = sqrtf(ux);
-            float ap = 0.5f * F.apSize;
-            f3 onAp = F.camPos + vmul(mk3(ap * dr * __cosf(phi), ap * dr * __sinf(phi), F.apDist), F.camM);
-            d = normalize(onAp - o);
-            float l = l0;
-            for (int pass = 0; pass < 2; ++pass) {
-                // the two spherical caps (radius 2f) of the lens
-                float best = 1e6f;
-                f3 nb = mk3(0.0f, 0.0f, 0.0f);
-                int outside = 1;
-                for (int sidx = 0; sidx < 2; ++sidx) {
-                    bool first = sidx == 0;
-                    float off = F.lensTwoF - F.lensHalf;
-                    f3 lo = vmul(o - F.lensPos, F.lensM);
-                    if (first) lo.x += F.lensSlicePos - F.lensHalf - off;
-                    else lo.x -= F.lensSlicePos - F.lensHalf - off;
-                    f3 dl = vmul(d, F.lensM);
-                    float b = 2.0f * dot(dl, lo), c = dot(lo, lo) - F.lensTwoF * F.lensTwoF;
-                    float disc = b * b - 4.0f * c;
-                    if (disc < 0.0f) continue;
-                    float sq = sqrtf(disc);
-                    float t1 = (-b - sq) * 0.5f, t2 = (-b + sq) * 0.5f;
-                    if (first) {
-                        t1 = (fmaf(dl.x, t1, lo.x) > -off) ? 1e6f : t1;
-                        t2 = (fmaf(dl.x, t2, lo.x) > -off) ? 1e6f : t2;
-                    } else {
-                        t1 = (fmaf(dl.x, t1, lo.x) < off) ? 1e6f : t1;
-                        t2 = (fmaf(dl.x, t2, lo.x) < off) ? 1e6f : t2;
-                    }
-                    float t = (t1 > 0.0f) ? t1 : 1e6f;
-                    int out = 1;
-                    if (t2 < t) { t = t2; out = -1; }
-                    if (t < 1e-4f || !(t < best)) continue;
-                    best = t;
-                    nb = mvmul(F.lensM, normalize(fma3s(dl, t, lo) * (float)out));
-                    outside = out;
-                }
-                float n1 = 1.0f, n2 = 1.0f;
-                if (outside == 1) n2 = bk7_index(l);
-                else n1 = bk7_index(l);
-                float n12 = n1 / n2;
-                l = l * n12;
-                o = fma3s(d, best, o);
-                d = refract(d, nb, n12);
-            }
-        }
-        Wv.org[i] = make_float4(o.x, o.y, o.z, 0.0f);
-        Wv.dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
-        Wv.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, l0);
-        Wv.aux[i] = make_uint4(key, 2u, 0u, 0u);  // the camera ray counts as specular for emitter MIS
+        gen_ray(S, F, F.p0 + i % F.pc, F.k0 + i / F.pc, Wv, i, 0u);
+        Wv.q[0][i] = i;
+    }
+}
+
+// One segment of a batched wave: request slot rid, its pixels [p0, p0 + pc) x
+// samples [k0, k0 + m) of its frame, at wave slots [base, base + pc * m)
+// (sample-major), and pix_base = the sum of pc over the earlier segments.
+struct ExtSeg {
+    int rid, base, p0, pc, k0, m, pix_base, pad;
+};
+// Index of the segment holding `i` of the ascending starts start(0..n-1).
+template <typename Start>
+__device__ __forceinline__ int seg_find(int n, int i, Start start) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start(mid) <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const ExtSeg* segs, int nseg, int npaths,
+                                                            ExtWave Wv) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)npaths);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npaths; i += gridDim.x * blockDim.x) {
+        const ExtSeg& g = segs[seg_find(nseg, i, [&](int k) { return segs[k].base; })];
+        const int local = i - g.base;
+        gen_ray(B.scenes[g.rid], B.frames[g.rid], g.p0 + local % g.pc, g.k0 + local / g.pc, Wv, i, (uint32_t)g.rid);
         Wv.q[0][i] = i;
     }
 }
@@ -730,8 +780,8 @@ __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& W
 
 // Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
 // is also appended to its material queue (scenes without SDFs).
-template <int W, int STACK, typename SE, bool CLASSIFY>
-__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave Wv, int cur) {
+template <int W, int STACK, typename SE, bool CLASSIFY, bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, cur);
@@ -743,6 +793,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S, ExtWave W
         int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = idx < n;
         int p = active ? Wv.q[cur][idx] : 0;
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
         uint32_t ref = kMiss;
         if (active) {
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
@@ -900,8 +951,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
 // bounce launch lasts as long as its slowest ray's chain of dependent node
 // fetches; one launch instead of two pays that once per bounce. Scenes without
 // SDF objects only (their shadow rays also march the SDFs).
-template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S, ExtWave Wv, int cur, int shq) {
+template <int W, int STACK, typename SE, bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int ne = *qlen(Wv, cur), ns = *qlen(Wv, shq);
@@ -917,35 +968,39 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S, ExtWave Wv, i
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         const bool ext = idx < ne;
         int p = 0;
-        uint32_t ref = kMiss;
+        uint32_t ref = kMiss, rid = 0u;
         if (ext) {
             p = Wv.q[cur][idx];
+            if (MULTI) rid = Wv.aux[p].w;
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
             float t = 1e30f;
             scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, ev);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
         } else if (idx < n) {
             const int i = idx - ne;
-            float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+            float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
             f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
             float t = so.w;
             uint32_t sref = kMiss;
             if (!scene_trace<W, STACK, SE, true>(S, o, d, t, sref, stack, sv))
-                add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
+                add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
         }
-        classify_append(S, Wv, ext, p, ref);
+        classify_append(scene_of<MULTI>(S0, B, rid), Wv, ext, p, ref);
     }
     wave_add_u64(stat_slot(Wv.stats, 11), ev);
     wave_add_u64(stat_slot(Wv.stats, 12), sv);
 }
 
 // Sphere tracing of every SDF object of one kind against the current hit.
-template <int KIND>
-__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, int cur) {
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     unsigned steps = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
         f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
         float2 h = Wv.hit[p];
         float t = h.x;
@@ -965,7 +1020,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S, ExtWave Wv, i
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
 }
 
-__global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S, ExtWave Wv, int cur) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
@@ -973,6 +1029,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S, ExtWave Wv
         int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = idx < n;
         int p = active ? Wv.q[cur][idx] : 0;
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
         classify_append(S, Wv, active, p, active ? __float_as_uint(Wv.hit[p].y) : kMiss);
     }
 }
@@ -988,11 +1045,13 @@ __device__ __forceinline__ bool roulette(const ExtSceneDev& S, uint32_t key, int
 }
 
 // Escaped paths (environment) and emitter hits (weighted by MIS against NEE).
-__global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S, ExtWave Wv, int which) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int which) {
     const int n = *qlen(Wv, CNT_MAT + which);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + which], (unsigned long long)n);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.mq[which][idx];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
         f4 T = ld4(Wv.T, p);
         float l[4];
         wavelengths(S, Wv.acc[p].w, l);
@@ -1030,8 +1089,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S, ExtW
 }
 
 // Lambert vertices: next-event estimation (one shadow ray) + cosine sampling.
-__global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
-                                                            int bounce) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtFrame F0, ExtBatch B, ExtWave Wv,
+                                                            int cur, int bounce) {
     const int nxt = cur ^ 1;
     const int n = *qlen(Wv, CNT_MAT + Q_LAMBERT);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_LAMBERT], (unsigned long long)n);
@@ -1044,8 +1104,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
         bool cont = false, shadow = false;
         f3 sP = mk3(0, 0, 0), sD = sP, sC = sP;
         float smax = 0.0f;
+        uint32_t rid = 0u;
         if (active) {
             p = Wv.mq[Q_LAMBERT][idx];
+            uint4 aux = Wv.aux[p];
+            rid = aux.w;
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
+            const int maxDepth = MULTI ? B.frames[rid].maxDepth : F0.maxDepth;
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
@@ -1057,7 +1122,6 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
             f4 rho = f4{reflectance(M, l[0]), 0.0f, 0.0f, 0.0f};
             if (multi) rho = f4{rho.x, reflectance(M, l[1]), reflectance(M, l[2]), reflectance(M, l[3])};
             f4 T = ld4(Wv.T, p);
-            uint4 aux = Wv.aux[p];
             uint32_t key = aux.x;
             f3 P = sf.p + N * offset_scale(sf.p);
             if (S.nlight > 0) {
@@ -1119,12 +1183,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
             onb(N, tb, bb);
             f3 wo = tb * (rr * __cosf(phi)) + bb * (rr * __sinf(phi)) + N * z;
             T = T * rho;
-            if (bounce + 1 < F.maxDepth && z > 0.0f && roulette(S, key, bounce, T)) {
+            if (bounce + 1 < maxDepth && z > 0.0f && roulette(S, key, bounce, T)) {
                 cont = true;
                 Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
                 Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), 0u);
+                Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), aux.w);
             }
         }
         int* const ctr[2] = {qlen(Wv, shadow_cnt(bounce)), qlen(Wv, nxt)};
@@ -1133,7 +1197,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
         if (shadow) {
             Wv.s_org[slot[0]] = make_float4(sP.x, sP.y, sP.z, smax);
             Wv.s_dir[slot[0]] = make_float4(sD.x, sD.y, sD.z, __int_as_float(p));
-            Wv.s_xyz[slot[0]] = make_float4(sC.x, sC.y, sC.z, 0.0f);
+            Wv.s_xyz[slot[0]] = make_float4(sC.x, sC.y, sC.z, __uint_as_float(rid));
         }
         if (cont) Wv.q[nxt][slot[1]] = p;
     }
@@ -1141,8 +1205,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S, ExtFr
 
 // Smooth dielectric: Fresnel-weighted choice of reflection or refraction at the
 // hero wavelength; a dispersive interface keeps only the hero.
-__global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
-                                                               int bounce) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, ExtFrame F0, ExtBatch B, ExtWave Wv,
+                                                               int cur, int bounce) {
     const int nxt = cur ^ 1;
     const int n = *qlen(Wv, CNT_MAT + Q_DIELECTRIC);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_DIELECTRIC], (unsigned long long)n);
@@ -1155,11 +1220,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, Ex
         bool cont = false;
         if (active) {
             p = Wv.mq[Q_DIELECTRIC][idx];
+            uint4 aux = Wv.aux[p];
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
+            const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
             const ExtMat& M = S.mat[sf.mat];
-            uint4 aux = Wv.aux[p];
             uint32_t key = aux.x;
             float l0 = Wv.acc[p].w;
             f3 N = sf.n;
@@ -1190,12 +1257,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, Ex
                 T = f4{T.x * (float)S.n_lambda, 0.0f, 0.0f, 0.0f};
                 fl |= 1u;
             }
-            if (bounce + 1 < F.maxDepth && roulette(S, key, bounce, T)) {
+            if (bounce + 1 < maxDepth && roulette(S, key, bounce, T)) {
                 cont = true;
                 Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
                 Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, 0u);
+                Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, aux.w);
             }
         }
         int* const ctr[1] = {qlen(Wv, nxt)};
@@ -1206,8 +1273,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S, Ex
 }
 
 // Smooth conductor: mirror reflection weighted by the complex-index Fresnel term.
-__global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, ExtFrame F, ExtWave Wv, int cur,
-                                                              int bounce) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, ExtFrame F0, ExtBatch B, ExtWave Wv,
+                                                              int cur, int bounce) {
     const int nxt = cur ^ 1;
     const int n = *qlen(Wv, CNT_MAT + Q_CONDUCTOR);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + Q_CONDUCTOR], (unsigned long long)n);
@@ -1220,6 +1288,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, Ext
         bool cont = false;
         if (active) {
             p = Wv.mq[Q_CONDUCTOR][idx];
+            uint4 aux = Wv.aux[p];
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
+            const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
             f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
@@ -1230,13 +1301,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, Ext
             f4 T = ld4(Wv.T, p) * Fr;
             f3 wo = d + N * (2.0f * ci);
             f3 P = sf.p + N * offset_scale(sf.p);
-            uint4 aux = Wv.aux[p];
-            if (bounce + 1 < F.maxDepth && roulette(S, aux.x, bounce, T)) {
+            if (bounce + 1 < maxDepth && roulette(S, aux.x, bounce, T)) {
                 cont = true;
                 Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
                 Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, 0u);
+                Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, aux.w);
             }
         }
         int* const ctr[1] = {qlen(Wv, nxt)};
@@ -1248,22 +1318,23 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S, Ext
 
 // Shadow rays: any hit before tmax over the scene and the SDFs; unoccluded
 // contributions are added to the path.
-template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S, ExtWave Wv, int shq) {
+template <int W, int STACK, typename SE, bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0, visits = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+        float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
         f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
         float t = so.w;
         uint32_t ref = kMiss;
         bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack, visits);
         for (int s = 0; !blocked && s < S.nsdf; ++s)
             blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
-        if (!blocked) add_xyz(Wv, __float_as_int(sd.w), ld3(Wv.s_xyz, i));
+        if (!blocked) add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
     wave_add_u64(stat_slot(Wv.stats, 12), visits);
@@ -1399,6 +1470,42 @@ __global__ void __launch_bounds__(kBlk) k_ext_moments(ExtFrame F, ExtWave Wv, in
     wave_add_u64(stat_slot(Wv.stats, 4), nans);
 }
 
+// A request's per-pixel moment buffers (k_ext_moments layout).
+struct ExtMomentSink {
+    float* sum;
+    float* sumsq;
+    int* nsamp;
+    int P;
+    int pad;
+};
+
+// Batched serving sink: one thread per (segment, pixel) adds the pixel's
+// samples of the segment, in sample order, to its request's moments.
+__global__ void __launch_bounds__(kBlk) k_ext_moments_batch(const ExtSeg* segs, int nseg, const ExtMomentSink* sinks,
+                                                            int npix, ExtWave Wv) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned nans = 0;
+    if (t < npix) {
+        const ExtSeg& g = segs[seg_find(nseg, t, [&](int k) { return segs[k].pix_base; })];
+        const ExtMomentSink& K = sinks[g.rid];
+        const int j = t - g.pix_base, px = g.p0 + j, P = K.P;
+        float ax = K.sum[px], ay = K.sum[P + px], az = K.sum[2 * P + px];
+        float qx = K.sumsq[px], qy = K.sumsq[P + px], qz = K.sumsq[2 * P + px];
+        int n = K.nsamp[px];
+        for (int s = 0; s < g.m; ++s) {
+            float4 a = Wv.acc[g.base + s * g.pc + j];
+            if (a.x != a.x || a.y != a.y || a.z != a.z) { ++nans; continue; }
+            ax += a.x; ay += a.y; az += a.z;
+            qx += a.x * a.x; qy += a.y * a.y; qz += a.z * a.z;
+            ++n;
+        }
+        K.sum[px] = ax; K.sum[P + px] = ay; K.sum[2 * P + px] = az;
+        K.sumsq[px] = qx; K.sumsq[P + px] = qy; K.sumsq[2 * P + px] = qz;
+        K.nsamp[px] = n;
+    }
+    wave_add_u64(stat_slot(Wv.stats, 4), nans);
+}
+
 // Relative RMSE estimate of a request's mean image: over pixels and XYZ
 // channels, sum of (sample variance / n) and sum of mean^2. Per block, then
 // one double atomic per block into line blockIdx.x % 64 of `out` (2 doubles
@@ -1451,6 +1558,7 @@ struct ExtCtxState {
     DevAlloc mem;
     ExtWave wave{};
     int cu = 0;  // compute units of the context's device (grid sizing)
+    DevAlloc batch_mem;  // batched waves: scene, frame, segment and sink tables of up to kMaxBatch requests
 };
 
 static inline f3 v3(const float* a) { return mk3(a[0], a[1], a[2]); }
@@ -1747,14 +1855,28 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
 void ext_wave_release(pt_ctx* ctx) {
     ExtCtxState* st = static_cast<ExtCtxState*>(ctx->ext_state);
     if (!st) return;
+    dev_free(ctx, st->batch_mem);
     dev_free(ctx, st->mem);
     delete st;
     ctx->ext_state = nullptr;
 }
 
-template <int KIND>
-static void launch_march(pt_ctx* ctx, const char* name, const ExtSceneDev& S, ExtWave& Wv, int cur, int grid) {
-    launch(ctx, name, k_ext_march<KIND>, dim3(grid), dim3(kBlk), S, Wv, cur);
+// What the bounce kernels of one wave need to know about its scene(s).
+struct ExtPlan {
+    ExtSceneDev S0{};  // the wave's scene (batched: unused by the MULTI kernels)
+    ExtFrame F0{};
+    ExtBatch B{};      // batched: per-request scene and frame tables in device memory
+    bool multi = false;
+    bool any_sdf = false;
+    bool quad_bvh = false;  // a scene tests its parallelograms through a BVH
+    std::vector<int> sdf_kinds;
+    int width = 2, stack_need = 1;
+    bool small_nodes = true;
+};
+
+template <int KIND, bool MULTI>
+static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int grid) {
+    launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
 }
 
 // One bounce b (extension queue `cur`). Scenes without SDF objects trace the
@@ -1762,78 +1884,126 @@ static void launch_march(pt_ctx* ctx, const char* name, const ExtSceneDev& S, Ex
 // last bounce's shadow rays are traced by launch_tail. Scenes with SDF objects,
 // and persistent traversal, keep separate intersect / march / classify /
 // shadow kernels.
-template <int W, int STACK, typename SE>
-static void launch_bounce(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
-                          int grid) {
-    const ExtSceneDev& S = H.dev;
-    // Persistent traversal on request; it tests parallelograms directly, so
-    // scenes with a parallelogram BVH always take the one-ray-per-lane kernels.
-    const bool persistent = ctx->cfg.traversal == 1 && !S.quad_bvh;
-    const bool fused = !persistent && S.nsdf == 0;
+template <int W, int STACK, typename SE, bool MULTI>
+static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, int b, int grid) {
+    const ExtSceneDev& S = P.S0;
+    // Persistent traversal on request (one scene only); it tests parallelograms
+    // directly, so scenes with a parallelogram BVH take the one-ray-per-lane kernels.
+    const bool persistent = !MULTI && ctx->cfg.traversal == 1 && !P.quad_bvh;
+    const bool fused = !persistent && !P.any_sdf;
     launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1, shadow_cnt(b));
     if (fused) {
-        launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, shadow_cnt(b - 1));
-    } else if (S.nsdf > 0) {
+        launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur,
+               shadow_cnt(b - 1));
+    } else if (P.any_sdf) {
         if (persistent)
             launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         else
-            launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false>, dim3(grid), dim3(kBlk), S, Wv, cur);
-        for (int kind : H.sdf_kinds) {
+            launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
+                   Wv, cur);
+        for (int kind : P.sdf_kinds) {
             switch (kind) {
-                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB>(ctx, "k_ext_march_mandelbulb", S, Wv, cur, grid); break;
-                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER>(ctx, "k_ext_march_menger", S, Wv, cur, grid); break;
-                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB>(ctx, "k_ext_march_blob", S, Wv, cur, grid); break;
-                default: launch_march<PT_SDF_TERRAIN>(ctx, "k_ext_march_terrain", S, Wv, cur, grid); break;
+                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, grid); break;
+                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER, MULTI>(ctx, "k_ext_march_menger", P, Wv, cur, grid); break;
+                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB, MULTI>(ctx, "k_ext_march_blob", P, Wv, cur, grid); break;
+                default: launch_march<PT_SDF_TERRAIN, MULTI>(ctx, "k_ext_march_terrain", P, Wv, cur, grid); break;
             }
         }
-        launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
+        launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     } else {
         launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
-        launch(ctx, "k_ext_classify", k_ext_classify, dim3(grid), dim3(kBlk), S, Wv, cur);
+        launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     }
-    launch(ctx, "k_ext_shade_miss", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_MISS);
-    launch(ctx, "k_ext_shade_emit", k_ext_shade_terminal, dim3(grid), dim3(kBlk), S, Wv, (int)Q_EMIT);
-    launch(ctx, "k_ext_shade_lambert", k_ext_shade_lambert, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
-    launch(ctx, "k_ext_shade_dielectric", k_ext_shade_dielectric, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
-    launch(ctx, "k_ext_shade_conductor", k_ext_shade_conductor, dim3(grid), dim3(kBlk), S, F, Wv, cur, b);
+    launch(ctx, "k_ext_shade_miss", k_ext_shade_terminal<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, (int)Q_MISS);
+    launch(ctx, "k_ext_shade_emit", k_ext_shade_terminal<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, (int)Q_EMIT);
+    launch(ctx, "k_ext_shade_lambert", k_ext_shade_lambert<MULTI>, dim3(grid), dim3(kBlk), S, P.F0, P.B, Wv, cur, b);
+    launch(ctx, "k_ext_shade_dielectric", k_ext_shade_dielectric<MULTI>, dim3(grid), dim3(kBlk), S, P.F0, P.B, Wv, cur,
+           b);
+    launch(ctx, "k_ext_shade_conductor", k_ext_shade_conductor<MULTI>, dim3(grid), dim3(kBlk), S, P.F0, P.B, Wv, cur,
+           b);
     // Shadow rays push children unordered: near-first ordering measured slower
     // on the 4096-sphere scene (most shadow rays are occluded early anyway).
     if (persistent)
         launch(ctx, "k_ext_shadow", k_ext_shadow_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, shadow_cnt(b));
     else if (!fused)
-        launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, shadow_cnt(b));
+        launch(ctx, "k_ext_shadow", k_ext_shadow<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv,
+               shadow_cnt(b));
 }
 
 // After the last bounce (`nb` bounces, extension queue nb & 1 empty): the
 // shadow rays the last bounce emitted, when the bounces are fused.
+template <int W, int STACK, typename SE, bool MULTI>
+static void launch_tail(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int nb, int grid) {
+    const bool persistent = !MULTI && ctx->cfg.traversal == 1 && !P.quad_bvh;
+    if (persistent || P.any_sdf || nb <= 0) return;
+    launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, nb & 1,
+           shadow_cnt(nb - 1));
+}
+
 template <int W, int STACK, typename SE>
-static void launch_tail(pt_ctx* ctx, const ExtSceneHost& H, ExtWave& Wv, int nb, int grid) {
-    const ExtSceneDev& S = H.dev;
-    const bool persistent = ctx->cfg.traversal == 1 && !S.quad_bvh;
-    if (persistent || S.nsdf > 0 || nb <= 0) return;
-    launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, nb & 1, shadow_cnt(nb - 1));
+static void launch_step(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, int b, int grid, bool tail) {
+    if (P.multi) {
+        if (tail) launch_tail<W, STACK, SE, true>(ctx, P, Wv, b, grid);
+        else launch_bounce<W, STACK, SE, true>(ctx, P, Wv, cur, b, grid);
+    } else {
+        if (tail) launch_tail<W, STACK, SE, false>(ctx, P, Wv, b, grid);
+        else launch_bounce<W, STACK, SE, false>(ctx, P, Wv, cur, b, grid);
+    }
 }
 
 template <int W>
-static void launch_bounce_w(pt_ctx* ctx, const ExtSceneHost& H, const ExtFrame& F, ExtWave& Wv, int cur, int b,
-                            int grid, bool tail) {
-    if (H.small_nodes) {
-        if (H.stack_need <= 32) {
-            if (tail) launch_tail<W, 32, uint16_t>(ctx, H, Wv, b, grid);
-            else launch_bounce<W, 32, uint16_t>(ctx, H, F, Wv, cur, b, grid);
-        } else {
-            if (tail) launch_tail<W, 64, uint16_t>(ctx, H, Wv, b, grid);
-            else launch_bounce<W, 64, uint16_t>(ctx, H, F, Wv, cur, b, grid);
-        }
+static void launch_bounce_w(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, int b, int grid, bool tail) {
+    if (P.small_nodes) {
+        if (P.stack_need <= 32) launch_step<W, 32, uint16_t>(ctx, P, Wv, cur, b, grid, tail);
+        else launch_step<W, 64, uint16_t>(ctx, P, Wv, cur, b, grid, tail);
     } else {
-        if (H.stack_need <= 32) {
-            if (tail) launch_tail<W, 32, int>(ctx, H, Wv, b, grid);
-            else launch_bounce<W, 32, int>(ctx, H, F, Wv, cur, b, grid);
-        } else {
-            if (tail) launch_tail<W, 64, int>(ctx, H, Wv, b, grid);
-            else launch_bounce<W, 64, int>(ctx, H, F, Wv, cur, b, grid);
+        if (P.stack_need <= 32) launch_step<W, 32, int>(ctx, P, Wv, cur, b, grid, tail);
+        else launch_step<W, 64, int>(ctx, P, Wv, cur, b, grid, tail);
+    }
+}
+
+// Bounces 0..depth-1, then the tail.
+static void launch_bounces(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int depth, int grid) {
+    for (int b = 0; b <= depth; ++b) {
+        const int cur = b & 1;
+        const bool tail = b == depth;  // b = the number of bounces traced
+        switch (P.width) {
+            case 2: launch_bounce_w<2>(ctx, P, Wv, cur, b, grid, tail); break;
+            case 8: launch_bounce_w<8>(ctx, P, Wv, cur, b, grid, tail); break;
+            default: launch_bounce_w<4>(ctx, P, Wv, cur, b, grid, tail); break;
         }
     }
+}
+
+// The frame of a dispatch: resolution, sample range, depth and camera of the
+// push constants (lens frame as the reference builds it, src/shader.comp:1409-1465).
+static ExtFrame frame_of(const pt_push_constants& pc) {
+    ExtFrame F{};
+    F.W = pc.resolution[0]; F.H = pc.resolution[1]; F.frame = pc.frame; F.spf = pc.samplesPerFrame;
+    F.maxDepth = pc.pathLength;
+    F.camSize = pc.cameraSize; F.apSize = pc.apertureSize; F.apDist = pc.apertureDist;
+    F.camPos = mk3(pc.cameraPosX, pc.cameraPosY, pc.cameraPosZ);
+    F.camM = rotation_deg(mk3(pc.cameraAngle[0], pc.cameraAngle[1], 0.0f));
+    F.fwd = mk3(F.camM.c0.z, F.camM.c1.z, F.camM.c2.z);
+    F.lensPos = F.camPos + F.fwd * pc.lensDistance;
+    F.lensM = rotation_deg(mk3(0.0f, 90.0f - pc.cameraAngle[1], pc.cameraAngle[0]));
+    float r = pc.lensRadius, f = pc.lensFocalLength;
+    F.lensHalf = 2.0f * f - sqrtf(4.0f * f * f - r * r);
+    F.lensSlicePos = 0.5f * pc.lensThickness + F.lensHalf;
+    F.lensTwoF = 2.0f * f;
+    return F;
+}
+
+// Compute units of the context's device (grid sizing), after ensure_ext_wave.
+static int ctx_cus(pt_ctx* ctx) {
+    ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
+    if (cs->cu == 0) {
+        hipDeviceProp_t prop;
+        cs->cu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+                     ? prop.multiProcessorCount
+                     : 256;
+    }
+    return cs->cu;
 }
 
 // Where a frame's per-path XYZ goes: the reference's texel accumulate
@@ -1858,27 +2028,16 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
     if (st != PT_OK) return st;
     ExtWave& Wv = static_cast<ExtCtxState*>(ctx->ext_state)->wave;
 
-    ExtFrame F{};
-    F.W = Wd; F.H = Ht; F.frame = pc->frame; F.spf = pc->samplesPerFrame; F.maxDepth = pc->pathLength;
-    F.camSize = pc->cameraSize; F.apSize = pc->apertureSize; F.apDist = pc->apertureDist;
-    F.camPos = mk3(pc->cameraPosX, pc->cameraPosY, pc->cameraPosZ);
-    F.camM = rotation_deg(mk3(pc->cameraAngle[0], pc->cameraAngle[1], 0.0f));
-    F.fwd = mk3(F.camM.c0.z, F.camM.c1.z, F.camM.c2.z);
-    F.lensPos = F.camPos + F.fwd * pc->lensDistance;
-    F.lensM = rotation_deg(mk3(0.0f, 90.0f - pc->cameraAngle[1], pc->cameraAngle[0]));
-    float r = pc->lensRadius, f = pc->lensFocalLength;
-    F.lensHalf = 2.0f * f - sqrtf(4.0f * f * f - r * r);
-    F.lensSlicePos = 0.5f * pc->lensThickness + F.lensHalf;
-    F.lensTwoF = 2.0f * f;
-
-    ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
-    if (cs->cu == 0) {
-        hipDeviceProp_t prop;
-        cs->cu = (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
-                     ? prop.multiProcessorCount
-                     : 256;
-    }
-    const int grid_cap = cs->cu * 8;
+    ExtFrame F = frame_of(*pc);
+    const int grid_cap = ctx_cus(ctx) * 8;
+    ExtPlan plan;
+    plan.S0 = H.dev;
+    plan.any_sdf = H.dev.nsdf > 0;
+    plan.quad_bvh = H.dev.quad_bvh != 0;
+    plan.sdf_kinds = H.sdf_kinds;
+    plan.width = H.width;
+    plan.stack_need = H.stack_need;
+    plan.small_nodes = H.small_nodes;
     const int pcmax = std::min(P, cap);
     for (int p0 = 0; p0 < P; p0 += pcmax) {
         const int pcur = std::min(pcmax, P - p0);
@@ -1889,15 +2048,8 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
             const int grid = std::min((npaths + kBlk - 1) / kBlk, grid_cap);
             launch(ctx, "k_ext_counts", k_ext_counts, dim3(1), dim3(64), Wv.counts, npaths);
             launch(ctx, "k_ext_raygen", k_ext_raygen, dim3(grid), dim3(kBlk), H.dev, F, Wv);
-            for (int b = 0; b <= pc->pathLength; ++b) {
-                const int cur = b & 1;
-                const bool tail = b == pc->pathLength;  // b = the number of bounces traced
-                switch (H.width) {
-                    case 2: launch_bounce_w<2>(ctx, H, F, Wv, cur, b, grid, tail); break;
-                    case 8: launch_bounce_w<8>(ctx, H, F, Wv, cur, b, grid, tail); break;
-                    default: launch_bounce_w<4>(ctx, H, F, Wv, cur, b, grid, tail); break;
-                }
-            }
+            plan.F0 = F;
+            launch_bounces(ctx, plan, Wv, pc->pathLength, grid);
             const dim3 pg((F.pc + kBlk - 1) / kBlk);
             if (sink.acc)
                 launch(ctx, "k_ext_accum", k_ext_accum, pg, dim3(kBlk), F, Wv);
@@ -1933,6 +2085,99 @@ int ext_dispatch_moments(pt_ctx* ctx, const pt_scene* s, const pt_push_constants
     sink.sumsq = sumsq;
     sink.nsamp = nsamp;
     return ext_run(ctx, s, pc, sink);
+}
+
+int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
+    if (n <= 0) return PT_OK;
+    if (n > kMaxBatch) {
+        ctx->err = "batched wave: more than " + std::to_string(kMaxBatch) + " requests";
+        return PT_ERR_CAPACITY;
+    }
+    ExtPlan plan;
+    plan.multi = true;
+    std::vector<ExtSceneDev> scenes((size_t)n);
+    std::vector<ExtFrame> frames((size_t)n);
+    std::vector<ExtSeg> segs((size_t)n);
+    std::vector<ExtMomentSink> sinks((size_t)n);
+    int64_t npaths = 0, npix = 0;
+    int depth = 0;
+    for (int i = 0; i < n; ++i) {
+        const ExtBatchItem& it = items[i];
+        int st = ext_scene_build(ctx, const_cast<pt_scene*>(it.scene));
+        if (st != PT_OK) return st;
+        const ExtSceneHost& H = *static_cast<const ExtSceneHost*>(it.scene->ext);
+        if (i == 0) {
+            plan.width = H.width;
+        } else if (H.width != plan.width) {
+            ctx->err = "batched wave: scenes with different BVH widths";
+            return PT_ERR_STATE;
+        }
+        plan.stack_need = std::max(plan.stack_need, H.stack_need);
+        plan.small_nodes = plan.small_nodes && H.small_nodes;
+        plan.any_sdf = plan.any_sdf || H.dev.nsdf > 0;
+        plan.quad_bvh = plan.quad_bvh || H.dev.quad_bvh;
+        for (int k : H.sdf_kinds)
+            if (std::find(plan.sdf_kinds.begin(), plan.sdf_kinds.end(), k) == plan.sdf_kinds.end())
+                plan.sdf_kinds.push_back(k);
+        const int P = it.pc.resolution[0] * it.pc.resolution[1];
+        const int q = it.pc.samplesPerFrame;
+        if (P <= 0 || q <= 0) {
+            ctx->err = "batched wave: empty request";
+            return PT_ERR_INVALID_ARG;
+        }
+        scenes[(size_t)i] = H.dev;
+        frames[(size_t)i] = frame_of(it.pc);
+        segs[(size_t)i] = ExtSeg{i, (int)npaths, 0, P, 0, q, (int)npix, 0};
+        sinks[(size_t)i] = ExtMomentSink{it.sum, it.sumsq, it.nsamp, P, 0};
+        npaths += (int64_t)P * q;
+        npix += P;
+        depth = std::max(depth, it.pc.pathLength);
+    }
+    int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
+    cap = std::max(cap, 1024);
+    if (npaths > cap) {
+        ctx->err = "batched wave: " + std::to_string(npaths) + " paths exceed the wave capacity " + std::to_string(cap);
+        return PT_ERR_CAPACITY;
+    }
+    int st = ensure_ext_wave(ctx, cap);
+    if (st != PT_OK) return st;
+    ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
+    ExtWave& Wv = cs->wave;
+    const size_t tb =
+        (sizeof(ExtSceneDev) + sizeof(ExtFrame) + sizeof(ExtSeg) + sizeof(ExtMomentSink)) * kMaxBatch + 4 * 256;
+    if (!cs->batch_mem.p) {
+        st = dev_alloc(ctx, cs->batch_mem, tb);
+        if (st != PT_OK) return st;
+    }
+    char* base = static_cast<char*>(cs->batch_mem.p);
+    bool ok = true;
+    auto put = [&](const void* src, size_t bytes) -> void* {
+        void* dst = base;
+        base += (bytes + 255) / 256 * 256;
+        if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ok = false;
+        return dst;
+    };
+    const ExtSceneDev* d_scenes = static_cast<const ExtSceneDev*>(put(scenes.data(), sizeof(ExtSceneDev) * (size_t)n));
+    const ExtFrame* d_frames = static_cast<const ExtFrame*>(put(frames.data(), sizeof(ExtFrame) * (size_t)n));
+    const ExtSeg* d_segs = static_cast<const ExtSeg*>(put(segs.data(), sizeof(ExtSeg) * (size_t)n));
+    const ExtMomentSink* d_sinks =
+        static_cast<const ExtMomentSink*>(put(sinks.data(), sizeof(ExtMomentSink) * (size_t)n));
+    if (!ok) return hip_fail(ctx, hipGetLastError(), "batched wave tables");
+    // the host copies above read pageable vectors: wait for them before the vectors go
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "batched wave tables");
+    plan.B = ExtBatch{d_scenes, d_frames};
+    plan.S0 = scenes[0];
+    plan.F0 = frames[0];
+    const int grid = (int)std::min<int64_t>((npaths + kBlk - 1) / kBlk, (int64_t)ctx_cus(ctx) * 8);
+    launch(ctx, "k_ext_counts", k_ext_counts, dim3(1), dim3(64), Wv.counts, (int)npaths);
+    launch(ctx, "k_ext_raygen", k_ext_raygen_batch, dim3(grid), dim3(kBlk), plan.B, d_segs, n, (int)npaths, Wv);
+    launch_bounces(ctx, plan, Wv, depth, grid);
+    launch(ctx, "k_ext_moments", k_ext_moments_batch, dim3((unsigned)((npix + kBlk - 1) / kBlk)), dim3(kBlk), d_segs,
+           n, d_sinks, (int)npix, Wv);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(ctx, e, "batched dispatch launch");
+    return PT_OK;
 }
 
 int ext_rel_error(pt_ctx* ctx, int P, const float* sum, const float* sumsq, const int* nsamp, double* d_out) {

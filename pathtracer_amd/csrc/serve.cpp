@@ -7,12 +7,13 @@
 //   - every admitted request keeps its scene, BVH and per-pixel moment buffers
 //     resident in HBM; admission stops at the server's HBM budget and request
 //     count;
-//   - a step traces one quantum of samples of one request: min_spp samples per
-//     pixel first, then the estimated remaining samples, at most as many as
-//     fill the context's wave (a small image traces many samples per pixel in
-//     one wave, so it still fills the GPU);
-//   - the request is chosen by the policy: shortest estimated remaining GPU time
-//     first (PT_SERVE_SRPT), or arrival order, one request to completion at a
+//   - a step traces one wave holding the paths of every active request that
+//     fits (continuous batching): each request gets a quantum of min_spp
+//     samples per pixel first, then its estimated remaining samples, in
+//     policy order while the wave has room, so requests that arrive while
+//     others are being traced join the next wave;
+//   - the policy orders the requests: shortest estimated remaining GPU time
+//     first (PT_SERVE_SRPT), or only the oldest request, one to completion at a
 //     time (PT_SERVE_FIFO, the one-job-at-a-time baseline);
 //   - after each quantum the relative RMSE is estimated on the GPU from the
 //     per-pixel second moments (k_ext_rel_error): sqrt(sum var/n) / sqrt(sum mean^2)
@@ -71,7 +72,7 @@ struct pt_server {
     std::vector<int> active;
     int64_t bytes_in_use = 0;
     double sec_per_path = 0.0;  // running estimate of GPU seconds per traced path
-    double* d_rel = nullptr;    // 64 lines x 32 doubles
+    double* d_rel = nullptr;    // per request of a wave: 64 lines x 32 doubles
     std::string err;
 };
 
@@ -169,7 +170,7 @@ pt_status pt_server_create(pt_ctx* ctx, const pt_server_config* cfg, pt_server**
     if (s->cfg.policy != PT_SERVE_FIFO) s->cfg.policy = PT_SERVE_SRPT;
     s->t0 = Clock::now();
     (void)hipSetDevice(ctx->device);
-    if (hipMalloc(&s->d_rel, kRelLines * kRelStride * sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&s->d_rel, (size_t)ptk::kMaxBatch * kRelLines * kRelStride * sizeof(double)) != hipSuccess) {
         delete s;
         return ptk::hip_fail(ctx, hipGetLastError(), "pt_server_create");
     }
@@ -211,89 +212,111 @@ pt_status pt_server_submit(pt_server* s, const pt_request_desc* d, double t_arri
     return PT_OK;
 }
 
-pt_status pt_server_step(pt_server* s, pt_request_status* stepped) {
-    if (!s) return PT_ERR_INVALID_ARG;
+pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, int32_t* n_stepped) {
+    if (!s || (cap > 0 && !stepped)) return PT_ERR_INVALID_ARG;
     pt_ctx* ctx = s->ctx;
     (void)hipSetDevice(ctx->device);
-    if (stepped) {
-        std::memset(stepped, 0, sizeof *stepped);
-        stepped->id = -1;
-    }
+    if (n_stepped) *n_stepped = 0;
     // admission in arrival order, within the request count and HBM budget
     while (!s->waiting.empty() && (int)s->active.size() < s->cfg.max_active) {
         Request& r = *s->reqs[s->waiting.front()];
         if (!s->active.empty() && s->bytes_in_use + estimate_bytes(r) > s->cfg.hbm_budget) break;
+        const auto ta = Clock::now();
         int st = admit(s, r, now_s(s));
+        r.gpu_s += std::chrono::duration<double>(Clock::now() - ta).count();
+        s->waiting.pop_front();
         if (st != PT_OK) {
-            s->waiting.pop_front();
             r.state = PT_REQ_FAILED;
             r.t_done = now_s(s);
             release_request(s, r);
-            if (stepped) (void)pt_server_status(s, r.id, stepped);
+            if (cap > 0 && n_stepped) {
+                (void)pt_server_status(s, r.id, &stepped[0]);
+                *n_stepped = 1;
+            }
             return st;
         }
-        s->waiting.pop_front();
         s->active.push_back(r.id);
     }
     if (s->active.empty()) return PT_SERVE_IDLE;
 
-    // choose the request
-    int pick = s->active.front();
-    if (s->cfg.policy == PT_SERVE_SRPT) {
-        double best = INFINITY;
-        for (int id : s->active) {
-            double c = remaining_cost(s, *s->reqs[id]);
-            if (c < best) {
-                best = c;
-                pick = id;
-            }
-        }
+    // the wave: requests in policy order, each with its quantum (min_spp samples
+    // first, then the estimated remaining ones), while paths fit the wave;
+    // FIFO traces only the oldest request
+    std::vector<int> order = s->active;
+    if (s->cfg.policy == PT_SERVE_SRPT)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return remaining_cost(s, *s->reqs[a]) < remaining_cost(s, *s->reqs[b]);
+        });
+    const int64_t wave = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
+    int64_t left = wave;
+    std::vector<ptk::ExtBatchItem> items;
+    std::vector<Request*> in;
+    for (int id : order) {
+        Request& r = *s->reqs[id];
+        int want = r.spp == 0 ? s->cfg.min_spp : (int)std::ceil(1.05 * remaining_spp(r, s->cfg.min_spp));
+        want = std::max(1, std::min(want, r.spp_max - r.spp));
+        const int q = (int)std::min<int64_t>(want, left / std::max(1, r.P));
+        if (q < 1) continue;
+        ptk::ExtBatchItem it;
+        it.scene = r.scene;
+        it.pc = r.pc;
+        it.pc.samplesPerFrame = q;
+        it.pc.frame = r.spp + q;
+        it.pc.currentSamples = r.spp + q;
+        it.sum = r.sum;
+        it.sumsq = r.sumsq;
+        it.nsamp = r.nsamp;
+        items.push_back(it);
+        in.push_back(&r);
+        left -= (int64_t)q * r.P;
+        if (s->cfg.policy == PT_SERVE_FIFO || (int)items.size() == ptk::kMaxBatch) break;
     }
-    Request& r = *s->reqs[pick];
-
-    // quantum: min_spp samples first, then the estimated remaining samples (5 %
-    // margin against the noise of the estimate), at most a full wave
-    const int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
-    int want = r.spp == 0 ? s->cfg.min_spp : (int)std::ceil(1.05 * remaining_spp(r, s->cfg.min_spp));
-    want = std::min(want, r.spp_max - r.spp);
-    const int q = std::max(1, std::min(cap / std::max(1, r.P), want));
+    if (items.empty()) return fail(s, PT_ERR_CAPACITY, "pt_server_step: no request fits the wave");
 
     pt_counters c0{}, c1{};
     (void)pt_counters_get(ctx, &c0);
     const auto t_start = Clock::now();
-    r.pc.samplesPerFrame = q;
-    r.pc.frame = r.spp + q;
-    r.pc.currentSamples = r.spp + q;
-    int st = ptk::ext_dispatch_moments(ctx, r.scene, &r.pc, r.sum, r.sumsq, r.nsamp);
+    int st = ptk::ext_dispatch_batch(ctx, items.data(), (int)items.size());
     if (st != PT_OK) return fail(s, st, ctx->err);
-    hipError_t e = hipMemsetAsync(s->d_rel, 0, kRelLines * kRelStride * sizeof(double), ctx->stream);
+    const size_t nline = (size_t)kRelLines * kRelStride;
+    hipError_t e = hipMemsetAsync(s->d_rel, 0, items.size() * nline * sizeof(double), ctx->stream);
     if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
-    st = ptk::ext_rel_error(ctx, r.P, r.sum, r.sumsq, r.nsamp, s->d_rel);
-    if (st != PT_OK) return fail(s, st, ctx->err);
-    std::vector<double> lines(kRelLines * kRelStride);
+    for (size_t i = 0; i < items.size(); ++i) {
+        st = ptk::ext_rel_error(ctx, in[i]->P, in[i]->sum, in[i]->sumsq, in[i]->nsamp, s->d_rel + i * nline);
+        if (st != PT_OK) return fail(s, st, ctx->err);
+    }
+    std::vector<double> lines(items.size() * nline);
     e = hipMemcpyAsync(lines.data(), s->d_rel, lines.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
     const double dt = std::chrono::duration<double>(Clock::now() - t_start).count();
     (void)pt_counters_get(ctx, &c1);
-    double v = 0.0, m = 0.0;
-    for (int l = 0; l < kRelLines; ++l) {
-        v += lines[(size_t)l * kRelStride];
-        m += lines[(size_t)l * kRelStride + 1];
+    const double total_paths = (double)(wave - left);
+    s->sec_per_path = s->sec_per_path > 0.0 ? 0.8 * s->sec_per_path + 0.2 * dt / total_paths : dt / total_paths;
+    const double now = now_s(s);
+    for (size_t i = 0; i < items.size(); ++i) {
+        Request& r = *in[i];
+        double v = 0.0, m = 0.0;
+        for (int l = 0; l < kRelLines; ++l) {
+            v += lines[i * nline + (size_t)l * kRelStride];
+            m += lines[i * nline + (size_t)l * kRelStride + 1];
+        }
+        const int q = items[i].pc.samplesPerFrame;
+        const double share = (double)q * r.P / total_paths;
+        r.spp += q;
+        r.rel = m > 0.0 ? std::sqrt(v / m) : INFINITY;
+        r.gpu_s += dt * share;
+        r.segments += (uint64_t)((double)(c1.segments - c0.segments) * share);
+        if ((r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max) {
+            r.state = PT_REQ_DONE;
+            r.t_done = now;
+            s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
+        }
+        if (n_stepped && (int)i < cap) {
+            (void)pt_server_status(s, r.id, &stepped[i]);
+            *n_stepped = (int32_t)i + 1;
+        }
     }
-    r.spp += q;
-    r.rel = m > 0.0 ? std::sqrt(v / m) : INFINITY;
-    r.gpu_s += dt;
-    r.segments += c1.segments - c0.segments;
-    const double paths = (double)q * r.P;
-    const double spp_cost = dt / paths;
-    s->sec_per_path = s->sec_per_path > 0.0 ? 0.8 * s->sec_per_path + 0.2 * spp_cost : spp_cost;
-    if ((r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max) {
-        r.state = PT_REQ_DONE;
-        r.t_done = now_s(s);
-        s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
-    }
-    if (stepped) (void)pt_server_status(s, r.id, stepped);
     return PT_OK;
 }
 

@@ -173,6 +173,21 @@ int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
 // per-pixel moments (sum, sumsq: 3 x P floats, channel-major; nsamp: P ints).
 int ext_dispatch_moments(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* sum, float* sumsq,
                          int* nsamp);
+// One request of a batched wave: its scene, the push constants of its quantum
+// (camera, resolution, pathLength; samples [frame - samplesPerFrame, frame)),
+// and its per-pixel moments.
+struct ExtBatchItem {
+    const pt_scene* scene = nullptr;
+    pt_push_constants pc{};
+    float* sum = nullptr;
+    float* sumsq = nullptr;
+    int* nsamp = nullptr;
+};
+constexpr int kMaxBatch = 64;
+// All items' paths in one wave (their total must fit the context's wave
+// capacity); each request's samples are added to its moments in sample order,
+// so a request's result does not depend on what else is in the wave.
+int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n);
 // Adds (sum over pixels and channels of variance of the mean, sum of mean^2)
 // into d_out: 64 lines of 32 doubles, slots 0 and 1 of each line.
 int ext_rel_error(pt_ctx* ctx, int P, const float* sum, const float* sumsq, const int* nsamp, double* d_out);

@@ -21,6 +21,7 @@ Workloads:
                    15 % conductor (n in [0.2, 2.5], k in [2, 8]).
   sdf_scene()      power-8 mandelbulb and menger sponge over a ground plane
                    under a uniform white environment.
+  serving_request(i)  request i of the serving workload, seeded i.
 """
 from __future__ import annotations
 
@@ -153,3 +154,24 @@ def sdf_scene_dict(jitter: float = 0.0, scene_seed: int = 0, wavelengths: int = 
 
 def sdf_scene(**kw) -> str:
     return json.dumps(sdf_scene_dict(**kw))
+
+
+SERVING_RESOLUTIONS = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+
+
+def serving_request(i: int) -> dict:
+    """Request i of the serving workload, from a generator seeded i: resolution
+    uniform over SERVING_RESOLUTIONS; sample budget round(exp(U[ln 16, ln 1024]));
+    with probability 1/2 the Cornell box with N = round(exp(U[0, ln 4096]))
+    random spheres (scene seed i, depth 12), else the mandelbulb + menger scene
+    with its parameters jittered by U[0.9, 1.1] (scene seed i, depth 8); 4 hero
+    wavelengths; render seed i."""
+    rng = np.random.default_rng(i)
+    w, h = SERVING_RESOLUTIONS[int(rng.integers(len(SERVING_RESOLUTIONS)))]
+    spp = int(round(math.exp(rng.uniform(math.log(16.0), math.log(1024.0)))))
+    if rng.uniform() < 0.5:
+        n = int(round(math.exp(rng.uniform(0.0, math.log(4096.0)))))
+        return {"id": i, "kind": f"spheres{n}", "text": random_spheres(n=n, scene_seed=i, wavelengths=4, seed=i),
+                "width": w, "height": h, "spp": spp, "depth": 12}
+    return {"id": i, "kind": "sdf", "text": sdf_scene(jitter=0.1, scene_seed=i, wavelengths=4, seed=i),
+            "width": w, "height": h, "spp": spp, "depth": 8}
