@@ -130,6 +130,9 @@ typedef struct pt_config {
     int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
     int32_t traversal;     /* extended mode: 0 = one ray per lane, 1 = persistent lanes refilled from the queue */
     int32_t bvh_leaf;      /* most primitives per BVH leaf, 1..16 (0 = 1) */
+    int32_t graph_mode;    /* batched serving waves: 0 = captured once per launch shape into a hipGraph and
+                              replayed, 1 = kernels launched one by one */
+    int32_t reserved;
 } pt_config;
 
 typedef struct pt_counters {

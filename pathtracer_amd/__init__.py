@@ -98,6 +98,8 @@ class Config(C.Structure):
         ("hbm_budget", C.c_int64),
         ("traversal", C.c_int32),
         ("bvh_leaf", C.c_int32),
+        ("graph_mode", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -310,8 +312,9 @@ class Context:
     """pt_ctx: one device, one stream."""
 
     def __init__(self, device: int = 0, wave_paths: int = 0, block_size: int = 0, bvh_width: int = 0,
-                 sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0, bvh_leaf: int = 0):
-        cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, bvh_leaf)
+                 sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0, bvh_leaf: int = 0,
+                 graph_mode: int = 0):
+        cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, bvh_leaf, graph_mode, 0)
         h = C.c_void_p()
         _check(lib.pt_create(C.byref(h), device, C.byref(cfg)))
         self.h = h

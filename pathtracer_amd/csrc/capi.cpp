@@ -82,7 +82,8 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     if (cfg) ctx->cfg = *cfg;
     if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && (ctx->cfg.block_size % 64) != 0) ||
         (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
-        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16) {
+        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16 ||
+        ctx->cfg.graph_mode < 0 || ctx->cfg.graph_mode > 1) {
         delete ctx;
         return set_global(PT_ERR_INVALID_ARG, "pt_create: invalid pt_config");
     }

@@ -25,6 +25,7 @@
 // -> shadow (any-hit) -> next bounce. Queues are compacted with wave64 ballots;
 // kernels run grid-stride over their queue.
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -634,6 +635,15 @@ __device__ __forceinline__ void add_xyz(ExtWave& Wv, int p, f3 c) {
 // Kernels.
 __device__ __forceinline__ int* qlen(const ExtWave& W, int i) { return W.counts + i * kCntStride; }
 
+// The per-wave sizes of a batched wave, in device memory so a captured graph
+// of the wave's kernels replays for any batch.
+struct ExtBatchHdr {
+    int npaths, npix, nseg, pad;
+};
+__global__ void k_ext_counts_batch(int* counts, const ExtBatchHdr* hdr) {
+    int i = threadIdx.x;
+    if (i < N_CNT) counts[i * kCntStride] = (i == 0) ? hdr->npaths : 0;
+}
 __global__ void k_ext_counts(int* counts, int n0) {
     int i = threadIdx.x;
     if (i < N_CNT) counts[i * kCntStride] = (i == 0) ? n0 : 0;
@@ -757,8 +767,9 @@ __device__ __forceinline__ int seg_find(int n, int i, Start start) {
     return lo;
 }
 
-__global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const ExtSeg* segs, int nseg, int npaths,
+__global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const ExtSeg* segs, const ExtBatchHdr* hdr,
                                                             ExtWave Wv) {
+    const int npaths = hdr->npaths, nseg = hdr->nseg;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)npaths);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npaths; i += gridDim.x * blockDim.x) {
         const ExtSeg& g = segs[seg_find(nseg, i, [&](int k) { return segs[k].base; })];
@@ -1481,11 +1492,11 @@ struct ExtMomentSink {
 
 // Batched serving sink: one thread per (segment, pixel) adds the pixel's
 // samples of the segment, in sample order, to its request's moments.
-__global__ void __launch_bounds__(kBlk) k_ext_moments_batch(const ExtSeg* segs, int nseg, const ExtMomentSink* sinks,
-                                                            int npix, ExtWave Wv) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlk) k_ext_moments_batch(const ExtSeg* segs, const ExtMomentSink* sinks,
+                                                            const ExtBatchHdr* hdr, ExtWave Wv) {
+    const int npix = hdr->npix, nseg = hdr->nseg;
     unsigned nans = 0;
-    if (t < npix) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < npix; t += gridDim.x * blockDim.x) {
         const ExtSeg& g = segs[seg_find(nseg, t, [&](int k) { return segs[k].pix_base; })];
         const ExtMomentSink& K = sinks[g.rid];
         const int j = t - g.pix_base, px = g.p0 + j, P = K.P;
@@ -1558,8 +1569,14 @@ struct ExtCtxState {
     DevAlloc mem;
     ExtWave wave{};
     int cu = 0;  // compute units of the context's device (grid sizing)
-    DevAlloc batch_mem;  // batched waves: scene, frame, segment and sink tables of up to kMaxBatch requests
+    DevAlloc batch_mem;  // batched waves: header + scene, frame, segment and sink tables of up to kMaxBatch requests
+    std::map<uint64_t, hipGraphExec_t> graphs;  // captured batched waves, by launch shape
 };
+
+static void drop_graphs(ExtCtxState* st) {
+    for (auto& kv : st->graphs) (void)hipGraphExecDestroy(kv.second);
+    st->graphs.clear();
+}
 
 static inline f3 v3(const float* a) { return mk3(a[0], a[1], a[2]); }
 
@@ -1828,6 +1845,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
         st = new ExtCtxState();
         ctx->ext_state = st;
     }
+    drop_graphs(st);  // they hold the old wave's addresses
     dev_free(ctx, st->mem);
     size_t n = (size_t)cap;
     // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
@@ -1855,6 +1873,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
 void ext_wave_release(pt_ctx* ctx) {
     ExtCtxState* st = static_cast<ExtCtxState*>(ctx->ext_state);
     if (!st) return;
+    drop_graphs(st);
     dev_free(ctx, st->batch_mem);
     dev_free(ctx, st->mem);
     delete st;
@@ -2143,38 +2162,71 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
     if (st != PT_OK) return st;
     ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
     ExtWave& Wv = cs->wave;
-    const size_t tb =
-        (sizeof(ExtSceneDev) + sizeof(ExtFrame) + sizeof(ExtSeg) + sizeof(ExtMomentSink)) * kMaxBatch + 4 * 256;
+    // tables at fixed addresses: header, scenes, frames, segments, sinks
+    const size_t o_sc = 256, o_fr = o_sc + (sizeof(ExtSceneDev) * kMaxBatch + 255) / 256 * 256,
+                 o_sg = o_fr + (sizeof(ExtFrame) * kMaxBatch + 255) / 256 * 256,
+                 o_sk = o_sg + (sizeof(ExtSeg) * kMaxBatch + 255) / 256 * 256,
+                 tb = o_sk + (sizeof(ExtMomentSink) * kMaxBatch + 255) / 256 * 256;
     if (!cs->batch_mem.p) {
         st = dev_alloc(ctx, cs->batch_mem, tb);
         if (st != PT_OK) return st;
     }
     char* base = static_cast<char*>(cs->batch_mem.p);
-    bool ok = true;
-    auto put = [&](const void* src, size_t bytes) -> void* {
-        void* dst = base;
-        base += (bytes + 255) / 256 * 256;
-        if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ok = false;
-        return dst;
-    };
-    const ExtSceneDev* d_scenes = static_cast<const ExtSceneDev*>(put(scenes.data(), sizeof(ExtSceneDev) * (size_t)n));
-    const ExtFrame* d_frames = static_cast<const ExtFrame*>(put(frames.data(), sizeof(ExtFrame) * (size_t)n));
-    const ExtSeg* d_segs = static_cast<const ExtSeg*>(put(segs.data(), sizeof(ExtSeg) * (size_t)n));
-    const ExtMomentSink* d_sinks =
-        static_cast<const ExtMomentSink*>(put(sinks.data(), sizeof(ExtMomentSink) * (size_t)n));
-    if (!ok) return hip_fail(ctx, hipGetLastError(), "batched wave tables");
-    // the host copies above read pageable vectors: wait for them before the vectors go
-    hipError_t e = hipStreamSynchronize(ctx->stream);
+    ExtBatchHdr hdr{(int)npaths, (int)npix, n, 0};
+    const ExtBatchHdr* d_hdr = reinterpret_cast<const ExtBatchHdr*>(base);
+    const ExtSceneDev* d_scenes = reinterpret_cast<const ExtSceneDev*>(base + o_sc);
+    const ExtFrame* d_frames = reinterpret_cast<const ExtFrame*>(base + o_fr);
+    const ExtSeg* d_segs = reinterpret_cast<const ExtSeg*>(base + o_sg);
+    const ExtMomentSink* d_sinks = reinterpret_cast<const ExtMomentSink*>(base + o_sk);
+    hipError_t e = hipMemcpyAsync(base, &hdr, sizeof hdr, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + o_sc, scenes.data(), sizeof(ExtSceneDev) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + o_fr, frames.data(), sizeof(ExtFrame) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + o_sg, segs.data(), sizeof(ExtSeg) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + o_sk, sinks.data(), sizeof(ExtMomentSink) * (size_t)n, hipMemcpyHostToDevice,
+                           ctx->stream);
+    // the copies read pageable host vectors: wait for them before the vectors go
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "batched wave tables");
     plan.B = ExtBatch{d_scenes, d_frames};
     plan.S0 = scenes[0];
     plan.F0 = frames[0];
-    const int grid = (int)std::min<int64_t>((npaths + kBlk - 1) / kBlk, (int64_t)ctx_cus(ctx) * 8);
-    launch(ctx, "k_ext_counts", k_ext_counts, dim3(1), dim3(64), Wv.counts, (int)npaths);
-    launch(ctx, "k_ext_raygen", k_ext_raygen_batch, dim3(grid), dim3(kBlk), plan.B, d_segs, n, (int)npaths, Wv);
-    launch_bounces(ctx, plan, Wv, depth, grid);
-    launch(ctx, "k_ext_moments", k_ext_moments_batch, dim3((unsigned)((npix + kBlk - 1) / kBlk)), dim3(kBlk), d_segs,
-           n, d_sinks, (int)npix, Wv);
+    // a fixed grid (the wave's sizes come from the header), so one captured
+    // graph per launch shape replays for every batch of that shape
+    const int grid = (int)std::min<int64_t>(((int64_t)cap + kBlk - 1) / kBlk, (int64_t)ctx_cus(ctx) * 8);
+    auto launch_wave = [&]() {
+        launch(ctx, "k_ext_counts", k_ext_counts_batch, dim3(1), dim3(64), Wv.counts, d_hdr);
+        launch(ctx, "k_ext_raygen", k_ext_raygen_batch, dim3(grid), dim3(kBlk), plan.B, d_segs, d_hdr, Wv);
+        launch_bounces(ctx, plan, Wv, depth, grid);
+        launch(ctx, "k_ext_moments", k_ext_moments_batch, dim3(grid), dim3(kBlk), d_segs, d_sinks, d_hdr, Wv);
+    };
+    if (ctx->cfg.graph_mode != 0 || ctx->prof) {
+        launch_wave();
+    } else {
+        uint32_t kinds = 0;
+        for (int k : plan.sdf_kinds) kinds |= 1u << (k & 7);
+        const uint64_t key = (uint64_t)plan.width | (uint64_t)(plan.stack_need <= 32) << 4 |
+                             (uint64_t)plan.small_nodes << 5 | (uint64_t)plan.any_sdf << 6 | (uint64_t)kinds << 8 |
+                             (uint64_t)depth << 16 | (uint64_t)grid << 32;
+        auto it = cs->graphs.find(key);
+        if (it == cs->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+            launch_wave();
+            e = hipStreamEndCapture(ctx->stream, &g);
+            if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return hip_fail(ctx, e, "batched wave graph capture");
+            it = cs->graphs.emplace(key, x).first;
+        }
+        e = hipGraphLaunch(it->second, ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "batched dispatch launch");
     return PT_OK;

@@ -41,8 +41,11 @@ def test_result_is_the_same_alone_and_in_the_mix(kind):
     others = [(scenes.cornell(4, seed=1), 24, 24, 6), (scenes.sdf_scene(seed=2), 32, 16, 5),
               (scenes.random_spheres(n=64, scene_seed=9, wavelengths=1, seed=3), 40, 20, 12)]
     imgs = []
-    for wave, mix in ((1 << 22, False), (20000, False), (1 << 22, True), (30000, True)):
-        with pt.Context(0, wave_paths=wave) as ctx, pt.Server(ctx) as srv:
+    # graph 0: each wave shape captured once into a hipGraph and replayed every
+    # step with new tables; graph 1: kernels launched one by one
+    for wave, mix, graph in ((1 << 22, False, 1), (20000, False, 0), (1 << 22, True, 0), (30000, True, 0),
+                             (30000, True, 1)):
+        with pt.Context(0, wave_paths=wave, graph_mode=graph) as ctx, pt.Server(ctx) as srv:
             rid = srv.submit(text, 48, 32, spp_max=48, path_length=depth, target=1e-6)  # runs to spp_max
             extra = [srv.submit(t, w, h, spp_max=24, path_length=d, target=1e-6) for t, w, h, d in others] if mix else []
             st = run_until_done(srv, [rid] + extra)
