@@ -322,6 +322,33 @@ pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t
 pt_status pt_server_release(pt_server* server, int32_t id);
 pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* active, int64_t* bytes_in_use);
 
+/* ---- Denoiser ------------------------------------------------------------
+ * No counterpart in the reference. A 3-level U-Net (32/64/128 channels, 3x3
+ * convolutions) that maps a low-sample image plus first-hit features to a
+ * denoised XYZ image, trained and run on the GPU's matrix cores in BF16 with
+ * FP32 accumulation and FP32 master weights (Adam).
+ * Features: N images H x W (H, W multiples of 4), 10 floats per pixel:
+ *   XYZ (3), albedo (3), normal (3), depth (1).
+ * The network predicts log1p(denoised XYZ) - log1p(input XYZ); the training
+ * loss is the mean absolute difference of log1p(denoised) and log1p(target). */
+typedef struct pt_denoiser pt_denoiser;
+pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width, uint64_t seed, pt_denoiser** out);
+void pt_dn_destroy(pt_denoiser* dn);
+const char* pt_dn_last_error(const pt_denoiser* dn);
+int64_t pt_dn_param_count(const pt_denoiser* dn);
+/* FP32 parameters: per layer, weights [9][Cout][Cin] then biases [Cout]. */
+pt_status pt_dn_get_params(pt_denoiser* dn, float* host, size_t n);
+pt_status pt_dn_set_params(pt_denoiser* dn, const float* host, size_t n);
+/* features (N, H, W, 10) -> denoised XYZ (N, H, W, 3) */
+pt_status pt_dn_denoise(pt_denoiser* dn, const float* features, int N, int H, int W, float* denoised);
+/* Loss against target XYZ (N, H, W, 3) and the gradient of every parameter
+ * (host_grads may be NULL). */
+pt_status pt_dn_gradients(pt_denoiser* dn, const float* features, const float* target, int N, int H, int W,
+                          float* loss, float* host_grads, size_t n_grads);
+/* One Adam step (beta 0.9 / 0.999) on a batch. */
+pt_status pt_dn_train_step(pt_denoiser* dn, const float* features, const float* target, int N, int H, int W, float lr,
+                           float* loss);
+
 #ifdef __cplusplus
 }
 #endif

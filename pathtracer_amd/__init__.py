@@ -221,6 +221,17 @@ SIGNATURES = {
     "pt_server_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
     "pt_server_release": (C.c_int, [_P, C.c_int32]),
     "pt_server_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "pt_dn_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "pt_dn_destroy": (None, [_P]),
+    "pt_dn_last_error": (C.c_char_p, [_P]),
+    "pt_dn_param_count": (C.c_int64, [_P]),
+    "pt_dn_get_params": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "pt_dn_set_params": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "pt_dn_denoise": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "pt_dn_gradients": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "pt_dn_train_step": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
+                                   C.c_float, C.POINTER(C.c_float)]),
     "pt_math_eval_host": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
     "pt_math_eval_device": (
         C.c_int,
@@ -591,3 +602,62 @@ class Server:
         w, a, b = C.c_int32(), C.c_int32(), C.c_int64()
         self._ck(lib.pt_server_counts(self.h, C.byref(w), C.byref(a), C.byref(b)))
         return w.value, a.value, b.value
+
+
+class Denoiser:
+    """pt_denoiser: the U-Net denoiser (BF16 on the matrix cores, FP32 masters)."""
+
+    def __init__(self, ctx: "Context", max_batch: int, max_height: int, max_width: int, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib.pt_dn_create(ctx.h, max_batch, max_height, max_width, seed, C.byref(h)), ctx.h)
+        self.h = h
+        self.n = int(lib.pt_dn_param_count(h))
+
+    def _ck(self, st: int) -> None:
+        if st != PT_OK:
+            raise PTError(st, (lib.pt_dn_last_error(self.h) or b"").decode(errors="replace"))
+
+    def close(self) -> None:
+        if self.h:
+            lib.pt_dn_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n, np.float32)
+        self._ck(lib.pt_dn_get_params(self.h, _fptr(out), self.n))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        self._ck(lib.pt_dn_set_params(self.h, _fptr(p), p.size))
+
+    def denoise(self, feat: np.ndarray) -> np.ndarray:
+        feat = np.ascontiguousarray(feat, np.float32)
+        n, h, w, _ = feat.shape
+        out = np.empty((n, h, w, 3), np.float32)
+        self._ck(lib.pt_dn_denoise(self.h, _fptr(feat), n, h, w, _fptr(out)))
+        return out
+
+    def gradients(self, feat: np.ndarray, target: np.ndarray) -> tuple[float, np.ndarray]:
+        feat = np.ascontiguousarray(feat, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w, _ = feat.shape
+        g = np.empty(self.n, np.float32)
+        loss = C.c_float()
+        self._ck(lib.pt_dn_gradients(self.h, _fptr(feat), _fptr(target), n, h, w, C.byref(loss), _fptr(g), self.n))
+        return loss.value, g
+
+    def train_step(self, feat: np.ndarray, target: np.ndarray, lr: float) -> float:
+        feat = np.ascontiguousarray(feat, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        n, h, w, _ = feat.shape
+        loss = C.c_float()
+        self._ck(lib.pt_dn_train_step(self.h, _fptr(feat), _fptr(target), n, h, w, lr, C.byref(loss)))
+        return loss.value

@@ -1,0 +1,681 @@
+// denoise.hip — a small U-Net denoiser for the tracer's low-sample images,
+// trained and run on gfx950 matrix cores (pt_dn_*). No counterpart in the
+// reference.
+//
+// Network (NHWC BF16 activations, FP32 accumulation, FP32 master weights):
+//   in  16 ch: log1p(XYZ) (3), albedo (3), normal (3), 1 / (1 + depth) (1), zeros (6)
+//   e0: conv 16->32, conv 32->32                      (H x W)      skip0
+//   e1: avgpool 2, conv 32->64, conv 64->64           (H/2)        skip1
+//   b : avgpool 2, conv 64->128, conv 128->128        (H/4)
+//   d1: upsample 2, conv [128 | skip1 64]->64, conv 64->64
+//   d0: upsample 2, conv [64 | skip0 32]->32, conv 32->32
+//   out: conv 32->16 (channels 0..2 used), no activation
+// Every conv is 3x3, zero padding 1, bias, ReLU except the output. The output
+// is a correction in log space: log1p(denoised XYZ) = log1p(input XYZ) + out.
+// Loss: mean |log1p(denoised) - log1p(target)| over pixels and the 3 channels.
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_32x32x16_bf16 (lane l holds
+// A[row l % 32][k (l / 32) * 8 + i], B[k (l / 32) * 8 + i][col l % 32]; the
+// 16 results of lane l are rows (v / 4) * 8 + (l / 32) * 4 + v % 4, column
+// l % 32):
+//   forward / backward-data: rows = pixels, columns = output channels,
+//     K = 9 taps x input channels in chunks of 16 (each lane loads 8
+//     consecutive channels: 16 bytes of NHWC); backward-data runs the forward
+//     kernel on the output gradient with the weights flipped and transposed;
+//   backward-weights: rows = output channels, columns = input channels,
+//     K = pixels; each wave stages 16 pixels x 32 channels of both operands in
+//     LDS to read them transposed, and adds its partial sums into the FP32
+//     gradient with atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/pathtracer.h"
+#include "tracer.hpp"
+
+namespace ptdn {
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef unsigned short u16;
+
+constexpr int kIn = 16;      // input channels (10 used)
+constexpr int kOutC = 16;    // output channels (3 used)
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ u16 f2bf(float x) {  // round to nearest even
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u16)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+union Frag {
+    uint4 u;
+    bf8 v;
+};
+
+// ---------------------------------------------------------------------------------
+// Conv forward (also backward-data). X1 (C1 channels) and X2 (C2 channels, may
+// be null) form the input [N, H, W, C1 + C2]; Wt is [9][Cout][C1 + C2] BF16;
+// output channels [0, Co1) go to Y1 (Co1 channels) and [Co1, Cout) to Y2.
+// One wave computes 32 pixels x 32 * NC output channels; a block holds 4 waves
+// along pixels. `flip` reads tap 8 - t (backward-data).
+struct ConvArgs {
+    const u16* X1;
+    const u16* X2;
+    int C1, C2;
+    const u16* Wt;
+    const float* bias;  // null: none
+    u16* Y1;
+    u16* Y2;
+    int Co1, Cout;
+    int N, H, W;
+    int relu;
+    int flip;
+};
+
+template <int NC>
+__global__ void __launch_bounds__(kThreads) k_conv(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = a.N * a.H * a.W;
+    const int pix0 = (blockIdx.x * 4 + wave) * 32;
+    const int co0 = blockIdx.y * 32 * NC;
+    const int Cin = a.C1 + a.C2;
+    // this lane's A row (pixel)
+    const int pm = pix0 + (lane & 31);
+    const bool pin = pm < P;
+    const int pn = pin ? pm / (a.H * a.W) : 0, prem = pin ? pm % (a.H * a.W) : 0;
+    const int py = prem / a.W, px = prem % a.W;
+    const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-chunk
+    f16v acc[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) acc[j] = f16v{};
+    for (int t = 0; t < 9; ++t) {
+        const int tt = a.flip ? 8 - t : t;
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        const int y = py + dy, x = px + dx;
+        const bool in = pin && y >= 0 && y < a.H && x >= 0 && x < a.W;
+        const size_t q = ((size_t)pn * a.H + (in ? y : 0)) * a.W + (in ? x : 0);
+        for (int c0 = 0; c0 < Cin; c0 += 16) {
+            Frag A;
+            A.u = make_uint4(0, 0, 0, 0);
+            if (in) {
+                const int c = c0 + kq;
+                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
+                A.u = *reinterpret_cast<const uint4*>(src);
+            }
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                const int co = co0 + j * 32 + (lane & 31);
+                Frag B;
+                B.u = make_uint4(0, 0, 0, 0);
+                if (co < a.Cout) B.u = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + c0 + kq);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.v, B.v, acc[j], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const int co = co0 + j * 32 + (lane & 31);
+        if (co >= a.Cout) continue;
+        const float b = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int p = pix0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+            if (p >= P) continue;
+            float r = acc[j][v] + b;
+            if (a.relu) r = fmaxf(r, 0.0f);
+            if (co < a.Co1) a.Y1[(size_t)p * a.Co1 + co] = f2bf(r);
+            else a.Y2[(size_t)p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(r);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Conv backward-weights: dW[t][co][ci] += sum over pixels of dZ[p][co] * X[p + tap t][ci]
+// (FP32 atomics), dbias[co] += sum of dZ[p][co] (tap 4 blocks). Grid: x = pixel
+// chunks, y = co tiles of 32, z = 9 taps x ci tiles of 32.
+struct WgradArgs {
+    const u16* dZ;
+    int Cout;
+    const u16* X1;
+    const u16* X2;
+    int C1, C2;
+    float* dW;   // [9][Cout][C1 + C2]
+    float* dB;   // [Cout]
+    int N, H, W;
+    int chunk;   // pixels per block (multiple of 64)
+};
+
+__global__ void __launch_bounds__(kThreads) k_conv_wgrad(WgradArgs a) {
+    __shared__ u16 s_dz[4][16][32 + 8];
+    __shared__ u16 s_x[4][16][32 + 8];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int P = a.N * a.H * a.W;
+    const int Cin = a.C1 + a.C2;
+    const int nci = (Cin + 31) / 32;
+    const int t = blockIdx.z / nci, ci0 = (blockIdx.z % nci) * 32;
+    const int co0 = blockIdx.y * 32;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int pbeg = blockIdx.x * a.chunk, pend = min(P, pbeg + a.chunk);
+    f16v acc = f16v{};
+    float bsum = 0.0f;  // lanes 0..31: this co's sum of dZ (tap 4 only)
+    const int lr = lane >> 2, lc = (lane & 3) * 8;  // staging: pixel row, 8-channel group
+    // each wave takes every 4th group of 16 pixels of the block's chunk
+    for (int g = pbeg + wave * 16; g < pend; g += 64) {
+        const int p = g + lr;
+        Frag dz, xv;
+        dz.u = make_uint4(0, 0, 0, 0);
+        xv.u = make_uint4(0, 0, 0, 0);
+        if (p < pend) {
+            if (co0 + lc < a.Cout) dz.u = *reinterpret_cast<const uint4*>(a.dZ + (size_t)p * a.Cout + co0 + lc);
+            const int n = p / (a.H * a.W), rem = p % (a.H * a.W);
+            const int y = rem / a.W + dy, x = rem % a.W + dx;
+            const int c = ci0 + lc;
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
+                const size_t q = ((size_t)n * a.H + y) * a.W + x;
+                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
+                xv.u = *reinterpret_cast<const uint4*>(src);
+            }
+        }
+        *reinterpret_cast<uint4*>(&s_dz[wave][lr][lc]) = dz.u;
+        *reinterpret_cast<uint4*>(&s_x[wave][lr][lc]) = xv.u;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        Frag A, B;
+        const int kq = (lane >> 5) * 8, col = lane & 31;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            reinterpret_cast<u16*>(&A.u)[i] = s_dz[wave][kq + i][col];
+            reinterpret_cast<u16*>(&B.u)[i] = s_x[wave][kq + i][col];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.v, B.v, acc, 0, 0, 0);
+        if (t == 4 && blockIdx.z % nci == 0 && lane < 32)
+            for (int i = 0; i < 16; ++i) bsum += bf2f(s_dz[wave][i][lane]);
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int co = co0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3), ci = ci0 + (lane & 31);
+        if (co < a.Cout && ci < Cin && acc[v] != 0.0f) atomicAdd(&a.dW[((size_t)t * a.Cout + co) * Cin + ci], acc[v]);
+    }
+    if (t == 4 && blockIdx.z % nci == 0 && lane < 32 && co0 + lane < a.Cout && bsum != 0.0f)
+        atomicAdd(&a.dB[co0 + lane], bsum);
+}
+
+// ---------------------------------------------------------------------------------
+// Elementwise kernels (NHWC BF16).
+__global__ void k_pool2(const u16* x, u16* y, int N, int H, int W, int C) {  // 2x2 average
+    const int Ho = H / 2, Wo = W / 2;
+    const size_t n = (size_t)N * Ho * Wo * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t r = i / C;
+        const int xo = (int)(r % Wo);
+        r /= Wo;
+        const int yo = (int)(r % Ho), b = (int)(r / Ho);
+        const size_t q = ((size_t)b * H + 2 * yo) * W + 2 * xo;
+        float s = bf2f(x[q * C + c]) + bf2f(x[(q + 1) * C + c]) + bf2f(x[(q + W) * C + c]) + bf2f(x[(q + W + 1) * C + c]);
+        y[i] = f2bf(0.25f * s);
+    }
+}
+// gradient of the 2x2 average: dX[q] += 0.25 dY (dX written, not accumulated)
+__global__ void k_pool2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) {
+    const size_t n = (size_t)N * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t r = i / C;
+        const int x = (int)(r % W);
+        r /= W;
+        const int y = (int)(r % H), b = (int)(r / H);
+        const size_t o = (((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c;
+        dx[i] = f2bf(0.25f * bf2f(dy[o]));
+    }
+}
+// nearest 2x upsample: y [N, 2H, 2W, C] from x [N, H, W, C]
+__global__ void k_up2(const u16* x, u16* y, int N, int H, int W, int C) {
+    const size_t n = (size_t)N * 2 * H * 2 * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t r = i / C;
+        const int xo = (int)(r % (2 * W));
+        r /= 2 * W;
+        const int yo = (int)(r % (2 * H)), b = (int)(r / (2 * H));
+        y[i] = x[(((size_t)b * H + yo / 2) * W + xo / 2) * C + c];
+    }
+}
+// gradient of the upsample: dX = sum of the 4 dY it fed
+__global__ void k_up2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) {
+    const size_t n = (size_t)N * H * W * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        size_t r = i / C;
+        const int x = (int)(r % W);
+        r /= W;
+        const int y = (int)(r % H), b = (int)(r / H);
+        const size_t q = ((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x;
+        float s = bf2f(dy[q * C + c]) + bf2f(dy[(q + 1) * C + c]) + bf2f(dy[(q + 2 * W) * C + c]) +
+                  bf2f(dy[(q + 2 * W + 1) * C + c]);
+        dx[i] = f2bf(s);
+    }
+}
+// dZ = dY * (Y > 0): gradient through the ReLU that produced Y; a second
+// gradient `add` (skip connection) is summed in when given
+__global__ void k_relu_bwd(const u16* dy, const u16* add, const u16* y, u16* dz, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float g = bf2f(dy[i]) + (add ? bf2f(add[i]) : 0.0f);
+        dz[i] = f2bf(bf2f(y[i]) > 0.0f ? g : 0.0f);
+    }
+}
+// features (N, H, W, 10) FP32 -> input tensor (16 channels BF16)
+__global__ void k_prep(const float* f, u16* x, size_t P) {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        const float* s = f + p * 10;
+        u16* d = x + p * kIn;
+        for (int c = 0; c < 3; ++c) d[c] = f2bf(log1pf(fmaxf(s[c], 0.0f)));
+        for (int c = 3; c < 9; ++c) d[c] = f2bf(s[c]);
+        d[9] = f2bf(1.0f / (1.0f + fmaxf(s[9], 0.0f)));
+        for (int c = 10; c < kIn; ++c) d[c] = 0;
+    }
+}
+// prediction log1p(denoised) = log1p(in) + out[0..2]; loss and its gradient
+// (L1 against log1p(target), mean over 3P values); also the denoised image.
+__global__ void k_head(const float* f, const u16* out, const float* target, float* denoised, u16* dout,
+                       float* loss, size_t P) {
+    float l = 0.0f;
+    const float inv = 1.0f / (3.0f * (float)P);
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < kOutC; ++c) {
+            if (c >= 3) {
+                if (dout) dout[p * kOutC + c] = 0;
+                continue;
+            }
+            const float z = log1pf(fmaxf(f[p * 10 + c], 0.0f)) + bf2f(out[p * kOutC + c]);
+            if (denoised) denoised[p * 3 + c] = fmaxf(expm1f(z), 0.0f);
+            if (target) {
+                const float d = z - log1pf(fmaxf(target[p * 3 + c], 0.0f));
+                l += fabsf(d);
+                if (dout) dout[p * kOutC + c] = f2bf(d > 0.0f ? inv : (d < 0.0f ? -inv : 0.0f));
+            }
+        }
+    }
+    if (loss) {
+        for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
+        if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, l * inv);
+    }
+}
+// Adam on FP32 masters; then the BF16 forward copy Wf[t][co][ci] and the
+// backward-data copy Wb[t][ci][co] (k_conv with `flip` reads tap 8 - t: the
+// gradient of tap t's offset o is gathered from offset -o, which is tap 8 - t).
+__global__ void k_adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+__global__ void k_wcopy(const float* w, u16* wf, u16* wb, int Cout, int Cin) {
+    const int n = 9 * Cout * Cin;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ci = i % Cin, co = (i / Cin) % Cout, t = i / (Cin * Cout);
+        const u16 h = f2bf(w[i]);
+        wf[i] = h;
+        wb[((size_t)t * Cin + ci) * Cout + co] = h;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Host side.
+struct Layer {
+    int cin1, cin2, cout;  // input split (concat), output channels
+    bool relu;
+    size_t w_off, b_off;   // into the FP32 parameter vector
+    u16 *wf = nullptr, *wb = nullptr;
+};
+
+}  // namespace ptdn
+
+using namespace ptdn;
+
+struct pt_denoiser {
+    pt_ctx* ctx = nullptr;
+    int maxN = 0, maxH = 0, maxW = 0;
+    std::vector<Layer> L;
+    size_t nparam = 0;
+    int64_t step = 0;
+    DevAlloc mem;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    u16* wbf = nullptr;  // BF16 copies, 2 per weight
+    // activations (BF16) and gradients
+    u16 *x0 = nullptr, *e0a = nullptr, *e0 = nullptr, *p1 = nullptr, *e1a = nullptr, *e1 = nullptr, *p2 = nullptr,
+        *ba = nullptr, *bb = nullptr, *u1 = nullptr, *d1a = nullptr, *d1 = nullptr, *u0 = nullptr, *d0a = nullptr,
+        *d0 = nullptr, *out = nullptr;
+    u16 *ga = nullptr, *gb = nullptr, *gc = nullptr, *gs1 = nullptr, *gs0 = nullptr;  // gradient scratch
+    float *feat = nullptr, *tgt = nullptr, *den = nullptr, *loss = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int dn_fail(pt_denoiser* d, int st, const std::string& msg) {
+    d->err = msg;
+    return st;
+}
+
+void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, int N, int H, int W, bool bwd_data,
+          u16* y2 = nullptr, int co1 = 0) {
+    ConvArgs a{};
+    a.N = N;
+    a.H = H;
+    a.W = W;
+    if (!bwd_data) {
+        a.X1 = x1; a.X2 = x2; a.C1 = l.cin1; a.C2 = l.cin2;
+        a.Wt = l.wf; a.bias = d->w + l.b_off;
+        a.Y1 = y; a.Y2 = nullptr; a.Co1 = l.cout; a.Cout = l.cout;
+        a.relu = l.relu; a.flip = 0;
+    } else {  // input = dZ (cout channels), output = dX split (cin1 | cin2)
+        a.X1 = x1; a.X2 = nullptr; a.C1 = l.cout; a.C2 = 0;
+        a.Wt = l.wb; a.bias = nullptr;
+        a.Y1 = y; a.Y2 = y2; a.Co1 = co1 ? co1 : l.cin1 + l.cin2; a.Cout = l.cin1 + l.cin2;
+        a.relu = 0; a.flip = 1;
+    }
+    const int P = N * H * W;
+    const int nc = a.Cout >= 128 ? 4 : (a.Cout >= 64 ? 2 : 1);
+    dim3 grid((P + 127) / 128, (a.Cout + 32 * nc - 1) / (32 * nc));
+    if (nc == 4) ptk::launch(d->ctx, "k_dn_conv", k_conv<4>, grid, dim3(kThreads), a);
+    else if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv<2>, grid, dim3(kThreads), a);
+    else ptk::launch(d->ctx, "k_dn_conv", k_conv<1>, grid, dim3(kThreads), a);
+}
+
+void wgrad(pt_denoiser* d, const Layer& l, const u16* dz, const u16* x1, const u16* x2, int N, int H, int W) {
+    WgradArgs a{};
+    a.dZ = dz; a.Cout = l.cout; a.X1 = x1; a.X2 = x2; a.C1 = l.cin1; a.C2 = l.cin2;
+    a.dW = d->g + l.w_off; a.dB = d->g + l.b_off;
+    a.N = N; a.H = H; a.W = W;
+    const int P = N * H * W;
+    const int cin = l.cin1 + l.cin2;
+    // about 2048 pixels per block, at least 16 blocks per (tap, tile)
+    a.chunk = std::max(64, std::min(2048, (P / 16 + 63) / 64 * 64));
+    dim3 grid((P + a.chunk - 1) / a.chunk, (l.cout + 31) / 32, 9 * ((cin + 31) / 32));
+    ptk::launch(d->ctx, "k_dn_wgrad", k_conv_wgrad, grid, dim3(kThreads), a);
+}
+
+dim3 ew_grid(size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)); }
+
+// Forward pass of features already in d->feat (N images H x W).
+void forward(pt_denoiser* d, int N, int H, int W) {
+    pt_ctx* ctx = d->ctx;
+    const int P = N * H * W, P1 = P / 4, P2 = P / 16;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
+    ptk::launch(ctx, "k_dn_prep", k_prep, ew_grid(P), dim3(256), (const float*)d->feat, d->x0, (size_t)P);
+    conv(d, d->L[0], d->x0, nullptr, d->e0a, N, H, W, false);
+    conv(d, d->L[1], d->e0a, nullptr, d->e0, N, H, W, false);
+    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P1 * 32), dim3(256), (const u16*)d->e0, d->p1, N, H, W, 32);
+    conv(d, d->L[2], d->p1, nullptr, d->e1a, N, H1, W1, false);
+    conv(d, d->L[3], d->e1a, nullptr, d->e1, N, H1, W1, false);
+    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P2 * 64), dim3(256), (const u16*)d->e1, d->p2, N, H1, W1, 64);
+    conv(d, d->L[4], d->p2, nullptr, d->ba, N, H2, W2, false);
+    conv(d, d->L[5], d->ba, nullptr, d->bb, N, H2, W2, false);
+    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P1 * 128), dim3(256), (const u16*)d->bb, d->u1, N, H2, W2, 128);
+    conv(d, d->L[6], d->u1, d->e1, d->d1a, N, H1, W1, false);
+    conv(d, d->L[7], d->d1a, nullptr, d->d1, N, H1, W1, false);
+    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P * 64), dim3(256), (const u16*)d->d1, d->u0, N, H1, W1, 64);
+    conv(d, d->L[8], d->u0, d->e0, d->d0a, N, H, W, false);
+    conv(d, d->L[9], d->d0a, nullptr, d->d0, N, H, W, false);
+    conv(d, d->L[10], d->d0, nullptr, d->out, N, H, W, false);
+}
+
+// Backward pass: d->gc holds dL/d(out) (kOutC channels); gradients summed into d->g.
+void backward(pt_denoiser* d, int N, int H, int W) {
+    pt_ctx* ctx = d->ctx;
+    const int P = N * H * W, P1 = P / 4, P2 = P / 16;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
+    auto relu_bwd = [&](const u16* dy, const u16* add, const u16* y, u16* dz, size_t n) {
+        ptk::launch(ctx, "k_dn_relu_bwd", k_relu_bwd, ew_grid(n), dim3(256), dy, add, y, dz, n);
+    };
+    // out = conv10(d0): no activation
+    wgrad(d, d->L[10], d->gc, d->d0, nullptr, N, H, W);
+    conv(d, d->L[10], d->gc, nullptr, d->ga, N, H, W, true);          // dL/d d0 (32)
+    relu_bwd(d->ga, nullptr, d->d0, d->gb, (size_t)P * 32);            // dZ9
+    wgrad(d, d->L[9], d->gb, d->d0a, nullptr, N, H, W);
+    conv(d, d->L[9], d->gb, nullptr, d->ga, N, H, W, true);           // dL/d d0a (32)
+    relu_bwd(d->ga, nullptr, d->d0a, d->gb, (size_t)P * 32);           // dZ8
+    wgrad(d, d->L[8], d->gb, d->u0, d->e0, N, H, W);
+    conv(d, d->L[8], d->gb, nullptr, d->gc, N, H, W, true, d->gs0, 64);  // dL/d u0 (64) | dL/d e0 via skip (32)
+    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P1 * 64), dim3(256), (const u16*)d->gc, d->ga, N, H1, W1,
+                64);                                                   // dL/d d1 (64)
+    relu_bwd(d->ga, nullptr, d->d1, d->gb, (size_t)P1 * 64);           // dZ7
+    wgrad(d, d->L[7], d->gb, d->d1a, nullptr, N, H1, W1);
+    conv(d, d->L[7], d->gb, nullptr, d->ga, N, H1, W1, true);         // dL/d d1a (64)
+    relu_bwd(d->ga, nullptr, d->d1a, d->gb, (size_t)P1 * 64);          // dZ6
+    wgrad(d, d->L[6], d->gb, d->u1, d->e1, N, H1, W1);
+    conv(d, d->L[6], d->gb, nullptr, d->gc, N, H1, W1, true, d->gs1, 128);  // dL/d u1 (128) | dL/d e1 via skip (64)
+    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P2 * 128), dim3(256), (const u16*)d->gc, d->ga, N, H2,
+                W2, 128);                                              // dL/d bb (128)
+    relu_bwd(d->ga, nullptr, d->bb, d->gb, (size_t)P2 * 128);          // dZ5
+    wgrad(d, d->L[5], d->gb, d->ba, nullptr, N, H2, W2);
+    conv(d, d->L[5], d->gb, nullptr, d->ga, N, H2, W2, true);         // dL/d ba
+    relu_bwd(d->ga, nullptr, d->ba, d->gb, (size_t)P2 * 128);          // dZ4
+    wgrad(d, d->L[4], d->gb, d->p2, nullptr, N, H2, W2);
+    conv(d, d->L[4], d->gb, nullptr, d->ga, N, H2, W2, true);         // dL/d p2 (64)
+    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P1 * 64), dim3(256), (const u16*)d->ga, d->gc, N, H1,
+                W1, 64);                                               // dL/d e1 via pool (64)
+    relu_bwd(d->gc, d->gs1, d->e1, d->gb, (size_t)P1 * 64);            // dZ3 (pool + skip)
+    wgrad(d, d->L[3], d->gb, d->e1a, nullptr, N, H1, W1);
+    conv(d, d->L[3], d->gb, nullptr, d->ga, N, H1, W1, true);         // dL/d e1a
+    relu_bwd(d->ga, nullptr, d->e1a, d->gb, (size_t)P1 * 64);          // dZ2
+    wgrad(d, d->L[2], d->gb, d->p1, nullptr, N, H1, W1);
+    conv(d, d->L[2], d->gb, nullptr, d->ga, N, H1, W1, true);         // dL/d p1 (32)
+    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P * 32), dim3(256), (const u16*)d->ga, d->gc, N, H, W,
+                32);                                                   // dL/d e0 via pool (32)
+    relu_bwd(d->gc, d->gs0, d->e0, d->gb, (size_t)P * 32);             // dZ1 (pool + skip)
+    wgrad(d, d->L[1], d->gb, d->e0a, nullptr, N, H, W);
+    conv(d, d->L[1], d->gb, nullptr, d->ga, N, H, W, true);           // dL/d e0a
+    relu_bwd(d->ga, nullptr, d->e0a, d->gb, (size_t)P * 32);           // dZ0
+    wgrad(d, d->L[0], d->gb, d->x0, nullptr, N, H, W);
+}
+
+void refresh_bf16(pt_denoiser* d) {
+    for (const Layer& l : d->L)
+        ptk::launch(d->ctx, "k_dn_wcopy", k_wcopy, ew_grid((size_t)9 * l.cout * (l.cin1 + l.cin2)), dim3(256),
+                    (const float*)(d->w + l.w_off), l.wf, l.wb, l.cout, l.cin1 + l.cin2);
+}
+
+int upload(pt_denoiser* d, float* dst, const float* src, size_t n) {
+    hipError_t e = hipMemcpyAsync(dst, src, n * 4, hipMemcpyHostToDevice, d->ctx->stream);
+    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(d->ctx, e, "denoiser upload"), d->ctx->err);
+    return PT_OK;
+}
+
+int check_shape(pt_denoiser* d, int N, int H, int W) {
+    if (N <= 0 || H <= 0 || W <= 0 || N > d->maxN || H > d->maxH || W > d->maxW || H % 4 || W % 4)
+        return dn_fail(d, PT_ERR_INVALID_ARG, "denoiser: batch shape outside the network's limits (H, W multiples of 4)");
+    return PT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width, uint64_t seed, pt_denoiser** out) {
+    if (!ctx || !out || max_batch <= 0 || max_height <= 0 || max_width <= 0 || max_height % 4 || max_width % 4)
+        return PT_ERR_INVALID_ARG;
+    *out = nullptr;
+    pt_denoiser* d = new (std::nothrow) pt_denoiser();
+    if (!d) return PT_ERR_INVALID_ARG;
+    d->ctx = ctx;
+    d->maxN = max_batch;
+    d->maxH = max_height;
+    d->maxW = max_width;
+    const int spec[11][4] = {{kIn, 0, 32, 1}, {32, 0, 32, 1}, {32, 0, 64, 1}, {64, 0, 64, 1}, {64, 0, 128, 1},
+                             {128, 0, 128, 1}, {128, 64, 64, 1}, {64, 0, 64, 1}, {64, 32, 32, 1}, {32, 0, 32, 1},
+                             {32, 0, kOutC, 0}};
+    size_t off = 0;
+    for (auto& sp : spec) {
+        Layer l;
+        l.cin1 = sp[0]; l.cin2 = sp[1]; l.cout = sp[2]; l.relu = sp[3] != 0;
+        l.w_off = off;
+        off += (size_t)9 * l.cout * (l.cin1 + l.cin2);
+        l.b_off = off;
+        off += (size_t)l.cout;
+        d->L.push_back(l);
+    }
+    d->nparam = off;
+    const size_t P = (size_t)max_batch * max_height * max_width;
+    // activations: BF16 element counts
+    const size_t act[] = {P * kIn, P * 32, P * 32, P / 4 * 32, P / 4 * 64, P / 4 * 64, P / 16 * 64, P / 16 * 128,
+                          P / 16 * 128, P / 4 * 128, P / 4 * 64, P / 4 * 64, P * 64, P * 32, P * 32, P * kOutC,
+                          P * 64, P * 64, P * 64, P / 4 * 64, P * 32};
+    size_t bytes = 4 * off * 4 + 2 * off * 2 + (P * 10 + P * 3 + P * 3 + 64) * 4 + 64 * 256;
+    for (size_t a : act) bytes += (a * 2 + 255) / 256 * 256;
+    int st = ptk::dev_alloc(ctx, d->mem, bytes);
+    if (st != PT_OK) {
+        delete d;
+        return st;
+    }
+    char* p = static_cast<char*>(d->mem.p);
+    auto carve = [&](size_t b) {
+        char* q = p;
+        p += (b + 255) / 256 * 256;
+        return q;
+    };
+    d->w = reinterpret_cast<float*>(carve(off * 4));
+    d->g = reinterpret_cast<float*>(carve(off * 4));
+    d->m = reinterpret_cast<float*>(carve(off * 4));
+    d->v = reinterpret_cast<float*>(carve(off * 4));
+    d->wbf = reinterpret_cast<u16*>(carve(off * 4));
+    u16** acts[] = {&d->x0, &d->e0a, &d->e0, &d->p1, &d->e1a, &d->e1, &d->p2, &d->ba, &d->bb, &d->u1, &d->d1a, &d->d1,
+                    &d->u0, &d->d0a, &d->d0, &d->out, &d->ga, &d->gb, &d->gc, &d->gs1, &d->gs0};
+    for (size_t i = 0; i < sizeof(act) / sizeof(act[0]); ++i) *acts[i] = reinterpret_cast<u16*>(carve(act[i] * 2));
+    d->feat = reinterpret_cast<float*>(carve(P * 10 * 4));
+    d->tgt = reinterpret_cast<float*>(carve(P * 3 * 4));
+    d->den = reinterpret_cast<float*>(carve(P * 3 * 4));
+    d->loss = reinterpret_cast<float*>(carve(64 * 4));
+    u16* wb = d->wbf;
+    for (Layer& l : d->L) {
+        const size_t n = (size_t)9 * l.cout * (l.cin1 + l.cin2);
+        l.wf = wb;
+        l.wb = wb + n;
+        wb += 2 * n;
+    }
+    // He-normal weights (inputs of the first layer beyond the 10 features are zero), zero biases
+    std::vector<float> h(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    for (const Layer& l : d->L) {
+        const int cin = l.cin1 + l.cin2;
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (9.0f * (float)cin)));
+        for (size_t i = 0; i < (size_t)9 * l.cout * cin; ++i) h[l.w_off + i] = nd(rng);
+    }
+    for (int t = 0; t < 9; ++t)  // the output layer starts small: the network starts near the identity
+        for (int co = 0; co < kOutC; ++co)
+            for (int ci = 0; ci < 32; ++ci) h[d->L[10].w_off + ((size_t)t * kOutC + co) * 32 + ci] *= 0.1f;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipMemcpyAsync(d->w, h.data(), off * 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d->m, 0, off * 8, ctx->stream);  // m and v are adjacent
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ptk::dev_free(ctx, d->mem);
+        delete d;
+        return ptk::hip_fail(ctx, e, "pt_dn_create");
+    }
+    refresh_bf16(d);
+    *out = d;
+    return PT_OK;
+}
+
+void pt_dn_destroy(pt_denoiser* d) {
+    if (!d) return;
+    (void)hipSetDevice(d->ctx->device);
+    (void)hipStreamSynchronize(d->ctx->stream);
+    ptk::dev_free(d->ctx, d->mem);
+    delete d;
+}
+
+const char* pt_dn_last_error(const pt_denoiser* d) { return d ? d->err.c_str() : ""; }
+
+int64_t pt_dn_param_count(const pt_denoiser* d) { return d ? (int64_t)d->nparam : 0; }
+
+pt_status pt_dn_get_params(pt_denoiser* d, float* host, size_t n) {
+    if (!d || !host || n != d->nparam) return PT_ERR_INVALID_ARG;
+    hipError_t e = hipMemcpyAsync(host, d->w, n * 4, hipMemcpyDeviceToHost, d->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->ctx->stream);
+    return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(d->ctx, e, "pt_dn_get_params"), d->ctx->err);
+}
+
+pt_status pt_dn_set_params(pt_denoiser* d, const float* host, size_t n) {
+    if (!d || !host || n != d->nparam) return PT_ERR_INVALID_ARG;
+    int st = upload(d, d->w, host, n);
+    if (st != PT_OK) return st;
+    refresh_bf16(d);
+    hipError_t e = hipStreamSynchronize(d->ctx->stream);
+    return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(d->ctx, e, "pt_dn_set_params"), d->ctx->err);
+}
+
+pt_status pt_dn_denoise(pt_denoiser* d, const float* features, int N, int H, int W, float* denoised) {
+    if (!d || !features || !denoised) return PT_ERR_INVALID_ARG;
+    int st = check_shape(d, N, H, W);
+    if (st != PT_OK) return st;
+    const size_t P = (size_t)N * H * W;
+    (void)hipSetDevice(d->ctx->device);
+    st = upload(d, d->feat, features, P * 10);
+    if (st != PT_OK) return st;
+    forward(d, N, H, W);
+    ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
+                (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, P);
+    hipError_t e = hipMemcpyAsync(denoised, d->den, P * 3 * 4, hipMemcpyDeviceToHost, d->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(d->ctx->stream);
+    return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(d->ctx, e, "pt_dn_denoise"), d->ctx->err);
+}
+
+pt_status pt_dn_gradients(pt_denoiser* d, const float* features, const float* target, int N, int H, int W,
+                          float* loss, float* host_grads, size_t n_grads) {
+    if (!d || !features || !target) return PT_ERR_INVALID_ARG;
+    if (host_grads && n_grads != d->nparam) return PT_ERR_INVALID_ARG;
+    int st = check_shape(d, N, H, W);
+    if (st != PT_OK) return st;
+    const size_t P = (size_t)N * H * W;
+    pt_ctx* ctx = d->ctx;
+    (void)hipSetDevice(ctx->device);
+    st = upload(d, d->feat, features, P * 10);
+    if (st == PT_OK) st = upload(d, d->tgt, target, P * 3);
+    if (st != PT_OK) return st;
+    hipError_t e = hipMemsetAsync(d->g, 0, d->nparam * 4, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d->loss, 0, 4, ctx->stream);
+    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "pt_dn_gradients"), ctx->err);
+    forward(d, N, H, W);
+    ptk::launch(ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
+                (const float*)d->tgt, (float*)nullptr, d->gc, d->loss, P);
+    backward(d, N, H, W);
+    if (loss) e = hipMemcpyAsync(loss, d->loss, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && host_grads) e = hipMemcpyAsync(host_grads, d->g, n_grads * 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(ctx, e, "pt_dn_gradients"), ctx->err);
+}
+
+pt_status pt_dn_train_step(pt_denoiser* d, const float* features, const float* target, int N, int H, int W, float lr,
+                           float* loss) {
+    int st = pt_dn_gradients(d, features, target, N, H, W, loss, nullptr, 0);
+    if (st != PT_OK) return st;
+    d->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)d->step), c2 = 1.0f - std::pow(b2, (float)d->step);
+    ptk::launch(d->ctx, "k_dn_adam", k_adam, ew_grid(d->nparam), dim3(256), d->w, (const float*)d->g, d->m, d->v,
+                d->nparam, lr, b1, b2, 1e-8f, c1, c2);
+    refresh_bf16(d);
+    hipError_t e = hipStreamSynchronize(d->ctx->stream);
+    return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(d->ctx, e, "pt_dn_train_step"), d->ctx->err);
+}
+
+}  // extern "C"

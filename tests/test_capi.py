@@ -11,7 +11,7 @@ import pathtracer_amd as pt
 from tests.conftest import ROOT
 
 HEADER = (ROOT / "include" / "pathtracer.h").read_text()
-DECLARED = sorted(set(re.findall(r"^\s*(?:pt_status|void|double|const char\*)\s+(pt_\w+)\s*\(", HEADER, re.M)))
+DECLARED = sorted(set(re.findall(r"^\s*(?:pt_status|void|double|int64_t|const char\*)\s+(pt_\w+)\s*\(", HEADER, re.M)))
 
 
 def test_header_declarations_found():
