@@ -223,6 +223,19 @@ pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* scene, pt_accum* acc, const p
                       pt_mode mode);
 pt_status pt_sync(pt_ctx* ctx);
 
+/* Denoiser features (extended mode; no counterpart in the reference, which
+ * shows the noisy running mean, src/shader.frag:81-99): traces the samples
+ * [currentSamples - samplesPerFrame, currentSamples) of every pixel, as
+ * pt_dispatch would, and writes per pixel the mean over those samples of 10
+ * floats: XYZ (the accumulator's texel units: exposure apertureSize^2 * ISO
+ * applied), the first hit's albedo
+ * (reflectance at 450, 550 and 650 nm; 1 for glass and emitters; the
+ * normal-incidence reflectance of a conductor), its shading normal facing the
+ * camera and its distance; zeros for rays that miss. Samples with a NaN are
+ * dropped. Synchronous; `n_floats` must be width * height * 10. */
+pt_status pt_render_features(pt_ctx* ctx, const pt_scene* scene, const pt_push_constants* pc, float* host_feat,
+                             size_t n_floats);
+
 /* Counters summed over dispatches since creation or the last reset (synchronises). */
 pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out);
 pt_status pt_counters_reset(pt_ctx* ctx);

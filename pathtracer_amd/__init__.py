@@ -205,6 +205,7 @@ SIGNATURES = {
     "pt_accum_write_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "pt_accum_release": (None, [_P]),
     "pt_dispatch": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants), C.c_int]),
+    "pt_render_features": (C.c_int, [_P, _P, C.POINTER(PushConstants), C.POINTER(C.c_float), C.c_size_t]),
     "pt_sync": (C.c_int, [_P]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),
     "pt_counters_reset": (C.c_int, [_P]),
@@ -369,6 +370,16 @@ class Context:
 
     def dispatch(self, scene: "Scene", acc: "Accum", pc: PushConstants, mode: int = MODE_REF) -> None:
         self.check(lib.pt_dispatch(self.h, scene.h, acc.h, C.byref(pc), mode))
+
+    def features(self, scene: "Scene", pc: PushConstants) -> np.ndarray:
+        """Per-pixel means over the samples [currentSamples - samplesPerFrame,
+        currentSamples) of XYZ, first-hit albedo, normal and distance:
+        (height, width, 10) float32."""
+        w, h = pc.resolution[0], pc.resolution[1]
+        out = np.empty((h, w, 10), np.float32)
+        self.check(lib.pt_render_features(self.h, scene.h, C.byref(pc),
+                                          out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
+        return out
 
     def sync(self) -> None:
         self.check(lib.pt_sync(self.h))

@@ -377,6 +377,55 @@ pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_pu
     }
 }
 
+pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* host, size_t n_floats) {
+    if (!ctx || !s || !pc || !host) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: NULL argument");
+    if (s->ctx != ctx) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: scene of another context");
+    if (!s->has_model) return set_err(ctx, PT_ERR_STATE, "pt_render_features: needs a scene loaded from JSON");
+    const int w = pc->resolution[0], h = pc->resolution[1];
+    if (w <= 0 || h <= 0 || pc->samplesPerFrame <= 0 || pc->pathLength < 0 || pc->pathLength > 1024 ||
+        pc->currentSamples < pc->samplesPerFrame)
+        return set_err(ctx, PT_ERR_INVALID_ARG,
+                       "pt_render_features: need a positive resolution, samplesPerFrame > 0, 0 <= pathLength <= 1024, "
+                       "currentSamples >= samplesPerFrame");
+    const size_t P = (size_t)w * h;
+    if (n_floats != P * 10) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: size mismatch");
+    (void)hipSetDevice(ctx->device);
+    DevAlloc m;
+    const size_t feat_bytes = (P * 40 + 255) / 256 * 256;
+    int st = ptk::dev_alloc(ctx, m, feat_bytes + P * 4);
+    if (st != PT_OK) return st;
+    float* d_feat = (float*)m.p;
+    int* d_n = (int*)((char*)m.p + feat_bytes);
+    std::vector<int> n(P);
+    try {
+        hipError_t e = hipMemsetAsync(m.p, 0, feat_bytes + P * 4, ctx->stream);
+        if (e != hipSuccess) st = ptk::hip_fail(ctx, e, "pt_render_features");
+        if (st == PT_OK) st = ptk::ext_dispatch_features(ctx, s, pc, d_feat, d_n);
+        if (st == PT_OK) st = ptk::count_dispatch(ctx);
+        if (st == PT_OK) {
+            e = hipMemcpyAsync(host, d_feat, P * 40, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(n.data(), d_n, P * 4, hipMemcpyDeviceToHost, ctx->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) st = ptk::hip_fail(ctx, e, "pt_render_features");
+        }
+    } catch (const std::exception& ex) {
+        st = set_err(ctx, PT_ERR_HIP, std::string("pt_render_features: ") + ex.what());
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    ptk::dev_free(ctx, m);
+    if (st != PT_OK) return st;
+    // XYZ in the accumulator's units: exposure apertureSize^2 * ISO per sample,
+    // as the texel resolve applies it
+    const float ex = pc->apertureSize * pc->apertureSize * (float)pc->ISO;
+    for (size_t i = 0; i < P; ++i) {
+        const float inv = n[i] > 0 ? 1.0f / (float)n[i] : 0.0f;
+        const float e = n[i] > 0 ? ex / (float)n[i] : 0.0f;
+        for (int c = 0; c < 3; ++c) host[i * 10 + c] *= e;
+        for (int c = 3; c < 10; ++c) host[i * 10 + c] *= inv;
+    }
+    return PT_OK;
+}
+
 pt_status pt_sync(pt_ctx* ctx) {
     if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_sync: NULL context");
     (void)hipSetDevice(ctx->device);

@@ -130,6 +130,8 @@ struct ExtWave {
     float4* s_org; // shadow ray origin, w = tmax
     float4* s_dir; // direction, w = path index bits
     float4* s_xyz; // unoccluded contribution
+    float4* aov0;  // first hit: albedo (xyz), distance (w)
+    float4* aov1;  // first hit: shading normal (xyz)
     float* sum;
     int* counts;   // N_CNT queue lengths, kCntStride apart
     int* cursor;   // 2 x kSeg persistent-kernel cursors, kCntStride apart
@@ -1045,6 +1047,69 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
     }
 }
 
+// First-hit features of the camera paths (queue 0 right after bounce 0's
+// trace): an albedo triple (reflectance at 450 / 550 / 650 nm; 1 for glass and
+// emitters; the normal-incidence reflectance of a conductor), the shading
+// normal facing the camera and the hit distance; zeros for a miss.
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_aov(ExtSceneDev S0, ExtBatch B, ExtWave Wv) {
+    const int n = *qlen(Wv, 0);
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        const int p = Wv.q[0][idx];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+        const float2 h = Wv.hit[p];
+        const uint32_t ref = __float_as_uint(h.y);
+        float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
+        if (hit_class(S, ref) != Q_MISS) {
+            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            Surf sf = surface(S, ref, o, d, h.x);
+            f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
+            f3 al = mk3(1.0f, 1.0f, 1.0f);
+            const ExtMat& M = S.mat[sf.mat];
+            if (sf.light < 0) {
+                if (M.type == pt::MAT_CONDUCTOR) {
+                    float f0 = fresnel_conductor(1.0f, M.p[0], M.p[1]);
+                    al = mk3(f0, f0, f0);
+                } else if (M.type != pt::MAT_DIELECTRIC) {
+                    al = mk3(reflectance(M, 450.0f), reflectance(M, 550.0f), reflectance(M, 650.0f));
+                }
+            }
+            a0 = make_float4(al.x, al.y, al.z, h.x);
+            a1 = make_float4(N.x, N.y, N.z, 0.0f);
+        }
+        Wv.aov0[p] = a0;
+        Wv.aov1[p] = a1;
+    }
+}
+
+// Features sink: per pixel, in sample order, sums of the per-sample XYZ and
+// first-hit features into feat[pixel * 10 + (XYZ 0..2, albedo 3..5,
+// normal 6..8, distance 9)]; samples with a NaN are dropped and counted.
+__global__ void __launch_bounds__(kBlk) k_ext_features(ExtFrame F, ExtWave Wv, float* feat, int* nsamp) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned nans = 0;
+    if (j < F.pc) {
+        float* f = feat + (size_t)(F.p0 + j) * 10;
+        float s[10];
+        for (int c = 0; c < 10; ++c) s[c] = f[c];
+        int n = nsamp[F.p0 + j];
+        for (int k = 0; k < F.m; ++k) {
+            const int p = k * F.pc + j;
+            float4 a = Wv.acc[p];
+            if (a.x != a.x || a.y != a.y || a.z != a.z) { ++nans; continue; }
+            float4 b0 = Wv.aov0[p], b1 = Wv.aov1[p];
+            s[0] += a.x; s[1] += a.y; s[2] += a.z;
+            s[3] += b0.x; s[4] += b0.y; s[5] += b0.z;
+            s[6] += b1.x; s[7] += b1.y; s[8] += b1.z;
+            s[9] += b0.w;
+            ++n;
+        }
+        for (int c = 0; c < 10; ++c) f[c] = s[c];
+        nsamp[F.p0 + j] = n;
+    }
+    wave_add_u64(stat_slot(Wv.stats, 4), nans);
+}
+
 // Russian roulette from depth rr_depth; returns false when the path ends.
 __device__ __forceinline__ bool roulette(const ExtSceneDev& S, uint32_t key, int bounce, f4& T) {
     if (bounce + 1 < S.rr_depth) return true;
@@ -1850,8 +1915,8 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     size_t n = (size_t)cap;
     // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
     // shadow org/dir/xyz 48, sums 12
-    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 12;
-    // + 256-byte rounding of 20 arrays + the counters and cursors, one line each
+    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 12;
+    // + 256-byte rounding of 22 arrays + the counters and cursors, one line each
     int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + 2 * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
     char* p = static_cast<char*>(st->mem.p);
@@ -1862,6 +1927,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     W.q[0] = carve_e<int>(p, n); W.q[1] = carve_e<int>(p, n);
     for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
     W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
+    W.aov0 = carve_e<float4>(p, n); W.aov1 = carve_e<float4>(p, n);
     W.sum = carve_e<float>(p, 3 * n);
     W.counts = carve_e<int>(p, (size_t)N_CNT * kCntStride);
     W.cursor = carve_e<int>(p, (size_t)2 * kSeg * kCntStride);
@@ -1891,6 +1957,7 @@ struct ExtPlan {
     std::vector<int> sdf_kinds;
     int width = 2, stack_need = 1;
     bool small_nodes = true;
+    bool aov = false;  // record first-hit features after bounce 0's trace
 };
 
 template <int KIND, bool MULTI>
@@ -1933,6 +2000,7 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
         launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     }
+    if (b == 0 && P.aov) launch(ctx, "k_ext_aov", k_ext_aov<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv);
     launch(ctx, "k_ext_shade_miss", k_ext_shade_terminal<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, (int)Q_MISS);
     launch(ctx, "k_ext_shade_emit", k_ext_shade_terminal<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, (int)Q_EMIT);
     launch(ctx, "k_ext_shade_lambert", k_ext_shade_lambert<MULTI>, dim3(grid), dim3(kBlk), S, P.F0, P.B, Wv, cur, b);
@@ -2034,6 +2102,7 @@ struct ExtSink {
     float* sum = nullptr;
     float* sumsq = nullptr;
     int* nsamp = nullptr;
+    float* feat = nullptr;  // features sink: sums of XYZ and first-hit features, 10 per pixel
 };
 
 static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, const ExtSink& sink) {
@@ -2057,6 +2126,7 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
     plan.width = H.width;
     plan.stack_need = H.stack_need;
     plan.small_nodes = H.small_nodes;
+    plan.aov = sink.feat != nullptr;
     const int pcmax = std::min(P, cap);
     for (int p0 = 0; p0 < P; p0 += pcmax) {
         const int pcur = std::min(pcmax, P - p0);
@@ -2072,6 +2142,8 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
             const dim3 pg((F.pc + kBlk - 1) / kBlk);
             if (sink.acc)
                 launch(ctx, "k_ext_accum", k_ext_accum, pg, dim3(kBlk), F, Wv);
+            else if (sink.feat)
+                launch(ctx, "k_ext_features", k_ext_features, pg, dim3(kBlk), F, Wv, sink.feat, sink.nsamp);
             else
                 launch(ctx, "k_ext_moments", k_ext_moments, pg, dim3(kBlk), F, Wv, P, sink.sum, sink.sumsq, sink.nsamp);
         }
@@ -2230,6 +2302,14 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
     e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "batched dispatch launch");
     return PT_OK;
+}
+
+int ext_dispatch_features(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* feat, int* nsamp) {
+    ExtSink sink;
+    sink.pc = pc;
+    sink.feat = feat;
+    sink.nsamp = nsamp;
+    return ext_run(ctx, s, pc, sink);
 }
 
 int ext_rel_error(pt_ctx* ctx, int P, const float* sum, const float* sumsq, const int* nsamp, double* d_out) {

@@ -1,0 +1,152 @@
+"""Training pairs for the U-Net denoiser (pt_dn_*) rendered by the tracer, and
+its training on the GPU by the library's own forward/backward kernels.
+
+A pair is (features of a low-sample render, mean of an independent
+high-sample render of the same view): the features come from
+pt_render_features over samples [0, spp_in), the target from pt_dispatch over
+samples [spp_in, spp_in + spp_ref). Inputs and targets are divided by the
+RMS of the noisy XYZ mean (`image_scale`), so the network sees radiance of
+order one whatever the scene's exposure; denoised images are scaled back.
+
+    python -m pathtracer_amd.denoise_train --out weights.npy
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import time
+
+import numpy as np
+
+import pathtracer_amd as pt
+from pathtracer_amd import scenes
+
+
+def image_scale(xyz: np.ndarray) -> float:
+    """RMS over pixels and channels of an XYZ image (a positive floor for black ones)."""
+    return max(float(np.sqrt(np.mean(np.square(xyz.astype(np.float64))))), 1e-12)
+
+
+def render_features(ctx: pt.Context, scene: pt.Scene, w: int, h: int, spp: int, depth: int, first: int = 0,
+                    shot: int = 1) -> np.ndarray:
+    """(h, w, 10) feature means over samples [first, first + spp)."""
+    pc = pt.push_constants(w, h, spp, depth)
+    scene.camera(shot, pc)
+    pc.frame = first + spp
+    pc.currentSamples = spp
+    return ctx.features(scene, pc)
+
+
+def render_mean(ctx: pt.Context, scene: pt.Scene, w: int, h: int, spp: int, depth: int, first: int = 0,
+                shot: int = 1, chunk: int = 64) -> np.ndarray:
+    """(h, w, 3) XYZ mean over samples [first, first + spp), traced `chunk` samples per dispatch."""
+    acc = ctx.accum(w, h)
+    try:
+        done = 0
+        while done < spp:
+            m = min(chunk, spp - done)
+            pc = pt.push_constants(w, h, m, depth)
+            scene.camera(shot, pc)
+            pc.frame = first + done + m
+            pc.currentSamples = done + m
+            ctx.dispatch(scene, acc, pc, pt.MODE_EXT)
+            done += m
+        return acc.read()[..., :3].copy()
+    finally:
+        acc.release()
+
+
+def training_scene(i: int) -> tuple[str, int]:
+    """Scene i of the training set: the serving workload's two families with
+    seeds disjoint from its requests (10**6 + i)."""
+    rng = np.random.default_rng(10 ** 6 + i)
+    seed = 10 ** 6 + i
+    if rng.uniform() < 0.5:
+        n = int(round(math.exp(rng.uniform(0.0, math.log(4096.0)))))
+        return scenes.random_spheres(n=n, scene_seed=seed, wavelengths=4, seed=seed), 12
+    return scenes.sdf_scene(jitter=0.1, scene_seed=seed, wavelengths=4, seed=seed), 8
+
+
+def make_pair(ctx: pt.Context, text: str, depth: int, w: int, h: int, spp_in: int, spp_ref: int):
+    """-> (features (h, w, 10), target (h, w, 3)) from independent samples."""
+    sc = ctx.scene_from_json(text)
+    try:
+        feat = render_features(ctx, sc, w, h, spp_in, depth)
+        target = render_mean(ctx, sc, w, h, spp_ref, depth, first=spp_in)
+    finally:
+        sc.release()
+    return feat, target
+
+
+def normalise(feat: np.ndarray, target: np.ndarray | None = None):
+    s = image_scale(feat[..., 0:3])
+    f = feat.copy()
+    f[..., 0:3] /= s
+    return (f, None if target is None else target / s, s)
+
+
+def denoise_image(dn: pt.Denoiser, feat: np.ndarray) -> np.ndarray:
+    """Denoised XYZ (h, w, 3) of one feature image, in the input's units."""
+    f, _, s = normalise(feat)
+    return dn.denoise(f[None])[0] * s
+
+
+def rel_rmse(img: np.ndarray, ref: np.ndarray) -> float:
+    """sqrt(sum of squared error) / sqrt(sum of ref^2) over pixels and channels."""
+    img, ref = img.astype(np.float64), ref.astype(np.float64)
+    return float(np.sqrt(np.sum((img - ref) ** 2)) / max(np.sqrt(np.sum(ref ** 2)), 1e-300))
+
+
+def train(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: int = 64, lr: float = 1e-3,
+          lr_end: float = 1e-4, seed: int = 0, log_every: int = 0) -> list[float]:
+    """Adam on random crops of the normalised pairs, learning rate decayed
+    geometrically from lr to lr_end; returns the per-step losses."""
+    rng = np.random.default_rng(seed)
+    norm = [normalise(f, t) for f, t in pairs]
+    losses = []
+    for step in range(steps):
+        fb = np.empty((batch, crop, crop, 10), np.float32)
+        tb = np.empty((batch, crop, crop, 3), np.float32)
+        for b in range(batch):
+            f, t, _ = norm[int(rng.integers(len(norm)))]
+            y = int(rng.integers(0, f.shape[0] - crop + 1))
+            x = int(rng.integers(0, f.shape[1] - crop + 1))
+            fb[b] = f[y:y + crop, x:x + crop]
+            tb[b] = t[y:y + crop, x:x + crop]
+        a = lr * (lr_end / lr) ** (step / max(steps - 1, 1))
+        losses.append(dn.train_step(fb, tb, a))
+        if log_every and (step % log_every == 0 or step == steps - 1):
+            print(f"step {step:5d} lr {a:.2e} loss {np.mean(losses[-log_every:]):.4f}", flush=True)
+    return losses
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--size", type=int, default=192)
+    ap.add_argument("--spp-in", type=str, default="4,16,64")
+    ap.add_argument("--spp-ref", type=int, default=2048)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=96)
+    ap.add_argument("--out", type=str, required=True)
+    a = ap.parse_args(argv)
+    spp_in = [int(v) for v in a.spp_in.split(",")]
+    t0 = time.perf_counter()
+    with pt.Context(0) as ctx:
+        pairs = []
+        for i in range(a.scenes):
+            text, depth = training_scene(i)
+            feat, target = make_pair(ctx, text, depth, a.size, a.size, spp_in[i % len(spp_in)], a.spp_ref)
+            pairs.append((feat, target))
+            print(f"pair {i}: spp {spp_in[i % len(spp_in)]} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
+                  f"({time.perf_counter() - t0:.0f} s)", flush=True)
+        with pt.Denoiser(ctx, a.batch, a.crop, a.crop, seed=1) as dn:
+            train(dn, pairs, a.steps, a.batch, a.crop, log_every=200)
+            np.save(a.out, dn.params())
+    print(f"saved {a.out} ({time.perf_counter() - t0:.0f} s)")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,0 +1,116 @@
+"""First-hit features (pt_render_features) on the GPU: the XYZ channel is the
+renderer's own mean of the same samples; albedo, normal and distance match
+what the scene's geometry and materials fix in closed form."""
+import json
+
+import numpy as np
+import pytest
+
+import pathtracer_amd as pt
+from pathtracer_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+LAMBDA = np.array([450.0, 550.0, 650.0])
+
+
+def gaussian_reflectance(g):
+    s = sum(w * np.exp(-0.5 * ((LAMBDA - mu) / sig) ** 2) for w, mu, sig in g)
+    return np.minimum(s, 1.0)
+
+
+def conductor_f0(n, k):
+    # normal-incidence reflectance of a metal of index n + ik seen from vacuum
+    return ((n - 1.0) ** 2 + k * k) / ((n + 1.0) ** 2 + k * k)
+
+
+def cornell_with_metal():
+    s = scenes.cornell_dict(4, seed=3)
+    s["material"][3] = {"type": "conductor", "n": 0.5, "k": 3.0}  # sphere at (0.68, 0.18, 0.62)
+    return s
+
+
+def render(ctx, text, w, h, spf, depth=6):
+    sc = ctx.scene_from_json(text)
+    pc = pt.push_constants(w, h, spf, depth)
+    sc.camera(1, pc)
+    pc.frame = spf
+    pc.currentSamples = spf
+    return sc, pc, ctx.features(sc, pc)
+
+
+def test_xyz_is_the_renderers_mean(gpu_ctx):
+    text = json.dumps(cornell_with_metal())
+    sc, pc, f = render(gpu_ctx, text, 48, 40, 8)
+    r = pt.Renderer(gpu_ctx, sc, 48, 40, spf=8, path_length=6, mode=pt.MODE_EXT)
+    r.render_frame()
+    texel = r.acc.read()[..., :3]
+    np.testing.assert_allclose(f[..., 0:3], texel, rtol=2e-6, atol=1e-30)
+    assert texel[..., 1].mean() > 0
+
+
+def test_albedo_normal_and_distance_follow_the_scene(gpu_ctx):
+    d = cornell_with_metal()
+    mats = d["material"]
+    white, red, green = (gaussian_reflectance(mats[i]["gaussians"]) for i in range(3))
+    metal = np.full(3, conductor_f0(0.5, 3.0))
+    # expected first hits: (albedo, facing normal or None for curved surfaces)
+    kinds = {
+        "floor": (white, (0, 1, 0)), "ceiling": (white, (0, -1, 0)), "back": (white, (0, 0, -1)),
+        "red": (red, (-1, 0, 0)), "green": (green, (1, 0, 0)),
+        "metal": (metal, None), "glass or light": (np.ones(3), None), "miss": (np.zeros(3), None),
+    }
+    _, _, f = render(gpu_ctx, json.dumps(d), 64, 64, 4)
+    alb, nrm, dist = f[..., 3:6], f[..., 6:9], f[..., 9]
+    seen = {k: 0 for k in kinds}
+    unmatched = 0
+    curved = []
+    for y in range(64):
+        for x in range(64):
+            hits = [k for k, (a, n) in kinds.items()
+                    if np.allclose(alb[y, x], a, rtol=0, atol=2e-5)
+                    and (n is None or np.allclose(nrm[y, x], n, atol=1e-4))]
+            if not hits:
+                unmatched += 1  # silhouette pixels average two surfaces
+                continue
+            k = hits[0]
+            seen[k] += 1
+            if k == "miss":
+                assert np.all(nrm[y, x] == 0) and dist[y, x] == 0
+            else:
+                # a pixel's samples see one plane (a unit normal), or a patch of
+                # a sphere whose normals spread (a mean slightly shorter)
+                ln = float(np.linalg.norm(nrm[y, x]))
+                if kinds[k][1] is not None:
+                    assert abs(ln - 1.0) < 1e-4
+                else:
+                    assert ln <= 1.0 + 1e-5
+                    curved.append(ln)
+                assert 1.5 < dist[y, x] < 3.5  # camera at z = -1.6, box [0, 1]^3
+            if k == "back":
+                # from the lens (within 0.1 of z = -1.6) to z = 1, longer off-axis
+                assert 2.4 < dist[y, x] < 3.0
+    print(f"\n{seen}, unmatched {unmatched}")
+    assert unmatched < 0.15 * 64 * 64
+    assert np.median(curved) > 0.99
+    for k in ("back", "floor", "red", "green", "metal", "glass or light"):
+        assert seen[k] > 20, k
+
+
+def test_sdf_scene_features_are_finite_and_unit(gpu_ctx):
+    _, _, f = render(gpu_ctx, scenes.sdf_scene(seed=5), 64, 48, 4)
+    assert np.isfinite(f).all()
+    hit = f[..., 9] > 0
+    assert 0.2 < hit.mean() < 1.0
+    n = np.linalg.norm(f[..., 6:9], axis=-1)
+    assert np.all(n <= 1.0 + 1e-5)
+    assert np.median(n[hit]) > 0.99
+    # misses carry zeros in every feature
+    assert np.all(f[~hit][:, 3:9] == 0)
+
+
+def test_bad_arguments_are_rejected(gpu_ctx):
+    sc = gpu_ctx.scene_from_json(scenes.cornell(1))
+    pc = pt.push_constants(16, 16, 0, 4)
+    with pytest.raises(pt.PTError):
+        gpu_ctx.features(sc, pc)
