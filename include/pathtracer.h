@@ -226,13 +226,15 @@ pt_status pt_sync(pt_ctx* ctx);
 /* Denoiser features (extended mode; no counterpart in the reference, which
  * shows the noisy running mean, src/shader.frag:81-99): traces the samples
  * [currentSamples - samplesPerFrame, currentSamples) of every pixel, as
- * pt_dispatch would, and writes per pixel the mean over those samples of 10
- * floats: XYZ (the accumulator's texel units: exposure apertureSize^2 * ISO
- * applied), the first hit's albedo
- * (reflectance at 450, 550 and 650 nm; 1 for glass and emitters; the
- * normal-incidence reflectance of a conductor), its shading normal facing the
- * camera and its distance; zeros for rays that miss. Samples with a NaN are
- * dropped. Synchronous; `n_floats` must be width * height * 10. */
+ * pt_dispatch would, and writes 12 floats per pixel (PT_FEATURES): the means
+ * over those samples of XYZ (the accumulator's texel units: exposure
+ * apertureSize^2 * ISO applied), of the first hit's albedo (reflectance at
+ * 450, 550 and 650 nm; 1 for glass and emitters; the normal-incidence
+ * reflectance of a conductor), of its shading normal facing the camera and of
+ * its distance (zeros for rays that miss); then the standard error of the
+ * mean of Y (same units as XYZ) and the number of samples. Samples with a NaN
+ * are dropped. Synchronous; `n_floats` must be width * height * 12. */
+#define PT_FEATURES 12
 pt_status pt_render_features(pt_ctx* ctx, const pt_scene* scene, const pt_push_constants* pc, float* host_feat,
                              size_t n_floats);
 
@@ -340,10 +342,13 @@ pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* a
  * convolutions) that maps a low-sample image plus first-hit features to a
  * denoised XYZ image, trained and run on the GPU's matrix cores in BF16 with
  * FP32 accumulation and FP32 master weights (Adam).
- * Features: N images H x W (H, W multiples of 4), 10 floats per pixel:
- *   XYZ (3), albedo (3), normal (3), depth (1).
- * The network predicts log1p(denoised XYZ) - log1p(input XYZ); the training
- * loss is the mean absolute difference of log1p(denoised) and log1p(target). */
+ * Features: N images H x W (H, W multiples of 4), PT_FEATURES floats per
+ * pixel as pt_render_features writes them: XYZ (3), albedo (3), normal (3),
+ * distance (1), standard error of the mean of Y (1), samples (1). The network
+ * sees log1p(XYZ), albedo, normal, 1 / (1 + distance), log1p(standard error)
+ * and 1 / sqrt(samples), and predicts log1p(denoised XYZ) - log1p(input XYZ);
+ * the training loss is the mean absolute difference of log1p(denoised) and
+ * log1p(target). Radiance is best scaled to order one before the call. */
 typedef struct pt_denoiser pt_denoiser;
 pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width, uint64_t seed, pt_denoiser** out);
 void pt_dn_destroy(pt_denoiser* dn);
@@ -352,7 +357,7 @@ int64_t pt_dn_param_count(const pt_denoiser* dn);
 /* FP32 parameters: per layer, weights [9][Cout][Cin] then biases [Cout]. */
 pt_status pt_dn_get_params(pt_denoiser* dn, float* host, size_t n);
 pt_status pt_dn_set_params(pt_denoiser* dn, const float* host, size_t n);
-/* features (N, H, W, 10) -> denoised XYZ (N, H, W, 3) */
+/* features (N, H, W, PT_FEATURES) -> denoised XYZ (N, H, W, 3) */
 pt_status pt_dn_denoise(pt_denoiser* dn, const float* features, int N, int H, int W, float* denoised);
 /* Loss against target XYZ (N, H, W, 3) and the gradient of every parameter
  * (host_grads may be NULL). */

@@ -40,13 +40,17 @@ def unpack(params: np.ndarray | torch.Tensor):
 
 
 def to_input(feat: torch.Tensor) -> torch.Tensor:
-    """features (N, H, W, 10) -> network input (N, 16, H, W)."""
+    """features (N, H, W, 12) -> network input (N, 16, H, W): log1p(XYZ),
+    albedo, normal, 1 / (1 + distance), log1p(standard error of Y),
+    1 / sqrt(samples), zeros."""
     n, h, w, _ = feat.shape
     x = torch.zeros(n, 16, h, w, dtype=torch.float64)
     f = feat.permute(0, 3, 1, 2)
     x[:, 0:3] = torch.log1p(torch.clamp(f[:, 0:3], min=0.0))
     x[:, 3:9] = f[:, 3:9]
     x[:, 9] = 1.0 / (1.0 + torch.clamp(f[:, 9], min=0.0))
+    x[:, 10] = torch.log1p(torch.clamp(f[:, 10], min=0.0))
+    x[:, 11] = 1.0 / torch.sqrt(torch.clamp(f[:, 11], min=1.0))
     return x
 
 

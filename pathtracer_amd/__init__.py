@@ -31,6 +31,7 @@ PT_ERR_STATE = -8
 
 MODE_REF = 0
 MODE_EXT = 1
+FEATURES = 12  # floats per pixel of a feature image (PT_FEATURES)
 
 SDF_MANDELBULB, SDF_MENGER, SDF_BLOB, SDF_TERRAIN = 1, 2, 3, 4
 SDFMAT_CONSTANT, SDFMAT_RADIAL, SDFMAT_BLOB = 0, 1, 2
@@ -373,10 +374,11 @@ class Context:
 
     def features(self, scene: "Scene", pc: PushConstants) -> np.ndarray:
         """Per-pixel means over the samples [currentSamples - samplesPerFrame,
-        currentSamples) of XYZ, first-hit albedo, normal and distance:
-        (height, width, 10) float32."""
+        currentSamples) of XYZ, first-hit albedo, normal and distance, then the
+        standard error of the mean of Y and the sample count:
+        (height, width, FEATURES) float32."""
         w, h = pc.resolution[0], pc.resolution[1]
-        out = np.empty((h, w, 10), np.float32)
+        out = np.empty((h, w, FEATURES), np.float32)
         self.check(lib.pt_render_features(self.h, scene.h, C.byref(pc),
                                           out.ctypes.data_as(C.POINTER(C.c_float)), out.size))
         return out

@@ -388,10 +388,10 @@ pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_const
                        "pt_render_features: need a positive resolution, samplesPerFrame > 0, 0 <= pathLength <= 1024, "
                        "currentSamples >= samplesPerFrame");
     const size_t P = (size_t)w * h;
-    if (n_floats != P * 10) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: size mismatch");
+    if (n_floats != P * 12) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: size mismatch");
     (void)hipSetDevice(ctx->device);
     DevAlloc m;
-    const size_t feat_bytes = (P * 40 + 255) / 256 * 256;
+    const size_t feat_bytes = (P * 48 + 255) / 256 * 256;
     int st = ptk::dev_alloc(ctx, m, feat_bytes + P * 4);
     if (st != PT_OK) return st;
     float* d_feat = (float*)m.p;
@@ -403,7 +403,7 @@ pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_const
         if (st == PT_OK) st = ptk::ext_dispatch_features(ctx, s, pc, d_feat, d_n);
         if (st == PT_OK) st = ptk::count_dispatch(ctx);
         if (st == PT_OK) {
-            e = hipMemcpyAsync(host, d_feat, P * 40, hipMemcpyDeviceToHost, ctx->stream);
+            e = hipMemcpyAsync(host, d_feat, P * 48, hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(n.data(), d_n, P * 4, hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
             if (e != hipSuccess) st = ptk::hip_fail(ctx, e, "pt_render_features");
@@ -418,10 +418,17 @@ pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_const
     // as the texel resolve applies it
     const float ex = pc->apertureSize * pc->apertureSize * (float)pc->ISO;
     for (size_t i = 0; i < P; ++i) {
+        float* f = host + i * 12;
+        const double sy = f[1], syy = f[10];
         const float inv = n[i] > 0 ? 1.0f / (float)n[i] : 0.0f;
         const float e = n[i] > 0 ? ex / (float)n[i] : 0.0f;
-        for (int c = 0; c < 3; ++c) host[i * 10 + c] *= e;
-        for (int c = 3; c < 10; ++c) host[i * 10 + c] *= inv;
+        for (int c = 0; c < 3; ++c) f[c] *= e;
+        for (int c = 3; c < 10; ++c) f[c] *= inv;
+        // standard error of the mean of Y: sqrt(sample variance / n)
+        double var = 0.0;
+        if (n[i] > 1) var = std::max(syy - sy * sy / n[i], 0.0) / ((double)n[i] - 1.0) / (double)n[i];
+        f[10] = (float)(std::sqrt(var) * (double)ex);
+        f[11] = (float)n[i];
     }
     return PT_OK;
 }

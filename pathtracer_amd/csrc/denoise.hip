@@ -44,7 +44,8 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef unsigned short u16;
 
-constexpr int kIn = 16;      // input channels (10 used)
+constexpr int kIn = 16;      // input channels (12 used)
+constexpr int kFeat = 12;    // floats per pixel of a feature image
 constexpr int kOutC = 16;    // output channels (3 used)
 constexpr int kThreads = 256;
 
@@ -278,12 +279,14 @@ __global__ void k_relu_bwd(const u16* dy, const u16* add, const u16* y, u16* dz,
 // features (N, H, W, 10) FP32 -> input tensor (16 channels BF16)
 __global__ void k_prep(const float* f, u16* x, size_t P) {
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
-        const float* s = f + p * 10;
+        const float* s = f + p * kFeat;
         u16* d = x + p * kIn;
         for (int c = 0; c < 3; ++c) d[c] = f2bf(log1pf(fmaxf(s[c], 0.0f)));
         for (int c = 3; c < 9; ++c) d[c] = f2bf(s[c]);
         d[9] = f2bf(1.0f / (1.0f + fmaxf(s[9], 0.0f)));
-        for (int c = 10; c < kIn; ++c) d[c] = 0;
+        d[10] = f2bf(log1pf(fmaxf(s[10], 0.0f)));
+        d[11] = f2bf(rsqrtf(fmaxf(s[11], 1.0f)));
+        for (int c = 12; c < kIn; ++c) d[c] = 0;
     }
 }
 // prediction log1p(denoised) = log1p(in) + out[0..2]; loss and its gradient
@@ -298,7 +301,7 @@ __global__ void k_head(const float* f, const u16* out, const float* target, floa
                 if (dout) dout[p * kOutC + c] = 0;
                 continue;
             }
-            const float z = log1pf(fmaxf(f[p * 10 + c], 0.0f)) + bf2f(out[p * kOutC + c]);
+            const float z = log1pf(fmaxf(f[p * kFeat + c], 0.0f)) + bf2f(out[p * kOutC + c]);
             if (denoised) denoised[p * 3 + c] = fmaxf(expm1f(z), 0.0f);
             if (target) {
                 const float d = z - log1pf(fmaxf(target[p * 3 + c], 0.0f));
@@ -538,7 +541,7 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     const size_t act[] = {P * kIn, P * 32, P * 32, P / 4 * 32, P / 4 * 64, P / 4 * 64, P / 16 * 64, P / 16 * 128,
                           P / 16 * 128, P / 4 * 128, P / 4 * 64, P / 4 * 64, P * 64, P * 32, P * 32, P * kOutC,
                           P * 64, P * 64, P * 64, P / 4 * 64, P * 32};
-    size_t bytes = 4 * off * 4 + 2 * off * 2 + (P * 10 + P * 3 + P * 3 + 64) * 4 + 64 * 256;
+    size_t bytes = 4 * off * 4 + 2 * off * 2 + (P * kFeat + P * 3 + P * 3 + 64) * 4 + 64 * 256;
     for (size_t a : act) bytes += (a * 2 + 255) / 256 * 256;
     int st = ptk::dev_alloc(ctx, d->mem, bytes);
     if (st != PT_OK) {
@@ -559,7 +562,7 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     u16** acts[] = {&d->x0, &d->e0a, &d->e0, &d->p1, &d->e1a, &d->e1, &d->p2, &d->ba, &d->bb, &d->u1, &d->d1a, &d->d1,
                     &d->u0, &d->d0a, &d->d0, &d->out, &d->ga, &d->gb, &d->gc, &d->gs1, &d->gs0};
     for (size_t i = 0; i < sizeof(act) / sizeof(act[0]); ++i) *acts[i] = reinterpret_cast<u16*>(carve(act[i] * 2));
-    d->feat = reinterpret_cast<float*>(carve(P * 10 * 4));
+    d->feat = reinterpret_cast<float*>(carve(P * kFeat * 4));
     d->tgt = reinterpret_cast<float*>(carve(P * 3 * 4));
     d->den = reinterpret_cast<float*>(carve(P * 3 * 4));
     d->loss = reinterpret_cast<float*>(carve(64 * 4));
@@ -570,7 +573,7 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
         l.wb = wb + n;
         wb += 2 * n;
     }
-    // He-normal weights (inputs of the first layer beyond the 10 features are zero), zero biases
+    // He-normal weights, zero biases
     std::vector<float> h(off, 0.0f);
     std::mt19937_64 rng(seed);
     for (const Layer& l : d->L) {
@@ -629,7 +632,7 @@ pt_status pt_dn_denoise(pt_denoiser* d, const float* features, int N, int H, int
     if (st != PT_OK) return st;
     const size_t P = (size_t)N * H * W;
     (void)hipSetDevice(d->ctx->device);
-    st = upload(d, d->feat, features, P * 10);
+    st = upload(d, d->feat, features, P * kFeat);
     if (st != PT_OK) return st;
     forward(d, N, H, W);
     ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
@@ -648,7 +651,7 @@ pt_status pt_dn_gradients(pt_denoiser* d, const float* features, const float* ta
     const size_t P = (size_t)N * H * W;
     pt_ctx* ctx = d->ctx;
     (void)hipSetDevice(ctx->device);
-    st = upload(d, d->feat, features, P * 10);
+    st = upload(d, d->feat, features, P * kFeat);
     if (st == PT_OK) st = upload(d, d->tgt, target, P * 3);
     if (st != PT_OK) return st;
     hipError_t e = hipMemsetAsync(d->g, 0, d->nparam * 4, ctx->stream);

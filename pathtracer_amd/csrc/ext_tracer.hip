@@ -1082,16 +1082,16 @@ __global__ void __launch_bounds__(kBlk) k_ext_aov(ExtSceneDev S0, ExtBatch B, Ex
     }
 }
 
-// Features sink: per pixel, in sample order, sums of the per-sample XYZ and
-// first-hit features into feat[pixel * 10 + (XYZ 0..2, albedo 3..5,
-// normal 6..8, distance 9)]; samples with a NaN are dropped and counted.
+// Features sink: per pixel, in sample order, sums of the per-sample XYZ, the
+// first-hit features and Y^2 into feat[pixel * 12 + (XYZ 0..2, albedo 3..5,
+// normal 6..8, distance 9, Y^2 10)]; samples with a NaN are dropped and counted.
 __global__ void __launch_bounds__(kBlk) k_ext_features(ExtFrame F, ExtWave Wv, float* feat, int* nsamp) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned nans = 0;
     if (j < F.pc) {
-        float* f = feat + (size_t)(F.p0 + j) * 10;
-        float s[10];
-        for (int c = 0; c < 10; ++c) s[c] = f[c];
+        float* f = feat + (size_t)(F.p0 + j) * 12;
+        float s[11];
+        for (int c = 0; c < 11; ++c) s[c] = f[c];
         int n = nsamp[F.p0 + j];
         for (int k = 0; k < F.m; ++k) {
             const int p = k * F.pc + j;
@@ -1102,9 +1102,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_features(ExtFrame F, ExtWave Wv, f
             s[3] += b0.x; s[4] += b0.y; s[5] += b0.z;
             s[6] += b1.x; s[7] += b1.y; s[8] += b1.z;
             s[9] += b0.w;
+            s[10] += a.y * a.y;
             ++n;
         }
-        for (int c = 0; c < 10; ++c) f[c] = s[c];
+        for (int c = 0; c < 11; ++c) f[c] = s[c];
         nsamp[F.p0 + j] = n;
     }
     wave_add_u64(stat_slot(Wv.stats, 4), nans);
@@ -2102,7 +2103,7 @@ struct ExtSink {
     float* sum = nullptr;
     float* sumsq = nullptr;
     int* nsamp = nullptr;
-    float* feat = nullptr;  // features sink: sums of XYZ and first-hit features, 10 per pixel
+    float* feat = nullptr;  // features sink: sums of XYZ, first-hit features and Y^2, 12 per pixel
 };
 
 static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, const ExtSink& sink) {

@@ -174,8 +174,8 @@ int ext_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
 int ext_dispatch_moments(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* sum, float* sumsq,
                          int* nsamp);
 // The samples [frame - spf, frame) of every pixel: per-pixel sums of the
-// per-sample XYZ and first-hit features (albedo, normal, distance) into feat
-// (10 floats per pixel), sample counts into nsamp.
+// per-sample XYZ, first-hit features (albedo, normal, distance) and Y^2 into
+// feat (12 floats per pixel, the last unused), sample counts into nsamp.
 int ext_dispatch_features(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* feat, int* nsamp);
 // One request of a batched wave: its scene, the push constants of its quantum
 // (camera, resolution, pathLength; samples [frame - samplesPerFrame, frame)),

@@ -29,7 +29,7 @@ def image_scale(xyz: np.ndarray) -> float:
 
 def render_features(ctx: pt.Context, scene: pt.Scene, w: int, h: int, spp: int, depth: int, first: int = 0,
                     shot: int = 1) -> np.ndarray:
-    """(h, w, 10) feature means over samples [first, first + spp)."""
+    """(h, w, FEATURES) features of samples [first, first + spp)."""
     pc = pt.push_constants(w, h, spp, depth)
     scene.camera(shot, pc)
     pc.frame = first + spp
@@ -68,7 +68,7 @@ def training_scene(i: int) -> tuple[str, int]:
 
 
 def make_pair(ctx: pt.Context, text: str, depth: int, w: int, h: int, spp_in: int, spp_ref: int):
-    """-> (features (h, w, 10), target (h, w, 3)) from independent samples."""
+    """-> (features (h, w, FEATURES), target (h, w, 3)) from independent samples."""
     sc = ctx.scene_from_json(text)
     try:
         feat = render_features(ctx, sc, w, h, spp_in, depth)
@@ -79,9 +79,11 @@ def make_pair(ctx: pt.Context, text: str, depth: int, w: int, h: int, spp_in: in
 
 
 def normalise(feat: np.ndarray, target: np.ndarray | None = None):
+    """Radiance channels (XYZ, standard error of Y) and target divided by image_scale."""
     s = image_scale(feat[..., 0:3])
     f = feat.copy()
     f[..., 0:3] /= s
+    f[..., 10] /= s
     return (f, None if target is None else target / s, s)
 
 
@@ -105,7 +107,7 @@ def train(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: int = 64, lr
     norm = [normalise(f, t) for f, t in pairs]
     losses = []
     for step in range(steps):
-        fb = np.empty((batch, crop, crop, 10), np.float32)
+        fb = np.empty((batch, crop, crop, pt.FEATURES), np.float32)
         tb = np.empty((batch, crop, crop, 3), np.float32)
         for b in range(batch):
             f, t, _ = norm[int(rng.integers(len(norm)))]

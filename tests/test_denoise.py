@@ -9,12 +9,14 @@ from oracle import denoise_ref as D
 
 
 def random_features(rng, n, h, w):
-    f = np.empty((n, h, w, 10), np.float32)
+    f = np.empty((n, h, w, 12), np.float32)
     f[..., 0:3] = np.exp(rng.normal(-1.0, 1.0, (n, h, w, 3)))          # XYZ, heavy-ish tails
     f[..., 3:6] = rng.uniform(0.0, 1.0, (n, h, w, 3))                   # albedo
     nv = rng.normal(size=(n, h, w, 3))
     f[..., 6:9] = nv / np.linalg.norm(nv, axis=-1, keepdims=True)       # normal
-    f[..., 9] = rng.uniform(0.5, 5.0, (n, h, w))                        # depth
+    f[..., 9] = rng.uniform(0.5, 5.0, (n, h, w))                        # distance
+    f[..., 10] = f[..., 1] * rng.uniform(0.05, 1.0, (n, h, w))           # standard error of Y
+    f[..., 11] = rng.integers(1, 256, (n, h, w))                        # samples
     return f
 
 

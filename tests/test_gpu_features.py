@@ -30,11 +30,11 @@ def cornell_with_metal():
     return s
 
 
-def render(ctx, text, w, h, spf, depth=6):
-    sc = ctx.scene_from_json(text)
+def render(ctx, text, w, h, spf, depth=6, first=0, sc=None):
+    sc = sc or ctx.scene_from_json(text)
     pc = pt.push_constants(w, h, spf, depth)
     sc.camera(1, pc)
-    pc.frame = spf
+    pc.frame = first + spf
     pc.currentSamples = spf
     return sc, pc, ctx.features(sc, pc)
 
@@ -107,6 +107,21 @@ def test_sdf_scene_features_are_finite_and_unit(gpu_ctx):
     assert np.median(n[hit]) > 0.99
     # misses carry zeros in every feature
     assert np.all(f[~hit][:, 3:9] == 0)
+
+
+def test_standard_error_matches_the_spread_of_independent_renders(gpu_ctx):
+    """Channel 10 is the standard error of the mean of Y: its square, averaged
+    over 128 renders of disjoint samples, matches the variance of their Y means
+    (summed over pixels); channel 11 counts the samples."""
+    # diffuse surfaces under a uniform sky: bounded radiance, so both sums
+    # settle quickly (a small light or caustics would give heavy tails)
+    sc = gpu_ctx.scene_from_json(scenes.sdf_scene(seed=5))
+    fs = np.stack([render(gpu_ctx, None, 32, 24, 4, depth=4, first=4 * r, sc=sc)[2] for r in range(128)])
+    assert np.all(fs[..., 11] == 4)
+    emp = fs[..., 1].var(axis=0, ddof=1).sum()
+    pred = (fs[..., 10].astype(np.float64) ** 2).mean(axis=0).sum()
+    print(f"\nsum of variances: predicted {pred:.4e} empirical {emp:.4e}")
+    assert 0.85 < pred / emp < 1.15
 
 
 def test_bad_arguments_are_rejected(gpu_ctx):
