@@ -346,14 +346,19 @@ pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* a
  * pixel as pt_render_features writes them: XYZ (3), albedo (3), normal (3),
  * distance (1), standard error of the mean of Y (1), samples (1). The network
  * sees log1p(XYZ), albedo, normal, 1 / (1 + distance), log1p(standard error)
- * and 1 / sqrt(samples), and predicts log1p(denoised XYZ) - log1p(input XYZ);
- * the training loss is the mean absolute difference of log1p(denoised) and
- * log1p(target). Radiance is best scaled to order one before the call. */
+ * and 1 / sqrt(samples), and predicts log1p(denoised XYZ) - log1p(input XYZ).
+ * Radiance is best scaled to order one before the call. */
 typedef struct pt_denoiser pt_denoiser;
 pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width, uint64_t seed, pt_denoiser** out);
 void pt_dn_destroy(pt_denoiser* dn);
 const char* pt_dn_last_error(const pt_denoiser* dn);
 int64_t pt_dn_param_count(const pt_denoiser* dn);
+/* Training loss: the mean over pixels and XYZ of |log1p(denoised) -
+ * log1p(target)| (the default) or of (denoised - target)^2, the squared error
+ * whose sum the relative RMSE measures. */
+#define PT_DN_LOSS_L1_LOG 0
+#define PT_DN_LOSS_L2 1
+pt_status pt_dn_set_loss(pt_denoiser* dn, int kind);
 /* FP32 parameters: per layer, weights [9][Cout][Cin] then biases [Cout]. */
 pt_status pt_dn_get_params(pt_denoiser* dn, float* host, size_t n);
 pt_status pt_dn_set_params(pt_denoiser* dn, const float* host, size_t n);

@@ -79,12 +79,16 @@ def denoise(params, feat) -> np.ndarray:
     return torch.clamp(torch.expm1(z), min=0.0).permute(0, 2, 3, 1).numpy()
 
 
-def loss_and_grad(params: np.ndarray, feat: np.ndarray, target: np.ndarray):
-    """Mean |z - log1p(target)| and its gradient w.r.t. the parameter vector."""
+def loss_and_grad(params: np.ndarray, feat: np.ndarray, target: np.ndarray, kind: str = "l1log"):
+    """The loss and its gradient w.r.t. the parameter vector: mean |z - log1p(target)|
+    ("l1log") or mean (expm1(z) - target)^2 ("l2")."""
     p = torch.tensor(np.asarray(params, np.float64), requires_grad=True)
     z = forward(p, feat)
-    t = torch.log1p(torch.clamp(torch.tensor(np.asarray(target, np.float64)), min=0.0)).permute(0, 3, 1, 2)
-    loss = (z - t).abs().mean()
+    t = torch.clamp(torch.tensor(np.asarray(target, np.float64)), min=0.0).permute(0, 3, 1, 2)
+    if kind == "l2":
+        loss = ((torch.expm1(z) - t) ** 2).mean()
+    else:
+        loss = (z - torch.log1p(t)).abs().mean()
     loss.backward()
     return loss.item(), p.grad.numpy()
 

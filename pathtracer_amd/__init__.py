@@ -32,6 +32,7 @@ PT_ERR_STATE = -8
 MODE_REF = 0
 MODE_EXT = 1
 FEATURES = 12  # floats per pixel of a feature image (PT_FEATURES)
+DN_LOSS_L1_LOG, DN_LOSS_L2 = 0, 1
 
 SDF_MANDELBULB, SDF_MENGER, SDF_BLOB, SDF_TERRAIN = 1, 2, 3, 4
 SDFMAT_CONSTANT, SDFMAT_RADIAL, SDFMAT_BLOB = 0, 1, 2
@@ -232,6 +233,7 @@ SIGNATURES = {
     "pt_dn_denoise": (C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "pt_dn_gradients": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "pt_dn_set_loss": (C.c_int, [_P, C.c_int]),
     "pt_dn_train_step": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
                                    C.c_float, C.POINTER(C.c_float)]),
     "pt_math_eval_host": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
@@ -666,6 +668,9 @@ class Denoiser:
         loss = C.c_float()
         self._ck(lib.pt_dn_gradients(self.h, _fptr(feat), _fptr(target), n, h, w, C.byref(loss), _fptr(g), self.n))
         return loss.value, g
+
+    def set_loss(self, kind: int) -> None:
+        self._ck(lib.pt_dn_set_loss(self.h, kind))
 
     def train_step(self, feat: np.ndarray, target: np.ndarray, lr: float) -> float:
         feat = np.ascontiguousarray(feat, np.float32)

@@ -289,10 +289,12 @@ __global__ void k_prep(const float* f, u16* x, size_t P) {
         for (int c = 12; c < kIn; ++c) d[c] = 0;
     }
 }
-// prediction log1p(denoised) = log1p(in) + out[0..2]; loss and its gradient
-// (L1 against log1p(target), mean over 3P values); also the denoised image.
+// prediction z = log1p(denoised) = log1p(in) + out[0..2]; loss and its
+// gradient with respect to z, mean over 3P values: |z - log1p(target)|
+// (l2 == 0) or (expm1(z) - target)^2 (l2 == 1); also the denoised image
+// max(expm1(z), 0).
 __global__ void k_head(const float* f, const u16* out, const float* target, float* denoised, u16* dout,
-                       float* loss, size_t P) {
+                       float* loss, size_t P, int l2) {
     float l = 0.0f;
     const float inv = 1.0f / (3.0f * (float)P);
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < P; p += (size_t)gridDim.x * blockDim.x) {
@@ -303,7 +305,11 @@ __global__ void k_head(const float* f, const u16* out, const float* target, floa
             }
             const float z = log1pf(fmaxf(f[p * kFeat + c], 0.0f)) + bf2f(out[p * kOutC + c]);
             if (denoised) denoised[p * 3 + c] = fmaxf(expm1f(z), 0.0f);
-            if (target) {
+            if (target && l2) {
+                const float y = expm1f(z), d = y - fmaxf(target[p * 3 + c], 0.0f);
+                l += d * d;
+                if (dout) dout[p * kOutC + c] = f2bf(2.0f * d * (y + 1.0f) * inv);
+            } else if (target) {
                 const float d = z - log1pf(fmaxf(target[p * 3 + c], 0.0f));
                 l += fabsf(d);
                 if (dout) dout[p * kOutC + c] = f2bf(d > 0.0f ? inv : (d < 0.0f ? -inv : 0.0f));
@@ -358,6 +364,7 @@ struct pt_denoiser {
     std::vector<Layer> L;
     size_t nparam = 0;
     int64_t step = 0;
+    int loss_kind = PT_DN_LOSS_L1_LOG;
     DevAlloc mem;
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     u16* wbf = nullptr;  // BF16 copies, 2 per weight
@@ -610,6 +617,12 @@ const char* pt_dn_last_error(const pt_denoiser* d) { return d ? d->err.c_str() :
 
 int64_t pt_dn_param_count(const pt_denoiser* d) { return d ? (int64_t)d->nparam : 0; }
 
+pt_status pt_dn_set_loss(pt_denoiser* d, int kind) {
+    if (!d || (kind != PT_DN_LOSS_L1_LOG && kind != PT_DN_LOSS_L2)) return PT_ERR_INVALID_ARG;
+    d->loss_kind = kind;
+    return PT_OK;
+}
+
 pt_status pt_dn_get_params(pt_denoiser* d, float* host, size_t n) {
     if (!d || !host || n != d->nparam) return PT_ERR_INVALID_ARG;
     hipError_t e = hipMemcpyAsync(host, d->w, n * 4, hipMemcpyDeviceToHost, d->ctx->stream);
@@ -636,7 +649,7 @@ pt_status pt_dn_denoise(pt_denoiser* d, const float* features, int N, int H, int
     if (st != PT_OK) return st;
     forward(d, N, H, W);
     ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
-                (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, P);
+                (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, P, 0);
     hipError_t e = hipMemcpyAsync(denoised, d->den, P * 3 * 4, hipMemcpyDeviceToHost, d->ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(d->ctx->stream);
     return e == hipSuccess ? PT_OK : dn_fail(d, ptk::hip_fail(d->ctx, e, "pt_dn_denoise"), d->ctx->err);
@@ -659,7 +672,7 @@ pt_status pt_dn_gradients(pt_denoiser* d, const float* features, const float* ta
     if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "pt_dn_gradients"), ctx->err);
     forward(d, N, H, W);
     ptk::launch(ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
-                (const float*)d->tgt, (float*)nullptr, d->gc, d->loss, P);
+                (const float*)d->tgt, (float*)nullptr, d->gc, d->loss, P, d->loss_kind);
     backward(d, N, H, W);
     if (loss) e = hipMemcpyAsync(loss, d->loss, 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && host_grads) e = hipMemcpyAsync(host_grads, d->g, n_grads * 4, hipMemcpyDeviceToHost, ctx->stream);

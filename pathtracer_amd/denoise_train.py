@@ -8,18 +8,31 @@ samples [spp_in, spp_in + spp_ref). Inputs and targets are divided by the
 RMS of the noisy XYZ mean (`image_scale`), so the network sees radiance of
 order one whatever the scene's exposure; denoised images are scaled back.
 
-    python -m pathtracer_amd.denoise_train --out weights.npy
+Training runs in two phases: the mean absolute log error first (robust to
+the fireflies of low-sample inputs), then the squared error of the scaled
+radiance, the quantity whose sum the relative RMSE measures.
+
+    python -m pathtracer_amd.denoise_train --out pathtracer_amd/data/denoiser_weights.npy
 """
 from __future__ import annotations
 
 import argparse
 import math
+import os
 import time
 
 import numpy as np
 
 import pathtracer_amd as pt
 from pathtracer_amd import scenes
+
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "denoiser_weights.npy")
+
+
+def load_weights(dn: pt.Denoiser, path: str = DEFAULT_WEIGHTS) -> None:
+    """Parameters saved by main() (float32 .npy, no pickles)."""
+    dn.set_params(np.load(path, allow_pickle=False))
 
 
 def image_scale(xyz: np.ndarray) -> float:
@@ -122,13 +135,24 @@ def train(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: int = 64, lr
     return losses
 
 
+def train_two_phase(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: int = 96, seed: int = 0,
+                    log_every: int = 0) -> list[float]:
+    """steps / 2 of mean absolute log error (lr 1e-3 -> 2e-4), then steps / 2
+    of squared error (lr 2e-4 -> 2e-5)."""
+    dn.set_loss(pt.DN_LOSS_L1_LOG)
+    losses = train(dn, pairs, steps // 2, batch, crop, 1e-3, 2e-4, seed, log_every)
+    dn.set_loss(pt.DN_LOSS_L2)
+    losses += train(dn, pairs, steps - steps // 2, batch, crop, 2e-4, 2e-5, seed + 1, log_every)
+    return losses
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
-    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--size", type=int, default=192)
-    ap.add_argument("--spp-in", type=str, default="4,16,64")
-    ap.add_argument("--spp-ref", type=int, default=2048)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--spp-in", type=str, default="4,16,64,256,1024")
+    ap.add_argument("--spp-ref", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=96)
     ap.add_argument("--out", type=str, required=True)
@@ -144,7 +168,7 @@ def main(argv=None) -> int:
             print(f"pair {i}: spp {spp_in[i % len(spp_in)]} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
                   f"({time.perf_counter() - t0:.0f} s)", flush=True)
         with pt.Denoiser(ctx, a.batch, a.crop, a.crop, seed=1) as dn:
-            train(dn, pairs, a.steps, a.batch, a.crop, log_every=200)
+            train_two_phase(dn, pairs, a.steps, a.batch, a.crop, log_every=500)
             np.save(a.out, dn.params())
     print(f"saved {a.out} ({time.perf_counter() - t0:.0f} s)")
     return 0

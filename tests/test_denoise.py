@@ -30,14 +30,15 @@ def random_params(rng):
     return out
 
 
-def test_oracle_gradient_matches_finite_differences():
+@pytest.mark.parametrize("kind", ["l1log", "l2"])
+def test_oracle_gradient_matches_finite_differences(kind):
     """Central differences of the FP64 loss on a tiny input pin the oracle's
     gradient (and so its layer wiring) independently of autograd."""
     rng = np.random.default_rng(0)
     p = random_params(rng)
     feat = random_features(rng, 1, 8, 8)
     target = np.exp(rng.normal(-1.0, 1.0, (1, 8, 8, 3)))
-    loss, g = D.loss_and_grad(p, feat, target)
+    loss, g = D.loss_and_grad(p, feat, target, kind)
     h = 1e-6
     checked = 0
     for name, sl in D.layer_slices():
@@ -48,7 +49,7 @@ def test_oracle_gradient_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (D.loss_and_grad(pp, feat, target)[0] - D.loss_and_grad(pm, feat, target)[0]) / (2 * h)
+        fd = (D.loss_and_grad(pp, feat, target, kind)[0] - D.loss_and_grad(pm, feat, target, kind)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-4 * abs(g[i]) + 1e-9, (name, fd, g[i])
         checked += 1
     assert checked >= 18
@@ -98,16 +99,18 @@ def grad_agreement(g_gpu, g_ref):
 
 
 @pytest.mark.gpu
-def test_gradients_match_the_oracle(gpu_ctx):
+@pytest.mark.parametrize("kind", ["l1log", "l2"])
+def test_gradients_match_the_oracle(gpu_ctx, kind):
     import pathtracer_amd as pt
     rng = np.random.default_rng(2)
     with pt.Denoiser(gpu_ctx, 2, 32, 32, seed=4) as dn:
+        dn.set_loss(pt.DN_LOSS_L2 if kind == "l2" else pt.DN_LOSS_L1_LOG)
         p = random_params(rng).astype(np.float32)
         dn.set_params(p)
         feat = random_features(rng, 2, 32, 32)
         target = np.exp(rng.normal(-1.0, 1.0, (2, 32, 32, 3))).astype(np.float32)
         loss, g = dn.gradients(feat, target)
-    ref_loss, ref_g = D.loss_and_grad(p.astype(np.float64), feat.astype(np.float64), target.astype(np.float64))
+    ref_loss, ref_g = D.loss_and_grad(p.astype(np.float64), feat.astype(np.float64), target.astype(np.float64), kind)
     rows = grad_agreement(g, ref_g)
     for name, nb, cos, rel in rows:
         print(f"{name:4s} |g| {nb:.3e} cos {cos:.5f} rel {rel:.4f}")
