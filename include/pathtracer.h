@@ -282,6 +282,7 @@ pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* 
  * server's monotonic clock (pt_server_now). A server is used by one host
  * thread at a time. */
 typedef struct pt_server pt_server;
+typedef struct pt_denoiser pt_denoiser; /* the denoiser section below */
 #define PT_SERVE_SRPT 0 /* shortest estimated remaining GPU time first */
 #define PT_SERVE_FIFO 1 /* arrival order, one request to completion at a time */
 #define PT_SERVE_IDLE 1 /* pt_server_step status: no request to trace */
@@ -305,7 +306,7 @@ typedef struct pt_request_desc {
     int32_t spp_max;       /* sample budget per pixel */
     int32_t path_length;   /* maximum segments per path */
     float target_rel;      /* relative RMSE at which the request is done, e.g. 0.01 */
-    int32_t reserved;
+    int32_t denoise;       /* 1: may finish with the denoiser's image (pt_server_set_denoiser) */
 } pt_request_desc;
 
 typedef struct pt_request_status {
@@ -313,10 +314,14 @@ typedef struct pt_request_status {
     int32_t spp;           /* samples per pixel traced */
     int32_t converged;     /* 1 when rel_error <= target_rel after at least min_spp samples */
     int32_t width, height;
-    double rel_error;      /* last estimate, -1 before two samples */
+    double rel_error;      /* last estimate for the image pt_server_read returns, -1 before two samples */
     double t_arrive, t_admit, t_done;
-    double gpu_seconds;    /* wall time of its admission (scene parse, BVH build) and of the steps that traced it */
+    double gpu_seconds;    /* wall time of its admission (scene parse, BVH build), of the steps that
+                              traced it and of its denoiser runs */
     uint64_t segments;
+    double rel_plain;      /* last estimate for the plain mean image */
+    int32_t denoised;      /* 1 when pt_server_read returns the denoised image */
+    int32_t reserved;
 } pt_request_status;
 
 pt_status pt_server_create(pt_ctx* ctx, const pt_server_config* cfg, pt_server** out);
@@ -333,6 +338,20 @@ pt_status pt_server_step(pt_server* server, pt_request_status* stepped, int32_t 
 pt_status pt_server_status(pt_server* server, int32_t id, pt_request_status* out);
 /* The request's mean XYZ image as RGBA32F texels in the units of pt_accum. */
 pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t n_floats);
+/* Attaches a denoiser (of the server's context; NULL detaches) for requests
+ * with `denoise` set. Such a request also keeps the sums of its odd-index
+ * samples and the first-hit features of a 4-sample camera-ray pass (84 more
+ * bytes per pixel). After min_spp samples, while its plain estimate misses the
+ * target, the error of its denoised image is estimated by denoising each
+ * half-sample image (even / odd sample indices) and comparing it with the
+ * other, independent half, whose variance is subtracted: a conservative
+ * estimate, since the full image has twice the samples of either half. The
+ * request finishes denoised when that estimate meets the target; at its
+ * sample budget it returns the image with the lower estimate, so the plain
+ * mean is the fallback. Resolutions the network does not take (width or
+ * height above the denoiser's maximum or not a multiple of 4) are served
+ * plain. */
+pt_status pt_server_set_denoiser(pt_server* server, pt_denoiser* dn);
 /* Frees a finished or waiting request's device memory and forgets it. */
 pt_status pt_server_release(pt_server* server, int32_t id);
 pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* active, int64_t* bytes_in_use);
@@ -348,7 +367,6 @@ pt_status pt_server_counts(const pt_server* server, int32_t* waiting, int32_t* a
  * sees log1p(XYZ), albedo, normal, 1 / (1 + distance), log1p(standard error)
  * and 1 / sqrt(samples), and predicts log1p(denoised XYZ) - log1p(input XYZ).
  * Radiance is best scaled to order one before the call. */
-typedef struct pt_denoiser pt_denoiser;
 pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width, uint64_t seed, pt_denoiser** out);
 void pt_dn_destroy(pt_denoiser* dn);
 const char* pt_dn_last_error(const pt_denoiser* dn);

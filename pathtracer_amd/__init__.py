@@ -152,7 +152,7 @@ class RequestDesc(C.Structure):
         ("spp_max", C.c_int32),
         ("path_length", C.c_int32),
         ("target_rel", C.c_float),
-        ("reserved", C.c_int32),
+        ("denoise", C.c_int32),
     ]
 
 
@@ -170,6 +170,9 @@ class RequestStatus(C.Structure):
         ("t_done", C.c_double),
         ("gpu_seconds", C.c_double),
         ("segments", C.c_uint64),
+        ("rel_plain", C.c_double),
+        ("denoised", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -224,6 +227,7 @@ SIGNATURES = {
     "pt_server_read": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float), C.c_size_t]),
     "pt_server_release": (C.c_int, [_P, C.c_int32]),
     "pt_server_counts": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "pt_server_set_denoiser": (C.c_int, [_P, _P]),
     "pt_dn_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
     "pt_dn_destroy": (None, [_P]),
     "pt_dn_last_error": (C.c_char_p, [_P]),
@@ -584,9 +588,9 @@ class Server:
         return lib.pt_server_now(self.h)
 
     def submit(self, text: str, width: int, height: int, spp_max: int, path_length: int, target: float = 0.01,
-               t_arrive: float | None = None) -> int:
+               t_arrive: float | None = None, denoise: bool = False) -> int:
         b = text.encode()
-        d = RequestDesc(b, len(b), width, height, spp_max, path_length, target, 0)
+        d = RequestDesc(b, len(b), width, height, spp_max, path_length, target, int(denoise))
         rid = C.c_int32()
         self._ck(lib.pt_server_submit(self.h, C.byref(d), self.now() if t_arrive is None else t_arrive,
                                       C.byref(rid)))
@@ -617,6 +621,11 @@ class Server:
         w, a, b = C.c_int32(), C.c_int32(), C.c_int64()
         self._ck(lib.pt_server_counts(self.h, C.byref(w), C.byref(a), C.byref(b)))
         return w.value, a.value, b.value
+
+    def set_denoiser(self, dn: "Denoiser | None") -> None:
+        """Requests submitted with denoise=True may finish with its image."""
+        self._ck(lib.pt_server_set_denoiser(self.h, dn.h if dn is not None else None))
+        self._dn = dn  # kept alive while attached
 
 
 class Denoiser:

@@ -346,6 +346,81 @@ __global__ void k_wcopy(const float* w, u16* wf, u16* wb, int Cout, int Cin) {
 }
 
 // ---------------------------------------------------------------------------------
+// Served requests: feature images assembled from a request's per-pixel moments
+// and its first-hit features, and the error sums of the dual-buffer estimate.
+
+// which: 0 all samples, 1 the even-index half, 2 the odd-index half. XYZ and
+// the standard error of Y in texel units times inv_scale; the standard error
+// of a half uses the sample variance of all samples.
+__global__ void k_dn_assemble(ptk::DnRequest q, int which, float inv_scale, float* feat) {
+    const int P = q.W * q.H;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int n = q.nsamp[p], no = q.n_odd[p];
+        const int cnt = which == 0 ? n : (which == 1 ? n - no : no);
+        float* f = feat + (size_t)p * kFeat;
+        const float k = cnt > 0 ? q.exposure * inv_scale / (float)cnt : 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            const float s = q.sum[c * P + p], so = q.sum_odd[c * P + p];
+            f[c] = (which == 0 ? s : (which == 1 ? s - so : so)) * k;
+        }
+        float var = 0.0f;
+        if (n > 1) {
+            const float s1 = q.sum[P + p], s2 = q.sumsq[P + p];
+            var = fmaxf(s2 - s1 * (s1 / (float)n), 0.0f) / (float)(n - 1);
+        }
+        f[10] = cnt > 0 ? sqrtf(var / (float)cnt) * q.exposure * inv_scale : 0.0f;
+        f[11] = (float)cnt;
+        const int na = q.aov_n[p];
+        const float ka = na > 0 ? 1.0f / (float)na : 0.0f;
+        for (int c = 3; c < 10; ++c) f[c] = q.aov[(size_t)p * kFeat + c] * ka;
+    }
+}
+
+// Against the other half (other: 1 even, 2 odd) of the same request, per block
+// then one double atomic per block into line blockIdx.x % 64 (stride 32):
+// slot 0 sum (den - other)^2, slot 1 sum of the other half's variance of the
+// mean, slot 2 sum of the full mean^2, all over pixels and XYZ in texel units
+// times inv_scale.
+__global__ void k_dn_cross(ptk::DnRequest q, const float* den, int other, float inv_scale, double* lines) {
+    __shared__ double red[3][4];
+    const int P = q.W * q.H;
+    double e = 0.0, v = 0.0, m = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const int n = q.nsamp[p], no = q.n_odd[p];
+        const int cnt = other == 1 ? n - no : no;
+        if (cnt < 1 || n < 2) continue;
+        const double k = (double)q.exposure * inv_scale;
+        for (int c = 0; c < 3; ++c) {
+            const double s = q.sum[c * P + p], so = q.sum_odd[c * P + p], s2 = q.sumsq[c * P + p];
+            const double o = (other == 1 ? s - so : so) / cnt * k;
+            const double d = (double)den[(size_t)p * 3 + c] - o;
+            e += d * d;
+            const double var = fmax(s2 - s * (s / n), 0.0) / (double)(n - 1);
+            v += var / cnt * k * k;
+            const double mean = s / n * k;
+            m += mean * mean;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_down(e, off);
+        v += __shfl_down(v, off);
+        m += __shfl_down(m, off);
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = e; red[1][wv] = v; red[2][wv] = m; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[threadIdx.x][i];
+        atomicAdd(lines + (size_t)(blockIdx.x % 64) * 32 + threadIdx.x, t);
+    }
+}
+
+__global__ void k_dn_scale(const float* den, float scale, float* out, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = den[i] * scale;
+}
+
+// ---------------------------------------------------------------------------------
 // Host side.
 struct Layer {
     int cin1, cin2, cout;  // input split (concat), output channels
@@ -516,6 +591,63 @@ int check_shape(pt_denoiser* d, int N, int H, int W) {
 }
 
 }  // namespace
+
+namespace ptk {
+
+int dn_fits(const pt_denoiser* d, int W, int H) {
+    return d && W > 0 && H > 0 && W <= d->maxW && H <= d->maxH && W % 4 == 0 && H % 4 == 0;
+}
+
+pt_ctx* dn_ctx(const pt_denoiser* d) { return d ? d->ctx : nullptr; }
+
+namespace {
+// Denoises half `which` of the request into d->den (texel units / scale).
+void dn_half(pt_denoiser* d, const DnRequest& q, int which, float inv_scale) {
+    const size_t P = (size_t)q.W * q.H;
+    ptk::launch(d->ctx, "k_dn_assemble", k_dn_assemble, ew_grid(P), dim3(256), q, which, inv_scale, d->feat);
+    forward(d, 1, q.H, q.W);
+    ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
+                (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, P, 0);
+}
+}  // namespace
+
+int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, double* d_lines, double* rel) {
+    if (!dn_fits(d, q.W, q.H)) return dn_fail(d, PT_ERR_INVALID_ARG, "denoiser: request outside the network's limits");
+    pt_ctx* ctx = d->ctx;
+    const size_t P = (size_t)q.W * q.H;
+    const float inv = 1.0f / scale;
+    hipError_t e = hipMemsetAsync(d_lines, 0, 2 * 64 * 32 * sizeof(double), ctx->stream);
+    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
+    const dim3 grid((unsigned)std::min<size_t>((P + 255) / 256, 2048));
+    dn_half(d, q, 1, inv);  // D(even half) against the odd half
+    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, 2, inv, d_lines);
+    dn_half(d, q, 2, inv);  // D(odd half) against the even half
+    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, 1, inv, d_lines + 64 * 32);
+    std::vector<double> h((size_t)2 * 64 * 32);
+    e = hipMemcpyAsync(h.data(), d_lines, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
+    double s[2][3] = {};
+    for (int k = 0; k < 2; ++k)
+        for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < 3; ++j) s[k][j] += h[(size_t)k * 64 * 32 + (size_t)l * 32 + j];
+    // E|D(A) - B|^2 = |D(A) - mu|^2 + E|B - mu|^2 for independent halves A, B
+    const double err2 = 0.5 * ((s[0][0] - s[0][1]) + (s[1][0] - s[1][1]));
+    const double m = s[0][2];
+    *rel = m > 0.0 ? std::sqrt(std::max(err2, 0.0) / m) : INFINITY;
+    return PT_OK;
+}
+
+int dn_request_denoise(pt_denoiser* d, const DnRequest& q, float scale, float* d_out) {
+    if (!dn_fits(d, q.W, q.H)) return dn_fail(d, PT_ERR_INVALID_ARG, "denoiser: request outside the network's limits");
+    const size_t P = (size_t)q.W * q.H;
+    dn_half(d, q, 0, 1.0f / scale);
+    ptk::launch(d->ctx, "k_dn_scale", k_dn_scale, ew_grid(P * 3), dim3(256), (const float*)d->den, scale, d_out,
+                (int)(P * 3));
+    return PT_OK;
+}
+
+}  // namespace ptk
 
 extern "C" {
 

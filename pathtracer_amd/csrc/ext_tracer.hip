@@ -1552,8 +1552,10 @@ struct ExtMomentSink {
     float* sum;
     float* sumsq;
     int* nsamp;
+    float* sum_odd;  // optional: sums and counts of the samples with an odd index
+    int* n_odd;
     int P;
-    int pad;
+    int first;       // index of the quantum's first sample
 };
 
 // Batched serving sink: one thread per (segment, pixel) adds the pixel's
@@ -1579,6 +1581,20 @@ __global__ void __launch_bounds__(kBlk) k_ext_moments_batch(const ExtSeg* segs, 
         K.sum[px] = ax; K.sum[P + px] = ay; K.sum[2 * P + px] = az;
         K.sumsq[px] = qx; K.sumsq[P + px] = qy; K.sumsq[2 * P + px] = qz;
         K.nsamp[px] = n;
+        if (K.sum_odd) {
+            // the odd-index samples alone: two independent half images for
+            // the denoiser's error estimate
+            float ox = K.sum_odd[px], oy = K.sum_odd[P + px], oz = K.sum_odd[2 * P + px];
+            int no = K.n_odd[px];
+            for (int s = (K.first + 1) & 1; s < g.m; s += 2) {
+                float4 a = Wv.acc[g.base + s * g.pc + j];
+                if (a.x != a.x || a.y != a.y || a.z != a.z) continue;
+                ox += a.x; oy += a.y; oz += a.z;
+                ++no;
+            }
+            K.sum_odd[px] = ox; K.sum_odd[P + px] = oy; K.sum_odd[2 * P + px] = oz;
+            K.n_odd[px] = no;
+        }
     }
     wave_add_u64(stat_slot(Wv.stats, 4), nans);
 }
@@ -2220,7 +2236,8 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
         scenes[(size_t)i] = H.dev;
         frames[(size_t)i] = frame_of(it.pc);
         segs[(size_t)i] = ExtSeg{i, (int)npaths, 0, P, 0, q, (int)npix, 0};
-        sinks[(size_t)i] = ExtMomentSink{it.sum, it.sumsq, it.nsamp, P, 0};
+        sinks[(size_t)i] = ExtMomentSink{it.sum, it.sumsq, it.nsamp, it.sum_odd, it.n_odd, P,
+                                         it.pc.frame - it.pc.samplesPerFrame};
         npaths += (int64_t)P * q;
         npix += P;
         depth = std::max(depth, it.pc.pathLength);

@@ -17,7 +17,14 @@
 //     time (PT_SERVE_FIFO, the one-job-at-a-time baseline);
 //   - after each quantum the relative RMSE is estimated on the GPU from the
 //     per-pixel second moments (k_ext_rel_error): sqrt(sum var/n) / sqrt(sum mean^2)
-//     over pixels and XYZ channels.
+//     over pixels and XYZ channels;
+//   - a request that asks for denoising also keeps the sums of its odd-index
+//     samples and the first-hit features of a 4-sample camera-ray pass; its
+//     denoised image's error is estimated by denoising each half-sample image
+//     and comparing it with the other half (ptk::dn_request_estimate). It
+//     finishes denoised when that estimate reaches the target before the
+//     plain one does; at its sample budget it returns whichever image has the
+//     lower estimate (the plain mean is the fallback).
 // The extended tracer's random numbers are keyed on (scene seed, pixel, sample,
 // bounce, dimension), so a request's image does not depend on how its samples
 // were split into quanta or interleaved with other requests.
@@ -48,10 +55,21 @@ struct Request {
     float target = 0.01f;
     pt_scene* scene = nullptr;
     pt_push_constants pc{};
-    DevAlloc mem;  // sum (3P floats), sumsq (3P floats), nsamp (P ints)
+    DevAlloc mem;  // sum (3P floats), sumsq (3P floats), nsamp (P ints); denoised requests also:
+                   // sum_odd (3P), n_odd (P), aov (12P), aov_n (P), den (3P)
     float* sum = nullptr;
     float* sumsq = nullptr;
     int* nsamp = nullptr;
+    bool denoise = false;  // asked for, and the network takes the resolution
+    float* sum_odd = nullptr;
+    int* n_odd = nullptr;
+    float* aov = nullptr;
+    int* aov_n = nullptr;
+    float* den = nullptr;
+    double rel_dn = INFINITY;  // dual-buffer estimate of the denoised image
+    int dn_spp = 0;            // samples at that estimate
+    bool denoised = false;     // the result is the denoised image
+    double mean2 = 0.0;        // sum of mean^2 over pixels and XYZ (per-sample units)
     int64_t bytes = 0;  // scene + moment buffers, counted against the server budget
     int spp = 0;
     double rel = INFINITY;
@@ -73,6 +91,7 @@ struct pt_server {
     int64_t bytes_in_use = 0;
     double sec_per_path = 0.0;  // running estimate of GPU seconds per traced path
     double* d_rel = nullptr;    // per request of a wave: 64 lines x 32 doubles
+    pt_denoiser* dn = nullptr;  // for requests that ask for denoising
     std::string err;
 };
 
@@ -93,7 +112,7 @@ int64_t estimate_bytes(const Request& r) {
     int64_t prims = 0;
     for (size_t i = r.json.find("\"radius\""); i != std::string::npos; i = r.json.find("\"radius\"", i + 1)) ++prims;
     for (size_t i = r.json.find("\"edge1\""); i != std::string::npos; i = r.json.find("\"edge1\"", i + 1)) ++prims;
-    return (int64_t)r.P * 28 + prims * 320 + (1 << 20);
+    return (int64_t)r.P * (r.denoise ? 112 : 28) + prims * 320 + (1 << 20);
 }
 
 void release_request(pt_server* s, Request& r) {
@@ -114,7 +133,7 @@ int admit(pt_server* s, Request& r, double now) {
     r.pc.resolution[1] = r.H;
     r.pc.pathLength = r.path_length;
     const size_t P = (size_t)r.P;
-    st = ptk::dev_alloc(ctx, r.mem, P * 28 + 3 * 256);
+    st = ptk::dev_alloc(ctx, r.mem, P * (r.denoise ? 112 : 28) + 8 * 256);
     if (st != PT_OK) return fail(s, st, ctx->err);
     char* p = static_cast<char*>(r.mem.p);
     auto carve = [&](size_t n) {
@@ -125,11 +144,28 @@ int admit(pt_server* s, Request& r, double now) {
     r.sum = reinterpret_cast<float*>(carve(3 * P * 4));
     r.sumsq = reinterpret_cast<float*>(carve(3 * P * 4));
     r.nsamp = reinterpret_cast<int*>(carve(P * 4));
+    if (r.denoise) {
+        r.sum_odd = reinterpret_cast<float*>(carve(3 * P * 4));
+        r.n_odd = reinterpret_cast<int*>(carve(P * 4));
+        r.aov = reinterpret_cast<float*>(carve(12 * P * 4));
+        r.aov_n = reinterpret_cast<int*>(carve(P * 4));
+        r.den = reinterpret_cast<float*>(carve(3 * P * 4));
+    }
     hipError_t e = hipMemsetAsync(r.mem.p, 0, r.mem.bytes, ctx->stream);
     if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "request buffers"), ctx->err);
     // build the extended scene (BVH upload) now, so its bytes are known
     st = ptk::ext_scene_build(ctx, r.scene);
     if (st != PT_OK) return fail(s, st, ctx->err);
+    if (r.denoise) {
+        // first-hit features: 4 camera rays per pixel, traced one segment deep
+        pt_push_constants fc = r.pc;
+        fc.pathLength = 1;
+        fc.samplesPerFrame = 4;
+        fc.frame = 4;
+        fc.currentSamples = 4;
+        st = ptk::ext_dispatch_features(ctx, r.scene, &fc, r.aov, r.aov_n);
+        if (st != PT_OK) return fail(s, st, ctx->err);
+    }
     int64_t sb = 0;
     (void)pt_scene_device_bytes(r.scene, &sb);
     r.bytes = (int64_t)r.mem.bytes + sb;
@@ -143,14 +179,61 @@ int admit(pt_server* s, Request& r, double now) {
 // up to the minimum before a convergence decision.
 int remaining_spp(const Request& r, int min_spp) {
     const int left = r.spp_max - r.spp;
-    if (r.spp < 2 || !std::isfinite(r.rel)) return left;
-    double need = (double)r.spp * ((r.rel / r.target) * (r.rel / r.target) - 1.0);
+    double rel = r.rel;
+    if (r.denoise && r.dn_spp > 0) rel = std::min(rel, r.rel_dn * std::sqrt((double)r.dn_spp / r.spp));
+    if (r.spp < 2 || !std::isfinite(rel)) return left;
+    double need = (double)r.spp * ((rel / r.target) * (rel / r.target) - 1.0);
     need = std::max(need, (double)(min_spp - r.spp));
     return (int)std::min<double>(left, std::max(1.0, std::ceil(need)));
 }
 
 double remaining_cost(const pt_server* s, const Request& r) {
     return (double)remaining_spp(r, s->cfg.min_spp) * r.P * (s->sec_per_path > 0.0 ? s->sec_per_path : 1e-9);
+}
+
+ptk::DnRequest dn_view(const Request& r) {
+    ptk::DnRequest q{};
+    q.sum = r.sum;
+    q.sumsq = r.sumsq;
+    q.nsamp = r.nsamp;
+    q.sum_odd = r.sum_odd;
+    q.n_odd = r.n_odd;
+    q.aov = r.aov;
+    q.aov_n = r.aov_n;
+    q.W = r.W;
+    q.H = r.H;
+    q.exposure = r.pc.apertureSize * r.pc.apertureSize * (float)r.pc.ISO;
+    return q;
+}
+
+// After a quantum of a request that asks for denoising and has not met its
+// target plainly: estimate the denoised image's error when it may have met
+// the target (its last estimate, extrapolated as 1/sqrt(samples), within 20 %)
+// or at the sample budget; finish denoised when the estimate meets the target,
+// or at the budget when it beats the plain estimate. The denoiser's time is
+// the request's GPU time.
+int denoise_step(pt_server* s, Request& r, bool last) {
+    const bool due = r.dn_spp == 0 || last ||
+                     r.rel_dn * std::sqrt((double)r.dn_spp / r.spp) <= 1.2 * r.target;
+    if (!due || r.spp < 2 || !(r.mean2 > 0.0)) return PT_OK;
+    const auto t = Clock::now();
+    const ptk::DnRequest q = dn_view(r);
+    // radiance of order one for the network: the RMS of the mean image
+    const float scale = (float)(q.exposure * std::sqrt(r.mean2 / (3.0 * r.P)));
+    double rel = INFINITY;
+    int st = ptk::dn_request_estimate(s->dn, q, scale, s->d_rel, &rel);
+    if (st != PT_OK) return fail(s, st, std::string("denoiser: ") + pt_dn_last_error(s->dn));
+    r.rel_dn = rel;
+    r.dn_spp = r.spp;
+    if (rel <= r.target || (last && rel < r.rel)) {
+        st = ptk::dn_request_denoise(s->dn, q, scale, r.den);
+        if (st != PT_OK) return fail(s, st, std::string("denoiser: ") + pt_dn_last_error(s->dn));
+        hipError_t e = hipStreamSynchronize(s->ctx->stream);
+        if (e != hipSuccess) return fail(s, ptk::hip_fail(s->ctx, e, "denoiser"), s->ctx->err);
+        r.denoised = true;
+    }
+    r.gpu_s += std::chrono::duration<double>(Clock::now() - t).count();
+    return PT_OK;
 }
 
 }  // namespace
@@ -196,6 +279,8 @@ pt_status pt_server_submit(pt_server* s, const pt_request_desc* d, double t_arri
     if (d->width <= 0 || d->height <= 0 || (int64_t)d->width * d->height > (int64_t)1 << 26 || d->spp_max <= 0 ||
         d->path_length <= 0 || !(d->target_rel > 0.0f))
         return fail(s, PT_ERR_INVALID_ARG, "pt_server_submit: bad request");
+    if (d->denoise && !s->dn)
+        return fail(s, PT_ERR_STATE, "pt_server_submit: denoising asked for but no denoiser attached");
     auto r = std::make_unique<Request>();
     r->id = s->next_id++;
     r->json.assign(d->json, d->len);
@@ -205,6 +290,8 @@ pt_status pt_server_submit(pt_server* s, const pt_request_desc* d, double t_arri
     r->spp_max = d->spp_max;
     r->path_length = d->path_length;
     r->target = d->target_rel;
+    // a resolution the network does not take is served without denoising
+    r->denoise = d->denoise != 0 && ptk::dn_fits(s->dn, r->W, r->H);
     r->t_arrive = t_arrive;
     *id = r->id;
     s->waiting.push_back(r->id);
@@ -266,6 +353,8 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
         it.sum = r.sum;
         it.sumsq = r.sumsq;
         it.nsamp = r.nsamp;
+        it.sum_odd = r.sum_odd;
+        it.n_odd = r.n_odd;
         items.push_back(it);
         in.push_back(&r);
         left -= (int64_t)q * r.P;
@@ -293,7 +382,6 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
     (void)pt_counters_get(ctx, &c1);
     const double total_paths = (double)(wave - left);
     s->sec_per_path = s->sec_per_path > 0.0 ? 0.8 * s->sec_per_path + 0.2 * dt / total_paths : dt / total_paths;
-    const double now = now_s(s);
     for (size_t i = 0; i < items.size(); ++i) {
         Request& r = *in[i];
         double v = 0.0, m = 0.0;
@@ -305,11 +393,18 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
         const double share = (double)q * r.P / total_paths;
         r.spp += q;
         r.rel = m > 0.0 ? std::sqrt(v / m) : INFINITY;
+        r.mean2 = m;
         r.gpu_s += dt * share;
         r.segments += (uint64_t)((double)(c1.segments - c0.segments) * share);
-        if ((r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max) {
+        bool done = (r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max;
+        if (r.denoise && r.spp >= s->cfg.min_spp && !(r.rel <= r.target)) {
+            st = denoise_step(s, r, r.spp >= r.spp_max);
+            if (st != PT_OK) return st;
+            done = done || r.denoised;
+        }
+        if (done) {
             r.state = PT_REQ_DONE;
-            r.t_done = now;
+            r.t_done = now_s(s);
             s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
         }
         if (n_stepped && (int)i < cap) {
@@ -329,8 +424,11 @@ pt_status pt_server_status(pt_server* s, int32_t id, pt_request_status* out) {
     out->id = r.id;
     out->state = r.state;
     out->spp = r.spp;
-    out->converged = r.spp >= s->cfg.min_spp && r.rel <= r.target;
-    out->rel_error = std::isfinite(r.rel) ? r.rel : -1.0;
+    const double rel = r.denoised ? r.rel_dn : r.rel;
+    out->converged = r.spp >= s->cfg.min_spp && rel <= r.target;
+    out->rel_error = std::isfinite(rel) ? rel : -1.0;
+    out->denoised = r.denoised;
+    out->rel_plain = std::isfinite(r.rel) ? r.rel : -1.0;
     out->t_arrive = r.t_arrive;
     out->t_admit = r.t_admit;
     out->t_done = r.t_done;
@@ -349,10 +447,21 @@ pt_status pt_server_read(pt_server* s, int32_t id, float* host_rgba, size_t n_fl
     if (!r.mem.p) return fail(s, PT_ERR_STATE, "pt_server_read: request not admitted or released");
     if (n_floats < (size_t)r.P * 4) return fail(s, PT_ERR_CAPACITY, "pt_server_read: buffer too small");
     const size_t P = (size_t)r.P;
-    std::vector<float> sum(3 * P);
-    std::vector<int> n(P);
     pt_ctx* ctx = s->ctx;
     (void)hipSetDevice(ctx->device);
+    if (r.denoised) {
+        std::vector<float> den(3 * P);
+        hipError_t e = hipMemcpyAsync(den.data(), r.den, 3 * P * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_read"), ctx->err);
+        for (size_t i = 0; i < P; ++i) {
+            for (int c = 0; c < 3; ++c) host_rgba[4 * i + c] = den[3 * i + c];
+            host_rgba[4 * i + 3] = 1.0f;
+        }
+        return PT_OK;
+    }
+    std::vector<float> sum(3 * P);
+    std::vector<int> n(P);
     hipError_t e = hipMemcpyAsync(sum.data(), r.sum, 3 * P * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(n.data(), r.nsamp, P * 4, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -379,6 +488,16 @@ pt_status pt_server_release(pt_server* s, int32_t id) {
     (void)hipStreamSynchronize(s->ctx->stream);
     release_request(s, r);
     s->reqs.erase(it);
+    return PT_OK;
+}
+
+pt_status pt_server_set_denoiser(pt_server* s, pt_denoiser* dn) {
+    if (!s) return PT_ERR_INVALID_ARG;
+    if (dn && ptk::dn_ctx(dn) != s->ctx) return fail(s, PT_ERR_INVALID_ARG, "pt_server_set_denoiser: another context");
+    for (const auto& kv : s->reqs)
+        if (kv.second->denoise && (kv.second->state == PT_REQ_ACTIVE || kv.second->state == PT_REQ_WAITING))
+            return fail(s, PT_ERR_STATE, "pt_server_set_denoiser: requests in flight use the denoiser");
+    s->dn = dn;
     return PT_OK;
 }
 

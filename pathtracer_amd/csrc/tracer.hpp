@@ -186,8 +186,37 @@ struct ExtBatchItem {
     float* sum = nullptr;
     float* sumsq = nullptr;
     int* nsamp = nullptr;
+    float* sum_odd = nullptr;  // optional: sums / counts of the odd-index samples
+    int* n_odd = nullptr;
 };
 constexpr int kMaxBatch = 64;
+
+// A served request's buffers as the denoiser reads them: per-pixel moments
+// (sum, sumsq: 3P floats channel-major; nsamp: P), sums and counts of the
+// odd-index samples, and the sums / counts of a first-hit features pass
+// (PT_FEATURES floats per pixel as ext_dispatch_features writes them).
+struct DnRequest {
+    const float* sum;
+    const float* sumsq;
+    const int* nsamp;
+    const float* sum_odd;
+    const int* n_odd;
+    const float* aov;
+    const int* aov_n;
+    int W, H;
+    float exposure;  // apertureSize^2 * ISO
+};
+// Whether the network takes a W x H image (within its limits, multiples of 4).
+int dn_fits(const pt_denoiser* d, int W, int H);
+pt_ctx* dn_ctx(const pt_denoiser* d);
+// Dual-buffer estimate of the relative RMSE of the request's denoised image:
+// each half-sample image is denoised and compared with the other, independent
+// half, whose own variance is subtracted; the mean over the two halves, over
+// sqrt(sum of mean^2). Inputs are divided by `scale` (an RMS of the image).
+// d_lines: device scratch of 2 x 64 x 32 doubles. Synchronous.
+int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, double* d_lines, double* rel);
+// The denoised image of all samples into d_out (3 floats per pixel, texel units).
+int dn_request_denoise(pt_denoiser* d, const DnRequest& q, float scale, float* d_out);
 // All items' paths in one wave (their total must fit the context's wave
 // capacity); each request's samples are added to its moments in sample order,
 // so a request's result does not depend on what else is in the wave.

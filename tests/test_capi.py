@@ -39,7 +39,7 @@ def test_struct_layouts():
     # serving structs (no reference counterpart): C layouts on x86-64
     assert C.sizeof(pt.ServerConfig) == 24
     assert C.sizeof(pt.RequestDesc) == 40 and pt.RequestDesc.target_rel.offset == 32
-    assert C.sizeof(pt.RequestStatus) == 72 and pt.RequestStatus.rel_error.offset == 24
+    assert C.sizeof(pt.RequestStatus) == 88 and pt.RequestStatus.rel_error.offset == 24
 
 
 def test_argument_validation_without_device():

@@ -134,3 +134,78 @@ def test_bad_requests_are_rejected(gpu_ctx):
             srv.step()
         assert srv.status(rid).state == pt.REQ_FAILED
         assert srv.step() is None
+
+
+def attach_shipped_denoiser(ctx, srv, size=128):
+    from pathtracer_amd import denoise_train as T
+    dn = pt.Denoiser(ctx, 1, size, size)
+    T.load_weights(dn)
+    srv.set_denoiser(dn)
+    return dn
+
+
+def test_denoised_result_is_the_same_alone_and_in_the_mix():
+    """A request that may finish denoised, run to its sample budget: its image
+    (denoised or, as fallback, plain) depends only on its own sums."""
+    text = scenes.sdf_scene(jitter=0.05, scene_seed=4, seed=12)
+    others = [(scenes.cornell(4, seed=1), 24, 24, 6), (scenes.random_spheres(n=64, scene_seed=9, seed=3), 40, 20, 12)]
+    outs = []
+    for wave, mix, graph in ((1 << 22, False, 0), (20000, True, 0), (30000, True, 1)):
+        with pt.Context(0, wave_paths=wave, graph_mode=graph) as ctx, pt.Server(ctx) as srv:
+            dn = attach_shipped_denoiser(ctx, srv, 64)
+            rid = srv.submit(text, 48, 32, spp_max=40, path_length=6, target=1e-6, denoise=True)
+            extra = [srv.submit(t, w, h, spp_max=24, path_length=d, target=1e-6) for t, w, h, d in others] if mix else []
+            st = run_until_done(srv, [rid] + extra)[rid]
+            assert st.state == pt.REQ_DONE and st.spp == 40
+            outs.append((srv.read(rid), st.denoised, st.rel_error, st.rel_plain))
+            dn.close()
+    for img, den, rel, relp in outs[1:]:
+        assert np.array_equal(outs[0][0], img) and den == outs[0][1] and rel == outs[0][2] and relp == outs[0][3]
+    img, den, rel, relp = outs[0]
+    # at the budget the image with the lower estimate is returned
+    assert rel == (min(rel, relp) if den else relp)
+    assert np.isfinite(img).all()
+
+
+def test_denoised_request_meets_its_target_with_fewer_samples(gpu_ctx):
+    """A 2 % target on the mandelbulb + menger scene: the denoised request
+    stops several times earlier than the plain one, and its actual relative
+    RMSE against an 8192-sample render is within the target (the dual-buffer
+    estimate is conservative)."""
+    from pathtracer_amd import denoise_train as T
+    text, depth = T.training_scene(101)
+    W = H = 128
+    with pt.Server(gpu_ctx) as srv:
+        dn = attach_shipped_denoiser(gpu_ctx, srv, 128)
+        a = srv.submit(text, W, H, spp_max=4096, path_length=depth, target=0.02, denoise=True)
+        b = srv.submit(text, W, H, spp_max=4096, path_length=depth, target=0.02)
+        st = run_until_done(srv, [a, b])
+        img_dn = srv.read(a)[..., :3]
+        img_plain = srv.read(b)[..., :3]
+        dn.close()
+    sa, sb = st[a], st[b]
+    print(f"\ndenoised: spp {sa.spp} est {sa.rel_error:.4f} (plain est {sa.rel_plain:.4f}); "
+          f"plain: spp {sb.spp} est {sb.rel_error:.4f}")
+    assert sa.converged and sa.denoised and sb.converged and not sb.denoised
+    assert sa.spp * 3 <= sb.spp
+    sc = gpu_ctx.scene_from_json(text)
+    ref = T.render_mean(gpu_ctx, sc, W, H, 8192, depth, first=10 ** 6)
+    sc.release()
+    act_dn, act_plain = T.rel_rmse(img_dn, ref), T.rel_rmse(img_plain, ref)
+    print(f"actual: denoised {act_dn:.4f}, plain {act_plain:.4f}")
+    assert act_dn <= 0.02 and act_plain <= 0.03
+
+
+def test_denoising_falls_back_to_plain(gpu_ctx):
+    text = scenes.sdf_scene(seed=3)
+    with pt.Server(gpu_ctx) as srv:
+        with pytest.raises(pt.PTError):  # no denoiser attached
+            srv.submit(text, 32, 32, spp_max=8, path_length=4, denoise=True)
+        dn = attach_shipped_denoiser(gpu_ctx, srv, 64)
+        # 30 x 30 is not a multiple of 4, 96 x 64 is wider than the network's 64: served plain
+        ids = [srv.submit(text, w, h, spp_max=16, path_length=4, target=1e-6, denoise=True)
+               for w, h in ((30, 30), (96, 64))]
+        st = run_until_done(srv, ids)
+        assert all(st[i].state == pt.REQ_DONE and not st[i].denoised for i in ids)
+        assert all(st[i].rel_error == st[i].rel_plain for i in ids)
+        dn.close()
