@@ -68,6 +68,10 @@ def parse_args():
     ap.add_argument("--serve-target", type=float, default=0.01, help="relative RMSE at which a request is done")
     ap.add_argument("--serve-load", type=float, default=0.85,
                     help="utilisation of the one-request-at-a-time tracer that sets the Poisson arrival rate")
+    ap.add_argument("--serve-denoise", action="store_true",
+                    help="the SRPT server may finish requests with the shipped denoiser's image")
+    ap.add_argument("--serve-speedup", type=float, default=1.0,
+                    help="the SRPT server's arrivals come this many times faster than the one-at-a-time ones")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run --steps device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -140,10 +144,13 @@ def serve_bench(args):
        completion (FIFO server, all queued at once).
     2. Poisson arrivals (seeded) at the rate that loads that one-at-a-time
        tracer to --serve-load.
-    3. The same arrivals served in real time by the SRPT server; per request the
-       time from arrival to its relative-RMSE target (or to its sample budget).
-    4. The one-at-a-time (FIFO) response times for the same arrivals, from the
-       service times of step 1 (a FIFO queue is exact given service times).
+    3. The same arrival sequence, compressed --serve-speedup times, served in
+       real time by the SRPT server (continuous batching; with --serve-denoise
+       requests may finish denoised); per request the time from arrival to its
+       relative-RMSE target (or to its sample budget).
+    4. The one-at-a-time (FIFO, plain) response times for the uncompressed
+       arrivals, from the service times of step 1 (a FIFO queue is exact given
+       service times).
     Prints one JSON line."""
     import numpy as np
     import pathtracer_amd as pt
@@ -179,14 +186,24 @@ def serve_bench(args):
     # 3. SRPT in real time
     done = {}
     segments = 0
+    dn = None
     with pt.Server(ctx, policy=pt.SERVE_SRPT) as srv:
+        if args.serve_denoise:
+            from pathtracer_amd import denoise_train
+            hmax = max(r["height"] for r in reqs)
+            wmax = max(r["width"] for r in reqs)
+            dn = pt.Denoiser(ctx, 1, (hmax + 3) // 4 * 4, (wmax + 3) // 4 * 4)
+            denoise_train.load_weights(dn)
+            srv.set_denoiser(dn)
         base = srv.now()
-        arrive = base + offsets
+        arrive = base + offsets / args.serve_speedup
         ids, nxt = {}, 0
         while len(done) < len(reqs):
             now = srv.now()
             while nxt < len(reqs) and arrive[nxt] <= now:
-                ids[submit(srv, reqs[nxt], float(arrive[nxt]))] = reqs[nxt]["id"]
+                r = reqs[nxt]
+                ids[srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target,
+                               t_arrive=float(arrive[nxt]), denoise=args.serve_denoise)] = r["id"]
                 nxt += 1
             sts = srv.step()
             if sts is None:
@@ -201,6 +218,8 @@ def serve_bench(args):
                     segments += d["segments"]
                     srv.release(st.id)
         makespan = srv.now() - base
+    if dn is not None:
+        dn.close()
 
     # 4. one request at a time, same arrivals
     fifo_resp, c = [], 0.0
@@ -225,15 +244,21 @@ def serve_bench(args):
                                "depth 8); resolution U{256^2, 512^2, 1024x768, 1920x1080}; "
                                "spp budget log-uniform 16..1024; Poisson arrivals",
                    "requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
-                   "arrival_rate_per_s": rate},
+                   "arrival_rate_per_s": rate, "srpt_arrival_speedup": args.serve_speedup,
+                   "srpt_denoise": bool(args.serve_denoise)},
         "srpt": {"p50_time_to_rmse_s": percentile(resp_conv, 50), "p99_time_to_rmse_s": percentile(resp_conv, 99),
                  "converged": len(conv), "p50_response_s": percentile(resp, 50),
                  "p99_response_s": percentile(resp, 99), "makespan_s": makespan,
-                 "segments_per_s": segments / makespan},
+                 "segments_per_s": segments / makespan,
+                 "denoised": int(sum(done[r["id"]]["denoised"] for r in reqs)),
+                 "mean_spp": float(np.mean([done[r["id"]]["spp"] for r in reqs])),
+                 "requests_per_s": len(reqs) / makespan},
         "one_at_a_time": {"p50_time_to_rmse_s": percentile(fifo_conv, 50),
                           "p99_time_to_rmse_s": percentile(fifo_conv, 99), "converged": int(alone_converged),
                           "p50_response_s": percentile(fifo_resp, 50), "p99_response_s": percentile(fifo_resp, 99),
-                          "mean_service_s": float(s_alone.mean())},
+                          "mean_service_s": float(s_alone.mean()),
+                          "mean_spp": float(np.mean([service[r["id"]]["spp"] for r in reqs])),
+                          "requests_per_s": len(reqs) / max(c, 1e-9)},
         "bench_seconds": time.perf_counter() - t_start,
     }
     print(json.dumps(line))

@@ -124,7 +124,8 @@ typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
 /* Launch/tuning knobs (0 = library default). */
 typedef struct pt_config {
     int32_t wave_paths;    /* paths in flight per wave (queue capacity) */
-    int32_t block_size;    /* threads per block of the queue kernels (multiple of 64) */
+    int32_t block_size;    /* threads per block of the queue kernels: 0 or 256 (they are compiled for 256
+                              lanes, 4 wavefronts; other values are rejected) */
     int32_t bvh_width;     /* 2, 4 or 8 (0 = 2) */
     int32_t sdf_max_steps; /* sphere-tracing budget of the extended mode */
     int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
@@ -132,7 +133,7 @@ typedef struct pt_config {
     int32_t bvh_leaf;      /* most primitives per BVH leaf, 1..16 (0 = 1) */
     int32_t graph_mode;    /* batched serving waves: 0 = captured once per launch shape into a hipGraph and
                               replayed, 1 = kernels launched one by one */
-    int32_t reserved;
+    int32_t blocks_per_cu; /* extended mode: grid of the queue kernels, blocks per compute unit, 1..64 (0 = 8) */
 } pt_config;
 
 typedef struct pt_counters {
@@ -150,6 +151,7 @@ typedef struct pt_counters {
     uint64_t conductor_hits;
     uint64_t bvh_nodes;       /* BVH nodes visited by extension rays (extended mode) */
     uint64_t shadow_bvh_nodes;/* BVH nodes visited by shadow rays (extended mode) */
+    uint64_t sdf_exhausted;   /* SDF marches that ran out of sdf_max_steps (taken as misses; extended mode) */
 } pt_counters;
 
 typedef struct pt_ctx pt_ctx;

@@ -80,7 +80,8 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     if (!ctx) return set_global(PT_ERR_INVALID_ARG, "pt_create: out of host memory");
     ctx->device = device_ordinal;
     if (cfg) ctx->cfg = *cfg;
-    if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && (ctx->cfg.block_size % 64) != 0) ||
+    if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && ctx->cfg.block_size != 256) ||
+        ctx->cfg.blocks_per_cu < 0 || ctx->cfg.blocks_per_cu > 64 || ctx->cfg.sdf_max_steps < 0 ||
         (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
         ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16 ||
         ctx->cfg.graph_mode < 0 || ctx->cfg.graph_mode > 1) {
@@ -465,6 +466,7 @@ pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     out->conductor_hits = h[10];
     out->bvh_nodes = h[11];
     out->shadow_bvh_nodes = h[12];
+    out->sdf_exhausted = h[13];
     return PT_OK;
 }
 

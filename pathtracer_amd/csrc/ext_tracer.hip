@@ -509,7 +509,8 @@ __device__ __forceinline__ float ext_de(const ExtSdf& s, f3 p) {
 // Sphere tracing of one SDF object inside its box, up to tmax. Returns the hit
 // distance or a negative value.
 template <int KIND>
-__device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_steps, unsigned& steps) {
+__device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_steps, unsigned& steps,
+                           unsigned& exhausted) {
     f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     f3 t0 = (s.lo - o) * inv, t1 = (s.hi - o) * inv;
     f3 a = min3(t0, t1), b = max3(t0, t1);
@@ -524,6 +525,7 @@ __device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_step
         t += dist;
         if (t > tout) return -1.0f;
     }
+    ++exhausted;  // the step budget ran out before a hit or the box's exit: a miss
     return -1.0f;
 }
 template <int KIND>
@@ -1010,7 +1012,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
 template <int KIND, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
-    unsigned steps = 0;
+    unsigned steps = 0, exhausted = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
@@ -1021,7 +1023,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
         bool upd = false;
         for (int s = 0; s < S.nsdf; ++s) {
             if (S.sdf[s].kind != KIND) continue;
-            float th = ext_march<KIND>(S.sdf[s], o, d, t, S.sdf_steps_max, steps);
+            float th = ext_march<KIND>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted);
             if (th > 0.0f && th < t) {
                 t = th;
                 ref = (REF_SDF << kRefShift) | (uint32_t)s;
@@ -1031,6 +1033,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
         if (upd) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
 }
 
 template <bool MULTI>
@@ -1401,7 +1404,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B,
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
-    unsigned steps = 0, visits = 0;
+    unsigned steps = 0, visits = 0, exhausted = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
@@ -1410,10 +1413,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B,
         uint32_t ref = kMiss;
         bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack, visits);
         for (int s = 0; !blocked && s < S.nsdf; ++s)
-            blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
+            blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted) > 0.0f;
         if (!blocked) add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
     wave_add_u64(stat_slot(Wv.stats, 12), visits);
 }
 
@@ -1426,7 +1430,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
-    unsigned steps = 0;
+    unsigned steps = 0, exhausted = 0;
     int i = -1, sp = 0, node = -1, best = -1, path = 0;
     unsigned visits = 0;
     bool blocked = false;
@@ -1474,12 +1478,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
                 blocked = tp > kEps && tp < t;
             }
             for (int s = 0; !blocked && s < S.nsdf; ++s)
-                blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps) > 0.0f;
+                blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted) > 0.0f;
             if (!blocked) add_xyz(Wv, path, ld3(Wv.s_xyz, i));
             i = -1;
         }
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
     wave_add_u64(stat_slot(Wv.stats, 12), visits);
 }
 
@@ -2099,6 +2104,9 @@ static ExtFrame frame_of(const pt_push_constants& pc) {
 }
 
 // Compute units of the context's device (grid sizing), after ensure_ext_wave.
+// Blocks of kBlk lanes per compute unit in the grid of the queue kernels.
+static int blocks_per_cu(const pt_ctx* ctx) { return ctx->cfg.blocks_per_cu > 0 ? ctx->cfg.blocks_per_cu : 8; }
+
 static int ctx_cus(pt_ctx* ctx) {
     ExtCtxState* cs = static_cast<ExtCtxState*>(ctx->ext_state);
     if (cs->cu == 0) {
@@ -2134,7 +2142,7 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
     ExtWave& Wv = static_cast<ExtCtxState*>(ctx->ext_state)->wave;
 
     ExtFrame F = frame_of(*pc);
-    const int grid_cap = ctx_cus(ctx) * 8;
+    const int grid_cap = ctx_cus(ctx) * blocks_per_cu(ctx);
     ExtPlan plan;
     plan.S0 = H.dev;
     plan.any_sdf = H.dev.nsdf > 0;
@@ -2286,7 +2294,8 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
     plan.F0 = frames[0];
     // a fixed grid (the wave's sizes come from the header), so one captured
     // graph per launch shape replays for every batch of that shape
-    const int grid = (int)std::min<int64_t>(((int64_t)cap + kBlk - 1) / kBlk, (int64_t)ctx_cus(ctx) * 8);
+    const int grid = (int)std::min<int64_t>(((int64_t)cap + kBlk - 1) / kBlk,
+                                            (int64_t)ctx_cus(ctx) * blocks_per_cu(ctx));
     auto launch_wave = [&]() {
         launch(ctx, "k_ext_counts", k_ext_counts_batch, dim3(1), dim3(64), Wv.counts, d_hdr);
         launch(ctx, "k_ext_raygen", k_ext_raygen_batch, dim3(grid), dim3(kBlk), plan.B, d_segs, d_hdr, Wv);
