@@ -338,13 +338,15 @@ pt_status pt_server_submit(pt_server* server, const pt_request_desc* req, double
  * *n_stepped. Returns PT_OK, PT_SERVE_IDLE when nothing is admitted, or an error. */
 pt_status pt_server_step(pt_server* server, pt_request_status* stepped, int32_t cap, int32_t* n_stepped);
 pt_status pt_server_status(pt_server* server, int32_t id, pt_request_status* out);
-/* The request's mean XYZ image as RGBA32F texels in the units of pt_accum. */
+/* The request's image as RGBA32F texels in the units of pt_accum: its mean XYZ,
+ * or its denoised image when the status says `denoised`. */
 pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t n_floats);
 /* Attaches a denoiser (of the server's context; NULL detaches) for requests
  * with `denoise` set. Such a request also keeps the sums of its odd-index
- * samples and the first-hit features of a 4-sample camera-ray pass (84 more
- * bytes per pixel). After min_spp samples, while its plain estimate misses the
- * target, the error of its denoised image is estimated by denoising each
+ * samples, the first-hit features of a 4-sample camera-ray pass and the
+ * denoised image (80 more bytes per pixel). After min_spp samples, while its
+ * plain estimate misses the target, the error of its denoised image is
+ * estimated by denoising each
  * half-sample image (even / odd sample indices) and comparing it with the
  * other, independent half, whose variance is subtracted: a conservative
  * estimate, since the full image has twice the samples of either half. The
