@@ -68,6 +68,11 @@ def parse_args():
     ap.add_argument("--serve-target", type=float, default=0.01, help="relative RMSE at which a request is done")
     ap.add_argument("--serve-load", type=float, default=0.85,
                     help="utilisation of the one-request-at-a-time tracer that sets the Poisson arrival rate")
+    ap.add_argument("--serve-check", type=int, default=0, metavar="N",
+                    help="accuracy instead: the requests among 0..N-1 of at most 512x512, served with the "
+                         "denoiser, against independent plain renders of 4096-8192 samples per pixel")
+    ap.add_argument("--serve-graphs", type=int, default=1,
+                    help="1: batched waves replayed from captured hipGraphs; 0: kernels launched one by one")
     ap.add_argument("--serve-denoise", action="store_true",
                     help="the SRPT server may finish requests with the shipped denoiser's image")
     ap.add_argument("--serve-speedup", type=str, default="1",
@@ -160,7 +165,7 @@ def serve_bench(args):
     from pathtracer_amd import scenes
 
     reqs = [scenes.serving_request(i) for i in range(args.serve)]
-    ctx = pt.Context(0)
+    ctx = pt.Context(0, graph_mode=0 if args.serve_graphs else 1)
     t_start = time.perf_counter()
 
     def submit(srv, r, t):
@@ -377,6 +382,74 @@ def denoiser_bench(args):
     ctx.close()
 
 
+def serve_check(args):
+    """Actual relative RMSE of served images. The requests among 0..N-1 of at
+    most 512 x 512 are served together (SRPT, shipped denoiser, target
+    --serve-target). Each one's reference is a plain render of the same scene
+    with another render seed (independent samples), 8192 samples per pixel at
+    256 x 256 and 4096 at 512 x 512, served plain to its budget; its own
+    variance of the mean is taken off the squared difference:
+    actual^2 = (|img - ref|^2 - sum var(ref)) / |ref|^2. Prints one JSON line."""
+    import numpy as np
+    import pathtracer_amd as pt
+    from pathtracer_amd import denoise_train, scenes
+
+    reqs = [r for r in (scenes.serving_request(i) for i in range(args.serve_check))
+            if r["width"] * r["height"] <= 512 * 512]
+    ctx = pt.Context(0)
+    t0 = time.perf_counter()
+
+    def run(srv, items):
+        ids = {srv.submit(t, r["width"], r["height"], spp, r["depth"], target, denoise=dn): r["id"]
+               for r, t, spp, target, dn in items}
+        out = {}
+        while len(out) < len(ids):
+            sts = srv.step()
+            if sts is None:
+                break
+            for st in sts:
+                if st.state in (pt.REQ_DONE, pt.REQ_FAILED):
+                    img = srv.read(st.id)[..., :3].astype(np.float64)
+                    out[ids[st.id]] = (st.as_dict(), img)
+                    srv.release(st.id)
+        return out
+
+    with pt.Denoiser(ctx, 1, 512, 512) as dn, pt.Server(ctx) as srv:
+        denoise_train.load_weights(dn)
+        srv.set_denoiser(dn)
+        served = run(srv, [(r, r["text"], r["spp"], args.serve_target, True) for r in reqs])
+    print(f"served {len(served)} requests ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    rows = []
+    for r in reqs:
+        d = json.loads(r["text"])
+        d["render"]["seed"] = int(d["render"].get("seed", 0)) + 7919
+        ref_spp = 8192 if r["width"] * r["height"] <= 256 * 256 else 4096
+        with pt.Server(ctx) as srv:
+            (st_ref, ref), = run(srv, [(r, json.dumps(d), ref_spp, 1e-9, False)]).values()
+        st, img = served[r["id"]]
+        n2 = float(np.sum(ref ** 2))
+        var_ref = st_ref["rel_plain"] ** 2 * n2
+        actual = float(np.sqrt(max(float(np.sum((img - ref) ** 2)) - var_ref, 0.0) / n2))
+        rows.append({"id": r["id"], "kind": r["kind"], "res": f"{r['width']}x{r['height']}", "spp": st["spp"],
+                     "spp_max": r["spp"], "denoised": st["denoised"], "converged": st["converged"],
+                     "estimate": st["rel_error"], "actual": actual, "ref_rel": st_ref["rel_plain"]})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    conv = [x for x in rows if x["converged"]]
+    line = {
+        "metric": "actual relative RMSE of served images against independent converged renders",
+        "target_rel": args.serve_target, "requests": len(rows),
+        "converged": len(conv),
+        "converged_actual_within_target": sum(x["actual"] <= args.serve_target for x in conv),
+        "converged_actual_within_1.1x_target": sum(x["actual"] <= 1.1 * args.serve_target for x in conv),
+        "converged_actual_max": max((x["actual"] for x in conv), default=None),
+        "median_actual_over_estimate": float(np.median([x["actual"] / x["estimate"] for x in rows if x["estimate"] > 0])),
+        "rows": rows,
+        "bench_seconds": time.perf_counter() - t0,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -385,6 +458,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe, args.steps)
+        return
+    if args.serve_check:
+        serve_check(args)
         return
     if args.serve:
         serve_bench(args)

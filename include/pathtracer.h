@@ -346,10 +346,13 @@ pt_status pt_server_read(pt_server* server, int32_t id, float* host_rgba, size_t
  * samples, the first-hit features of a 4-sample camera-ray pass and the
  * denoised image (80 more bytes per pixel). After min_spp samples, while its
  * plain estimate misses the target, the error of its denoised image is
- * estimated by denoising each
- * half-sample image (even / odd sample indices) and comparing it with the
- * other, independent half, whose variance is subtracted: a conservative
- * estimate, since the full image has twice the samples of either half. The
+ * estimated from dual buffers: each half-sample image (even / odd sample
+ * indices) is denoised and compared with the other, independent half, whose
+ * variance is subtracted (the squared error of a denoised half, bias
+ * included); half the variance of a denoised half, measured from the two
+ * denoised halves, is taken off for the full sample count (phased in
+ * linearly up to 64 samples per pixel, below which the network's output
+ * variance falls more slowly than 1/samples). The
  * request finishes denoised when that estimate meets the target; at its
  * sample budget it returns the image with the lower estimate, so the plain
  * mean is the fallback. Resolutions the network does not take (width or

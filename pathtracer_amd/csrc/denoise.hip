@@ -66,8 +66,13 @@ union Frag {
 // Conv forward (also backward-data). X1 (C1 channels) and X2 (C2 channels, may
 // be null) form the input [N, H, W, C1 + C2]; Wt is [9][Cout][C1 + C2] BF16;
 // output channels [0, Co1) go to Y1 (Co1 channels) and [Co1, Cout) to Y2.
-// One wave computes 32 pixels x 32 * NC output channels; a block holds 4 waves
-// along pixels. `flip` reads tap 8 - t (backward-data).
+// A block of 4 waves computes 4 * 32 * R pixels x 32 * NC output channels: per
+// tap it stages the tap's weights for its channel tile in LDS (rows padded by
+// 8 channels against bank conflicts), then each wave runs R x NC MFMA tiles of
+// 32 pixels x 32 channels over the input channels, 16 at a time; each B
+// fragment read from LDS feeds R MFMAs, each A fragment (one 16-byte load of 8
+// channels of one pixel per lane) feeds NC. `flip` reads tap 8 - t
+// (backward-data).
 struct ConvArgs {
     const u16* X1;
     const u16* X2;
@@ -82,43 +87,76 @@ struct ConvArgs {
     int flip;
 };
 
-template <int NC>
+// Largest input channel count a k_conv<NC> block stages (its LDS size).
+__host__ __device__ constexpr int conv_max_cin(int nc) { return nc == 4 ? 128 : 192; }
+
+template <int NC, int R>
 __global__ void __launch_bounds__(kThreads) k_conv(ConvArgs a) {
+    constexpr int kLdw = conv_max_cin(NC) + 8;
+    __shared__ __align__(16) u16 s_w[32 * NC * kLdw];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int P = a.N * a.H * a.W;
-    const int pix0 = (blockIdx.x * 4 + wave) * 32;
+    const int pix0 = (blockIdx.x * 4 + wave) * 32 * R;
     const int co0 = blockIdx.y * 32 * NC;
     const int Cin = a.C1 + a.C2;
-    // this lane's A row (pixel)
-    const int pm = pix0 + (lane & 31);
-    const bool pin = pm < P;
-    const int pn = pin ? pm / (a.H * a.W) : 0, prem = pin ? pm % (a.H * a.W) : 0;
-    const int py = prem / a.W, px = prem % a.W;
-    const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-chunk
-    f16v acc[NC];
+    const int ldw = Cin + 8;
+    // this lane's R A rows (pixels pix0 + lane % 32, 32 further, ...)
+    int pn[R], py[R], px[R];
+    bool pin[R];
 #pragma unroll
-    for (int j = 0; j < NC; ++j) acc[j] = f16v{};
+    for (int r = 0; r < R; ++r) {
+        const int pm = pix0 + r * 32 + (lane & 31);
+        pin[r] = pm < P;
+        pn[r] = pin[r] ? pm / (a.H * a.W) : 0;
+        const int prem = pin[r] ? pm % (a.H * a.W) : 0;
+        py[r] = prem / a.W;
+        px[r] = prem % a.W;
+    }
+    const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-chunk
+    f16v acc[R][NC];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc[r][j] = f16v{};
+    const int nvec = 32 * NC * (Cin >> 3);  // 16-byte vectors of the tap's weight tile
     for (int t = 0; t < 9; ++t) {
         const int tt = a.flip ? 8 - t : t;
+        __syncthreads();  // every wave is done with the previous tap's tile
+        for (int i = threadIdx.x; i < nvec; i += kThreads) {
+            const int row = i / (Cin >> 3), cv = (i % (Cin >> 3)) * 8;
+            const int co = co0 + row;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (co < a.Cout) v = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + cv);
+            *reinterpret_cast<uint4*>(&s_w[row * ldw + cv]) = v;
+        }
+        __syncthreads();
         const int dy = t / 3 - 1, dx = t % 3 - 1;
-        const int y = py + dy, x = px + dx;
-        const bool in = pin && y >= 0 && y < a.H && x >= 0 && x < a.W;
-        const size_t q = ((size_t)pn * a.H + (in ? y : 0)) * a.W + (in ? x : 0);
+        size_t q[R];
+        bool in[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int y = py[r] + dy, x = px[r] + dx;
+            in[r] = pin[r] && y >= 0 && y < a.H && x >= 0 && x < a.W;
+            q[r] = ((size_t)pn[r] * a.H + (in[r] ? y : 0)) * a.W + (in[r] ? x : 0);
+        }
         for (int c0 = 0; c0 < Cin; c0 += 16) {
-            Frag A;
-            A.u = make_uint4(0, 0, 0, 0);
-            if (in) {
-                const int c = c0 + kq;
-                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
-                A.u = *reinterpret_cast<const uint4*>(src);
+            const int c = c0 + kq;
+            Frag A[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                A[r].u = make_uint4(0, 0, 0, 0);
+                if (in[r]) {
+                    const u16* src = c < a.C1 ? a.X1 + q[r] * a.C1 + c : a.X2 + q[r] * a.C2 + (c - a.C1);
+                    A[r].u = *reinterpret_cast<const uint4*>(src);
+                }
             }
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
-                const int co = co0 + j * 32 + (lane & 31);
                 Frag B;
-                B.u = make_uint4(0, 0, 0, 0);
-                if (co < a.Cout) B.u = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + c0 + kq);
-                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A.v, B.v, acc[j], 0, 0, 0);
+                B.u = *reinterpret_cast<const uint4*>(&s_w[(j * 32 + (lane & 31)) * ldw + c]);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, B.v, acc[r][j], 0, 0, 0);
             }
         }
     }
@@ -128,13 +166,16 @@ __global__ void __launch_bounds__(kThreads) k_conv(ConvArgs a) {
         if (co >= a.Cout) continue;
         const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int p = pix0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-            if (p >= P) continue;
-            float r = acc[j][v] + b;
-            if (a.relu) r = fmaxf(r, 0.0f);
-            if (co < a.Co1) a.Y1[(size_t)p * a.Co1 + co] = f2bf(r);
-            else a.Y2[(size_t)p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(r);
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int p = pix0 + r * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (p >= P) continue;
+                float o = acc[r][j][v] + b;
+                if (a.relu) o = fmaxf(o, 0.0f);
+                if (co < a.Co1) a.Y1[(size_t)p * a.Co1 + co] = f2bf(o);
+                else a.Y2[(size_t)p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(o);
+            }
         }
     }
 }
@@ -379,12 +420,14 @@ __global__ void k_dn_assemble(ptk::DnRequest q, int which, float inv_scale, floa
 // Against the other half (other: 1 even, 2 odd) of the same request, per block
 // then one double atomic per block into line blockIdx.x % 64 (stride 32):
 // slot 0 sum (den - other)^2, slot 1 sum of the other half's variance of the
-// mean, slot 2 sum of the full mean^2, all over pixels and XYZ in texel units
-// times inv_scale.
-__global__ void k_dn_cross(ptk::DnRequest q, const float* den, int other, float inv_scale, double* lines) {
-    __shared__ double red[3][4];
+// mean, slot 2 sum of the full mean^2, slot 3 sum (den - prev)^2 when prev
+// (the other half's denoised image) is given; all over pixels and XYZ in
+// texel units times inv_scale.
+__global__ void k_dn_cross(ptk::DnRequest q, const float* den, const float* prev, int other, float inv_scale,
+                           double* lines) {
+    __shared__ double red[4][4];
     const int P = q.W * q.H;
-    double e = 0.0, v = 0.0, m = 0.0;
+    double e = 0.0, v = 0.0, m = 0.0, dd = 0.0;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
         const int n = q.nsamp[p], no = q.n_odd[p];
         const int cnt = other == 1 ? n - no : no;
@@ -399,17 +442,22 @@ __global__ void k_dn_cross(ptk::DnRequest q, const float* den, int other, float 
             v += var / cnt * k * k;
             const double mean = s / n * k;
             m += mean * mean;
+            if (prev) {
+                const double g = (double)den[(size_t)p * 3 + c] - (double)prev[(size_t)p * 3 + c];
+                dd += g * g;
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
         e += __shfl_down(e, off);
         v += __shfl_down(v, off);
         m += __shfl_down(m, off);
+        dd += __shfl_down(dd, off);
     }
     const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { red[0][wv] = e; red[1][wv] = v; red[2][wv] = m; }
+    if ((threadIdx.x & 63) == 0) { red[0][wv] = e; red[1][wv] = v; red[2][wv] = m; red[3][wv] = dd; }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
         double t = 0.0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += red[threadIdx.x][i];
         atomicAdd(lines + (size_t)(blockIdx.x % 64) * 32 + threadIdx.x, t);
@@ -459,6 +507,9 @@ int dn_fail(pt_denoiser* d, int st, const std::string& msg) {
     return st;
 }
 
+// Output channels per block of k_conv: 32 * conv_nc(Cout).
+int conv_nc(int cout) { return cout >= 128 ? 4 : (cout >= 64 ? 2 : 1); }
+
 void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, int N, int H, int W, bool bwd_data,
           u16* y2 = nullptr, int co1 = 0) {
     ConvArgs a{};
@@ -477,11 +528,13 @@ void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, 
         a.relu = 0; a.flip = 1;
     }
     const int P = N * H * W;
-    const int nc = a.Cout >= 128 ? 4 : (a.Cout >= 64 ? 2 : 1);
-    dim3 grid((P + 127) / 128, (a.Cout + 32 * nc - 1) / (32 * nc));
-    if (nc == 4) ptk::launch(d->ctx, "k_dn_conv", k_conv<4>, grid, dim3(kThreads), a);
-    else if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv<2>, grid, dim3(kThreads), a);
-    else ptk::launch(d->ctx, "k_dn_conv", k_conv<1>, grid, dim3(kThreads), a);
+    const int nc = conv_nc(a.Cout);
+    // 128 accumulators per lane at NC = 4 with one pixel row; two rows below
+    const int rows = nc == 4 ? 1 : 2;
+    dim3 grid((P + 128 * rows - 1) / (128 * rows), (a.Cout + 32 * nc - 1) / (32 * nc));
+    if (nc == 4) ptk::launch(d->ctx, "k_dn_conv", k_conv<4, 1>, grid, dim3(kThreads), a);
+    else if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv<2, 2>, grid, dim3(kThreads), a);
+    else ptk::launch(d->ctx, "k_dn_conv", k_conv<1, 2>, grid, dim3(kThreads), a);
 }
 
 void wgrad(pt_denoiser* d, const Layer& l, const u16* dz, const u16* x1, const u16* x2, int N, int H, int W) {
@@ -611,7 +664,7 @@ void dn_half(pt_denoiser* d, const DnRequest& q, int which, float inv_scale) {
 }
 }  // namespace
 
-int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, double* d_lines, double* rel) {
+int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, int spp, double* d_lines, double* rel) {
     if (!dn_fits(d, q.W, q.H)) return dn_fail(d, PT_ERR_INVALID_ARG, "denoiser: request outside the network's limits");
     pt_ctx* ctx = d->ctx;
     const size_t P = (size_t)q.W * q.H;
@@ -620,19 +673,32 @@ int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, double*
     if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
     const dim3 grid((unsigned)std::min<size_t>((P + 255) / 256, 2048));
     dn_half(d, q, 1, inv);  // D(even half) against the odd half
-    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, 2, inv, d_lines);
-    dn_half(d, q, 2, inv);  // D(odd half) against the even half
-    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, 1, inv, d_lines + 64 * 32);
+    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)nullptr, 2,
+                inv, d_lines);
+    // keep D(A) (the training-target buffer is free during inference)
+    e = hipMemcpyAsync(d->tgt, d->den, P * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
+    dn_half(d, q, 2, inv);  // D(odd half) against the even half, and against D(even half)
+    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)d->tgt, 1,
+                inv, d_lines + 64 * 32);
     std::vector<double> h((size_t)2 * 64 * 32);
     e = hipMemcpyAsync(h.data(), d_lines, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
-    double s[2][3] = {};
+    double s[2][4] = {};
     for (int k = 0; k < 2; ++k)
         for (int l = 0; l < 64; ++l)
-            for (int j = 0; j < 3; ++j) s[k][j] += h[(size_t)k * 64 * 32 + (size_t)l * 32 + j];
-    // E|D(A) - B|^2 = |D(A) - mu|^2 + E|B - mu|^2 for independent halves A, B
-    const double err2 = 0.5 * ((s[0][0] - s[0][1]) + (s[1][0] - s[1][1]));
+            for (int j = 0; j < 4; ++j) s[k][j] += h[(size_t)k * 64 * 32 + (size_t)l * 32 + j];
+    // E|D(A) - B|^2 = |D(A) - mu|^2 + E|B - mu|^2 for independent halves A, B:
+    // the squared error of a denoised half image, bias included
+    const double err2_half = 0.5 * ((s[0][0] - s[0][1]) + (s[1][0] - s[1][1]));
+    // E|D(A) - D(B)|^2 = 2 var(D(half)). With the bias independent of the
+    // sample count and the variance falling as 1/n, the full image's squared
+    // error is err2_half - var(D(half)) / 2. At few samples the network's
+    // output variance falls more slowly than 1/n (it smooths low-sample
+    // inputs harder), so that credit is phased in linearly up to 64 samples.
+    const double credit = std::min(1.0, spp / 64.0);
+    const double err2 = err2_half - credit * 0.25 * s[1][3];
     const double m = s[0][2];
     *rel = m > 0.0 ? std::sqrt(std::max(err2, 0.0) / m) : INFINITY;
     return PT_OK;
@@ -673,6 +739,13 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
         l.b_off = off;
         off += (size_t)l.cout;
         d->L.push_back(l);
+        // k_conv stages one tap's weight tile in LDS: the input channels of both
+        // passes (forward, backward-data) must fit it and be multiples of 16
+        const int cin = l.cin1 + l.cin2;
+        if (cin % 16 || l.cout % 16 || cin > conv_max_cin(conv_nc(l.cout)) || l.cout > conv_max_cin(conv_nc(cin))) {
+            delete d;
+            return PT_ERR_INVALID_ARG;
+        }
     }
     d->nparam = off;
     const size_t P = (size_t)max_batch * max_height * max_width;

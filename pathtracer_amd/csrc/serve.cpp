@@ -20,8 +20,8 @@
 //     over pixels and XYZ channels;
 //   - a request that asks for denoising also keeps the sums of its odd-index
 //     samples and the first-hit features of a 4-sample camera-ray pass; its
-//     denoised image's error is estimated by denoising each half-sample image
-//     and comparing it with the other half (ptk::dn_request_estimate). It
+//     denoised image's error is estimated from the two half-sample images,
+//     each denoised and compared with the other (ptk::dn_request_estimate). It
 //     finishes denoised when that estimate reaches the target before the
 //     plain one does; at its sample budget it returns whichever image has the
 //     lower estimate (the plain mean is the fallback).
@@ -221,7 +221,7 @@ int denoise_step(pt_server* s, Request& r, bool last) {
     // radiance of order one for the network: the RMS of the mean image
     const float scale = (float)(q.exposure * std::sqrt(r.mean2 / (3.0 * r.P)));
     double rel = INFINITY;
-    int st = ptk::dn_request_estimate(s->dn, q, scale, s->d_rel, &rel);
+    int st = ptk::dn_request_estimate(s->dn, q, scale, r.spp, s->d_rel, &rel);
     if (st != PT_OK) return fail(s, st, std::string("denoiser: ") + pt_dn_last_error(s->dn));
     r.rel_dn = rel;
     r.dn_spp = r.spp;

@@ -211,10 +211,13 @@ int dn_fits(const pt_denoiser* d, int W, int H);
 pt_ctx* dn_ctx(const pt_denoiser* d);
 // Dual-buffer estimate of the relative RMSE of the request's denoised image:
 // each half-sample image is denoised and compared with the other, independent
-// half, whose own variance is subtracted; the mean over the two halves, over
-// sqrt(sum of mean^2). Inputs are divided by `scale` (an RMS of the image).
-// d_lines: device scratch of 2 x 64 x 32 doubles. Synchronous.
-int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, double* d_lines, double* rel);
+// half, whose own variance is subtracted (the squared error of a denoised
+// half, bias included); half the variance of a denoised half, from the
+// difference of the two denoised halves, is then taken off for the full
+// sample count (phased in up to 64 samples per pixel); over sqrt(sum of
+// mean^2). Inputs are divided by `scale` (an RMS of the image). d_lines:
+// device scratch of 2 x 64 x 32 doubles. Synchronous.
+int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, int spp, double* d_lines, double* rel);
 // The denoised image of all samples into d_out (3 floats per pixel, texel units).
 int dn_request_denoise(pt_denoiser* d, const DnRequest& q, float scale, float* d_out);
 // All items' paths in one wave (their total must fit the context's wave
