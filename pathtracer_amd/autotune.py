@@ -15,7 +15,10 @@ returns the fastest configuration:
                          first, a speed knob second)
 
 The search is coordinate descent in that order, each parameter tried with the
-others at their best so far. Every candidate renders the same samples, so
+others at their best so far. A configuration is measured once: the incumbent
+is not re-timed for every coordinate, so timing noise cannot make the same
+configuration both win and lose, and the choice is always the fastest
+configuration measured. Every candidate renders the same samples, so
 the image does not depend on the choice except through sdf_max_steps.
 Measurement is wall time around pt_dispatch + pt_sync after one warm-up
 dispatch, median of `repeats`.
@@ -108,9 +111,14 @@ def autotune(text: str, width: int, height: int, depth: int, spp: int = 0, repea
         spp = max(1, (1 << 22) // (width * height))
     has_sdf = '"sdf"' in text and '"kind"' in text
     res = Result(Choice())
+    seen: dict[str, Trial] = {}
 
     def run(c: Choice) -> Trial:
+        key = json.dumps(asdict(c), sort_keys=True)
+        if key in seen:
+            return seen[key]
         t = measure(c, text, width, height, depth, spp, repeats)
+        seen[key] = t
         res.trials.append(t)
         if log:
             log(f"{json.dumps(t.params)} -> {t.segments_per_s / 1e9:.3f} G segments/s"

@@ -26,7 +26,11 @@ def test_descent_finds_the_separable_optimum(monkeypatch):
     for k, v in opt.items():
         assert getattr(r.best, k) == v, k
     assert r.best.sdf_max_steps == 256  # no SDF in the scene: budget untouched
-    assert len(calls) == len(A.WIDTHS) + len(A.LEAVES) + 2 + len(A.WAVES) + len(A.BLOCKS_PER_CU)
+    # each configuration is measured once: the incumbent of every coordinate
+    # after the first is not timed again
+    assert len({tuple(sorted(c.items())) for c in calls}) == len(calls)
+    assert len(calls) == len(A.WIDTHS) + (len(A.LEAVES) - 1) + 1 + (len(A.WAVES) - 1) + (len(A.BLOCKS_PER_CU) - 1)
+    assert max(r.trials, key=lambda t: t.segments_per_s).params == A.asdict(r.best)
 
 
 def test_sdf_budget_is_the_smallest_within_tolerance(monkeypatch):
