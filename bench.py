@@ -1,14 +1,25 @@
 """Benchmark of the extended spectral wavefront tracer on one MI355X.
 
-Workload: Cornell box plus 4096 random spheres (60 % Lambert, 25 % dispersive
-dielectric, 15 % conductor), 1920x1080, path depth 12, 4 hero wavelengths.
+Workloads (--workload):
+  spheres (default)  Cornell box plus 4096 random spheres (60 % Lambert, 25 %
+                     dispersive dielectric, 15 % conductor), 1920x1080, path
+                     depth 12, 4 hero wavelengths, 2 spp per step.
+  cornell            the Cornell box alone (5 quads, a Lambert and a Cauchy
+                     dielectric sphere, 6500 K quad light), 512x512, depth 8,
+                     one wavelength per path, 16 spp per step: 4 steps are the
+                     whole 64-spp render.
+  sdf                power-8 mandelbulb (8 iterations) and level-4 menger
+                     sponge over a ground plane under a uniform white sky,
+                     sphere tracing of at most 256 steps to 1e-4, 1920x1080,
+                     depth 8, 4 hero wavelengths, 2 spp per step.
 A step is one dispatch tracing `--spp` samples of every pixel (one wave of
 paths); the timed region is exactly `--steps` dispatches bracketed by stream
 synchronisation, with the scene, BVH and path buffers resident in HBM.
 
 Prints one JSON line: metric, value (spectral path-segments/s), roofline of
-the dominant kernel (algorithmic bytes / its average HIP-event duration), and
-the CPU baseline (the oracle timed on this host on a bounded sample).
+the dominant kernel (algorithmic bytes, or for a sphere-tracing kernel FP32
+operations, over its average HIP-event duration), and the CPU baseline (the
+oracle timed on this host on a bounded sample).
 """
 from __future__ import annotations
 
@@ -24,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "spectral path-segments/s; p50/p99 time-to-RMSE≤1% per render request"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 # Algorithmic HBM bytes per unit of work of each kernel: the SoA records the
 # kernel must read and write once (path state, hit, queue entries), excluding
@@ -45,16 +57,60 @@ BYTES_PER_UNIT = {
     "k_ext_shade_lambert": [(160, "lambert_hits")],
 }
 
+# Sphere-tracing kernels are bound by FP32 VALU work, not HBM. Their algorithmic
+# FP32 operations are counted on the device (pt_counters.march_flops_*,
+# shadow_march_flops): per distance-estimator evaluation, the operations the
+# estimator's source performs (one per add, multiply, divide, min, max, floor,
+# sqrt, rsqrt, log or sin, two per fused multiply-add): mandelbulb 10 + 65 per
+# iteration executed, menger 16 + 33 per level, blob 65, terrain 93; plus 10 per
+# march step for the point, its offset and the step update. The shadow kernel
+# of an SDF scene also traverses the analytic scene; only its marching counts.
+FLOPS_COUNTER = {
+    "k_ext_march_mandelbulb": "march_flops_mandelbulb",
+    "k_ext_march_menger": "march_flops_menger",
+    "k_ext_march_blob": "march_flops_blob",
+    "k_ext_march_terrain": "march_flops_terrain",
+    "k_ext_shadow": "shadow_march_flops",
+}
+
+
+def workload(args):
+    """(scene JSON, width, height, depth, spp per step, description) of --workload."""
+    from pathtracer_amd import scenes
+    if args.workload == "cornell":
+        text = scenes.cornell(wavelengths=1, seed=0)
+        dflt = (512, 512, 8, 16)
+        what = ("Cornell box (5 quads with 3-Gaussian spectra, Lambert sphere, Cauchy dielectric sphere "
+                "A=1.5 B=0.01, 6500 K quad light), {w}x{h}, depth {d}, 1 wavelength per path in 380-780 nm")
+    elif args.workload == "sdf":
+        text = scenes.sdf_scene(seed=0)
+        dflt = (1920, 1080, 8, 2)
+        what = ("mandelbulb (power 8, 8 iterations, bailout 16) + level-4 menger sponge over a ground plane, "
+                "uniform white environment, sphere tracing <= 256 steps to 1e-4, {w}x{h}, depth {d}, "
+                "4 hero wavelengths")
+    else:
+        text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
+        dflt = (1920, 1080, 12, 2)
+        what = ("Cornell box + " + str(args.spheres) + " random spheres (60% Lambert / 25% dielectric / "
+                "15% conductor), {w}x{h}, depth {d}, 4 hero wavelengths")
+    w = args.width or dflt[0]
+    h = args.height or dflt[1]
+    d = args.depth or dflt[2]
+    spp = args.spp or dflt[3]
+    return text, w, h, d, spp, what.format(w=w, h=h, d=d)
+
 
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--spp", type=int, default=2, help="samples per pixel per step")
-    ap.add_argument("--width", type=int, default=1920)
-    ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--workload", choices=("spheres", "cornell", "sdf"), default="spheres",
+                    help="scene of the tracer bench (module docstring)")
+    ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (default: the workload's)")
+    ap.add_argument("--width", type=int, default=0)
+    ap.add_argument("--height", type=int, default=0)
+    ap.add_argument("--depth", type=int, default=0)
     ap.add_argument("--spheres", type=int, default=4096)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--bvh-leaf", type=int, default=0)
@@ -114,29 +170,35 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def cpu_baseline(args, scene_text: str):
+def cpu_baseline(args, scene_text: str, W: int, H: int, depth: int, full_spp: int = 0):
     """The FP64 oracle tracer (CPU restatement with its own BVH) on a bounded
     sample of the same workload: the full frame at a number of samples per
     pixel chosen to take about --cpu-seconds, on this process's host cores
-    (at most 16, the share of one GPU)."""
+    (at most 16, the share of one GPU). With full_spp, the whole render of
+    that many samples per pixel when it fits in about 3 x --cpu-seconds."""
     try:
         from oracle import ptext
     except Exception as e:  # oracle not built
         return {"value": None, "unit": "segments/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    band = max(1, args.height // 16)
-    probe = ptext.render(scene_text, args.width, args.height, spp=1, depth=args.depth, threads=threads,
-                         rows=(0, band))
-    frame_s = probe["seconds"] * args.height / band
+    # time of one sample per pixel over the frame, from 8 bands spread over its height
+    band = max(1, H // 64)
+    probe_s = 0.0
+    for k in range(8):
+        r0 = min(H - band, (2 * k + 1) * H // 16)
+        probe_s += ptext.render(scene_text, W, H, spp=1, depth=depth, threads=threads, rows=(r0, r0 + band))["seconds"]
+    frame_s = probe_s * H / (8 * band)
     spp = max(1, int(round(args.cpu_seconds / max(frame_s, 1e-9))))
-    run = ptext.render(scene_text, args.width, args.height, spp=spp, depth=args.depth, threads=threads)
+    if full_spp and frame_s * full_spp <= 3.0 * args.cpu_seconds:
+        spp = full_spp
+    run = ptext.render(scene_text, W, H, spp=spp, depth=depth, threads=threads)
     return {
         "value": run["segments"] / run["seconds"],
         "unit": "segments/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"FP64 CPU oracle with its own BVH, the full {args.width}x{args.height} frame x {spp} spp, "
-                  f"depth {args.depth}: {run['segments']} segments in {run['seconds']:.2f} s",
+        "sample": f"FP64 CPU oracle with its own BVH, the full {W}x{H} frame x {spp} spp, "
+                  f"depth {depth}: {run['segments']} segments in {run['seconds']:.2f} s",
     }
 
 
@@ -313,7 +375,7 @@ def denoiser_bench(args):
     import pathtracer_amd as pt
     from pathtracer_amd import denoise_train as T, scenes
 
-    W, H = (args.width + 3) // 4 * 4, (args.height + 3) // 4 * 4
+    W, H = ((args.width or 1920) + 3) // 4 * 4, ((args.height or 1080) + 3) // 4 * 4
     ctx = pt.Context(0)
     sc = ctx.scene_from_json(scenes.sdf_scene(seed=1))
     feat = T.render_features(ctx, sc, W, H, 4, 8)
@@ -468,15 +530,17 @@ def main():
     if args.denoiser:
         denoiser_bench(args)
         return
-    import pathtracer_amd as pt
-    from pathtracer_amd import scenes
+    trace_bench(args)
 
-    text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
-    W, H = args.width, args.height
+
+def trace_bench(args):
+    import pathtracer_amd as pt
+
+    text, W, H, depth, spp, what = workload(args)
     ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal,
                      bvh_leaf=args.bvh_leaf)
     scene = ctx.scene_from_json(text)
-    r = pt.Renderer(ctx, scene, W, H, spf=args.spp, path_length=args.depth, mode=pt.MODE_EXT)
+    r = pt.Renderer(ctx, scene, W, H, spf=spp, path_length=depth, mode=pt.MODE_EXT)
     for _ in range(args.warmup):
         r.render_frame()
     ctx.sync()
@@ -513,22 +577,52 @@ def main():
         dom = max(prof.items(), key=lambda kv: kv[1][1])
         name, (launches, total_ms) = dom
         terms = BYTES_PER_UNIT.get(name)
-        if terms and launches > 0:
+        fl = FLOPS_COUNTER.get(name) if args.workload == "sdf" else None
+        if fl and launches > 0:
+            flops = pcnt.get(fl, 0)
+            achieved = flops / (total_ms * 1e-3) / 1e12
+            roofline = {
+                "bound": "valu",
+                "achieved": round(achieved, 3),
+                "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                "traffic": traffic_from_profiles(name),
+                "kernel": name,
+                "avg_launch_ms": round(total_ms / launches, 4),
+                "algorithmic_flops_per_launch": round(flops / launches),
+            }
+        elif terms and launches > 0:
             total_bytes = sum(bpu * pcnt.get(unit, 0) for bpu, unit in terms)
             achieved = total_bytes / (total_ms * 1e-3) / 1e9
-            traffic = traffic_from_profiles(name)
             roofline = {
                 "bound": "hbm",
                 "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic,
+                "traffic": traffic_from_profiles(name),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_bytes_per_launch": round(total_bytes / launches),
             }
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args, text)
+    full = spp * args.steps if args.workload == "cornell" else 0
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, text, W, H, depth, full_spp=full)
+    config = {
+        "workload": what,
+        "spp_per_step": spp,
+        "paths_per_step": W * H * spp,
+        "segments": cnt["segments"],
+        "shadow_segments": cnt["shadow_segments"],
+        "segments_per_path": cnt["segments"] / max(1, cnt["paths"]),
+    }
+    if args.workload == "sdf":
+        config["sdf_steps_per_segment"] = cnt["sdf_steps"] / max(1, cnt["segments"] + cnt["shadow_segments"])
+        config["sdf_exhausted"] = cnt["sdf_exhausted"]
+        config["march_tflops"] = {k: round(pcnt[v] / 1e12, 4) for k, v in FLOPS_COUNTER.items() if pcnt.get(v)}
+    else:
+        config["bvh_nodes_per_segment"] = cnt["bvh_nodes"] / max(1, cnt["segments"])
+        config["bvh_nodes_per_shadow_ray"] = cnt["shadow_bvh_nodes"] / max(1, cnt["shadow_segments"])
     line = {
         "metric": METRIC,
         "value": seg_rate,
@@ -542,17 +636,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {
-            "workload": f"Cornell box + {args.spheres} random spheres (60% Lambert / 25% dielectric / 15% conductor), "
-                        f"{W}x{H}, depth {args.depth}, 4 hero wavelengths",
-            "spp_per_step": args.spp,
-            "paths_per_step": W * H * args.spp,
-            "segments": cnt["segments"],
-            "shadow_segments": cnt["shadow_segments"],
-            "segments_per_path": cnt["segments"] / max(1, cnt["paths"]),
-            "bvh_nodes_per_segment": cnt["bvh_nodes"] / max(1, cnt["segments"]),
-            "bvh_nodes_per_shadow_ray": cnt["shadow_bvh_nodes"] / max(1, cnt["shadow_segments"]),
-        },
+        "config": config,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

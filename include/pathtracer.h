@@ -134,6 +134,8 @@ typedef struct pt_config {
     int32_t graph_mode;    /* batched serving waves: 0 = captured once per launch shape into a hipGraph and
                               replayed, 1 = kernels launched one by one */
     int32_t blocks_per_cu; /* extended mode: grid of the queue kernels, blocks per compute unit, 1..64 (0 = 8) */
+    int32_t march;         /* extended mode, SDF sphere tracing: 0 = persistent lanes refilled from the queue,
+                              1 = one ray per lane */
 } pt_config;
 
 typedef struct pt_counters {
@@ -152,6 +154,13 @@ typedef struct pt_counters {
     uint64_t bvh_nodes;       /* BVH nodes visited by extension rays (extended mode) */
     uint64_t shadow_bvh_nodes;/* BVH nodes visited by shadow rays (extended mode) */
     uint64_t sdf_exhausted;   /* SDF marches that ran out of sdf_max_steps (taken as misses; extended mode) */
+    /* extended mode: FP32 operations of sphere tracing (distance-estimator
+     * evaluations plus the step update), extension rays by SDF kind, then shadow rays */
+    uint64_t march_flops_mandelbulb;
+    uint64_t march_flops_menger;
+    uint64_t march_flops_blob;
+    uint64_t march_flops_terrain;
+    uint64_t shadow_march_flops;
 } pt_counters;
 
 typedef struct pt_ctx pt_ctx;

@@ -102,6 +102,7 @@ class Config(C.Structure):
         ("bvh_leaf", C.c_int32),
         ("graph_mode", C.c_int32),
         ("blocks_per_cu", C.c_int32),
+        ("march", C.c_int32),
     ]
 
 
@@ -125,6 +126,11 @@ class Counters(C.Structure):
         ("bvh_nodes", C.c_uint64),
         ("shadow_bvh_nodes", C.c_uint64),
         ("sdf_exhausted", C.c_uint64),
+        ("march_flops_mandelbulb", C.c_uint64),
+        ("march_flops_menger", C.c_uint64),
+        ("march_flops_blob", C.c_uint64),
+        ("march_flops_terrain", C.c_uint64),
+        ("shadow_march_flops", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -333,9 +339,9 @@ class Context:
 
     def __init__(self, device: int = 0, wave_paths: int = 0, block_size: int = 0, bvh_width: int = 0,
                  sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0, bvh_leaf: int = 0,
-                 graph_mode: int = 0, blocks_per_cu: int = 0):
+                 graph_mode: int = 0, blocks_per_cu: int = 0, march: int = 0):
         cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, bvh_leaf, graph_mode,
-                     blocks_per_cu)
+                     blocks_per_cu, march)
         h = C.c_void_p()
         _check(lib.pt_create(C.byref(h), device, C.byref(cfg)))
         self.h = h

@@ -84,7 +84,7 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
         ctx->cfg.blocks_per_cu < 0 || ctx->cfg.blocks_per_cu > 64 || ctx->cfg.sdf_max_steps < 0 ||
         (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
         ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16 ||
-        ctx->cfg.graph_mode < 0 || ctx->cfg.graph_mode > 1) {
+        ctx->cfg.graph_mode < 0 || ctx->cfg.graph_mode > 1 || ctx->cfg.march < 0 || ctx->cfg.march > 1) {
         delete ctx;
         return set_global(PT_ERR_INVALID_ARG, "pt_create: invalid pt_config");
     }
@@ -467,6 +467,11 @@ pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     out->bvh_nodes = h[11];
     out->shadow_bvh_nodes = h[12];
     out->sdf_exhausted = h[13];
+    out->march_flops_mandelbulb = h[ptd::kStatMarchFlops + 0];
+    out->march_flops_menger = h[ptd::kStatMarchFlops + 1];
+    out->march_flops_blob = h[ptd::kStatMarchFlops + 2];
+    out->march_flops_terrain = h[ptd::kStatMarchFlops + 3];
+    out->shadow_march_flops = h[ptd::kStatShadowMarchFlops];
     return PT_OK;
 }
 

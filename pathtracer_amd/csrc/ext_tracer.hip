@@ -54,7 +54,9 @@ enum : int { CNT_SHADOW0 = 2, CNT_MAT = 3, CNT_SHADOW1 = 3 + N_MATQ, N_CNT = 4 +
 constexpr int shadow_cnt(int b) { return (b & 1) ? CNT_SHADOW1 : CNT_SHADOW0; }
 // Persistent kernels' queue cursors: kSeg segments per queue, one line each.
 constexpr int kSeg = 32;
-enum : int { CUR_ISECT = 0, CUR_SHADOW = 1 };
+// Cursor sets: persistent intersect, persistent shadow, and one per SDF kind for
+// the persistent march kernels (CUR_MARCH + kind - 1).
+enum : int { CUR_ISECT = 0, CUR_SHADOW = 1, CUR_MARCH = 2, N_CUR = 6 };
 
 struct ExtMat {
     int type;        // pt::MaterialType
@@ -440,11 +442,13 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
 // ---------------------------------------------------------------------------------
 // SDF distance estimators (extended mode: the shapes of sdfs/*.glsl in plain
 // FP32 arithmetic).
-__device__ __forceinline__ float ext_mandelbulb(f3 p, int iters, float bailout) {
+// `it` counts the iterations executed (the loop leaves early past the bailout).
+__device__ __forceinline__ float ext_mandelbulb(f3 p, int iters, float bailout, unsigned& it) {
     f3 z = p;
     float r = length(z);
     float dr = 1.0f;
     for (int i = 0; i < iters; ++i) {
+        ++it;
         float r2 = r * r, r4 = r2 * r2;
         dr = 8.0f * (r4 * r2 * r) * dr + 1.0f;
         float invr = 1.0f / r;
@@ -495,45 +499,75 @@ __device__ __forceinline__ float ext_terrain_h(float x) {
            0.125f * __sinf(4.0f * x + 2.0f) + 0.0625f * __sinf(8.0f * x + 1.0f) +
            0.03125f * __sinf(16.0f * x + 5.0f) + 0.015625f * __sinf(32.0f * x) + 0.0078125f * __sinf(64.0f * x);
 }
-template <int KIND>
-__device__ __forceinline__ float ext_de(const ExtSdf& s, f3 p) {
-    f3 q = p - s.c;
-    const int kind = KIND ? KIND : s.kind;
+// FP32 operations of the distance estimators as written above, counted one per
+// add, subtract, multiply, divide, min, max, floor, sqrt, rsqrt, log and sin and
+// two per fused multiply-add (abs and negation are source modifiers; compares
+// and selects are not counted): mandelbulb 10 + 65 per iteration executed,
+// menger 16 + 33 per level, blob 65, terrain 93. A march step adds 10: the
+// point o + d t, its offset from the object's centre, and t += dist.
+__device__ __forceinline__ unsigned de_step_flops(int kind, int iters) {
     switch (kind) {
-        case PT_SDF_MANDELBULB: return ext_mandelbulb(q, s.iters, s.bailout);
-        case PT_SDF_MENGER: return ext_menger(q, s.iters);
+        case PT_SDF_MANDELBULB: return 10u + 10u;
+        case PT_SDF_MENGER: return 10u + 16u + 33u * (unsigned)iters;
+        case PT_SDF_BLOB: return 10u + 65u;
+        default: return 10u + 93u;
+    }
+}
+__device__ __forceinline__ unsigned de_iter_flops(int kind) { return kind == PT_SDF_MANDELBULB ? 65u : 0u; }
+
+// Distance to an SDF of `kind` centred at c with `iters` iterations (levels).
+__device__ __forceinline__ float ext_de_at(int kind, f3 c, int iters, float bailout, f3 p, unsigned& it) {
+    f3 q = p - c;
+    switch (kind) {
+        case PT_SDF_MANDELBULB: return ext_mandelbulb(q, iters, bailout, it);
+        case PT_SDF_MENGER: return ext_menger(q, iters);
         case PT_SDF_BLOB: return ext_blob(q);
         default: return (q.y - (ext_terrain_h(q.x) + ext_terrain_h(q.z))) * 0.5f;  // gradient bound ~2
     }
 }
+template <int KIND>
+__device__ __forceinline__ float ext_de(const ExtSdf& s, f3 p, unsigned& it) {
+    return ext_de_at(KIND ? KIND : s.kind, s.c, s.iters, s.bailout, p, it);
+}
 // Sphere tracing of one SDF object inside its box, up to tmax. Returns the hit
-// distance or a negative value.
+// distance or a negative value. Adds its steps to `steps` and their FP32
+// operations (de_step_flops, de_iter_flops) to `flops`.
 template <int KIND>
 __device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_steps, unsigned& steps,
-                           unsigned& exhausted) {
+                           unsigned& exhausted, unsigned long long& flops) {
     f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     f3 t0 = (s.lo - o) * inv, t1 = (s.hi - o) * inv;
     f3 a = min3(t0, t1), b = max3(t0, t1);
     float tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
     float tout = fminf(fminf(fminf(b.x, b.y), b.z), tmax);
     if (!(tin < tout)) return -1.0f;
-    float t = tin;
-    for (int i = 0; i < max_steps; ++i) {
-        float dist = ext_de<KIND>(s, o + d * t);
-        ++steps;
-        if (dist < 1e-4f) return t;
-        t += dist;
-        if (t > tout) return -1.0f;
+    float t = tin, res = -1.0f;
+    unsigned n = 0, it = 0;
+    bool open = true;  // neither a hit nor past the box's exit yet
+    while (open && n < (unsigned)max_steps) {
+        float dist = ext_de<KIND>(s, o + d * t, it);
+        ++n;
+        if (dist < 1e-4f) {
+            res = t;
+            open = false;
+        } else {
+            t += dist;
+            open = !(t > tout);
+        }
     }
-    ++exhausted;  // the step budget ran out before a hit or the box's exit: a miss
-    return -1.0f;
+    if (open) ++exhausted;  // the step budget ran out before a hit or the box's exit: a miss
+    steps += n;
+    const int kind = KIND ? KIND : s.kind;
+    flops += (unsigned long long)n * de_step_flops(kind, s.iters) + (unsigned long long)it * de_iter_flops(kind);
+    return res;
 }
 template <int KIND>
 __device__ f3 ext_sdf_normal(const ExtSdf& s, f3 p) {
     const float e = 1e-4f;
-    return normalize(mk3(ext_de<KIND>(s, p + mk3(e, 0, 0)) - ext_de<KIND>(s, p - mk3(e, 0, 0)),
-                         ext_de<KIND>(s, p + mk3(0, e, 0)) - ext_de<KIND>(s, p - mk3(0, e, 0)),
-                         ext_de<KIND>(s, p + mk3(0, 0, e)) - ext_de<KIND>(s, p - mk3(0, 0, e))));
+    unsigned it = 0;
+    return normalize(mk3(ext_de<KIND>(s, p + mk3(e, 0, 0), it) - ext_de<KIND>(s, p - mk3(e, 0, 0), it),
+                         ext_de<KIND>(s, p + mk3(0, e, 0), it) - ext_de<KIND>(s, p - mk3(0, e, 0), it),
+                         ext_de<KIND>(s, p + mk3(0, 0, e), it) - ext_de<KIND>(s, p - mk3(0, 0, e), it)));
 }
 
 // ---------------------------------------------------------------------------------
@@ -658,7 +692,7 @@ __global__ void k_ext_counts(int* counts, int n0) {
 __global__ void k_ext_reset(int* counts, int* cursor, int nxt, int shq) {
     int i = threadIdx.x;
     if (i == nxt || i == shq || (i >= CNT_MAT && i < CNT_MAT + N_MATQ)) counts[i * kCntStride] = 0;
-    if (i < 2 * kSeg) cursor[i * kCntStride] = 0;
+    for (int j = i; j < N_CUR * kSeg; j += blockDim.x) cursor[j * kCntStride] = 0;
 }
 
 // BK7 Sellmeier index (the reference camera lens, src/shader.comp:1064-1073).
@@ -1013,6 +1047,7 @@ template <int KIND, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     unsigned steps = 0, exhausted = 0;
+    unsigned long long flops = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
@@ -1023,7 +1058,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
         bool upd = false;
         for (int s = 0; s < S.nsdf; ++s) {
             if (S.sdf[s].kind != KIND) continue;
-            float th = ext_march<KIND>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted);
+            float th = ext_march<KIND>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted, flops);
             if (th > 0.0f && th < t) {
                 t = th;
                 ref = (REF_SDF << kRefShift) | (uint32_t)s;
@@ -1034,6 +1069,126 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
     wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
+    wave_add_u64(stat_slot(Wv.stats, kStatMarchFlops + KIND - 1), flops);
+}
+
+// Persistent sphere tracing with lane refill (the default march). A ray's march
+// lasts from one step to the whole step budget, so a one-ray-per-lane wave runs
+// at the pace of its longest march with most lanes idle. Here each wave loops
+// over a segmented queue (WaveChunk, as the persistent traversal): every busy
+// lane takes one distance-estimator step per iteration, and lanes whose march
+// has ended take the next queue entries while at least kMarchRefill lanes are
+// idle. Per ray, the SDFs of this kind are marched in scene order with the same
+// box clip, steps and comparisons as ext_march, each bounded by the nearest hit
+// so far, so hits, step counts and operation counts equal k_ext_march's.
+constexpr int kMarchRefill = 8;
+
+template <int KIND, bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
+    const int n = *qlen(Wv, cur);
+    const int* q = Wv.q[cur];
+    unsigned steps = 0, exhausted = 0, it = 0;
+    unsigned long long flops = 0;
+    // Lane state: path p (-1 idle), its ray, nearest hit (t, ref), index s of
+    // the SDF being marched, and that march's position tm, exit tout, step
+    // count and SDF parameters. `seek`: start the next SDF's march.
+    int p = -1, s = -1, nsdf = 0, smax = 0, nst = 0;
+    bool upd = false, seek = false;
+    f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, sc = o;
+    float t = 0.0f, tm = 0.0f, tout = 0.0f, sbail = 0.0f;
+    int siters = 0;
+    uint32_t ref = kMiss, sflops = 0;
+    const ExtSdf* sdfs = nullptr;
+    WaveChunk chunk = wave_chunk_init(Wv.cursor + (CUR_MARCH + KIND - 1) * kSeg * kCntStride, n);
+    while (true) {
+        const uint64_t idle = __ballot(p < 0);
+        if (!chunk.drained() && __popcll(idle) >= kMarchRefill) {
+            const int e = wave_refill(idle, chunk);
+            if (e >= 0) {
+                p = q[e];
+                const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+                sdfs = S.sdf;
+                nsdf = S.nsdf;
+                smax = S.sdf_steps_max;
+                o = ld3(Wv.org, p);
+                d = ld3(Wv.dir, p);
+                const float2 h = Wv.hit[p];
+                t = h.x;
+                ref = __float_as_uint(h.y);
+                upd = false;
+                s = -1;
+                seek = true;
+            }
+        }
+        // The next SDF of this kind whose box the ray enters before its nearest
+        // hit; with none left, the hit is stored and the lane goes idle.
+        if (seek) {
+            seek = false;
+            int ns = s + 1;
+            bool found = false;
+            float tin = 0.0f, tx = 0.0f;
+            for (; ns < nsdf; ++ns) {
+                const ExtSdf& e = sdfs[ns];
+                if (e.kind != KIND) continue;
+                f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                f3 t0 = (e.lo - o) * inv, t1 = (e.hi - o) * inv;
+                f3 a = min3(t0, t1), b = max3(t0, t1);
+                tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
+                tx = fminf(fminf(fminf(b.x, b.y), b.z), t);
+                if (tin < tx) {
+                    found = true;
+                    break;
+                }
+            }
+            s = ns;
+            if (found) {
+                const ExtSdf& e = sdfs[ns];
+                tm = tin;
+                tout = tx;
+                nst = 0;
+                sc = e.c;
+                siters = e.iters;
+                sbail = e.bailout;
+                sflops = de_step_flops(KIND, e.iters);
+            } else {
+                if (upd) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+                p = -1;
+            }
+        }
+        const uint64_t busy = __ballot(p >= 0);
+        if (busy == 0ull) {
+            if (chunk.drained()) break;
+            continue;
+        }
+        // fill the wave before stepping while the queue has entries
+        if (!chunk.drained() && __popcll(~busy) >= kMarchRefill) continue;
+        if (p >= 0) {
+            if (nst < smax) {
+                const float dist = ext_de_at(KIND, sc, siters, sbail, o + d * tm, it);
+                ++nst;
+                ++steps;
+                flops += sflops;
+                if (dist < 1e-4f) {
+                    if (tm > 0.0f && tm < t) {
+                        t = tm;
+                        ref = (REF_SDF << kRefShift) | (uint32_t)s;
+                        upd = true;
+                    }
+                    seek = true;
+                } else {
+                    tm += dist;
+                    seek = tm > tout;
+                }
+            } else {
+                ++exhausted;  // the step budget ran out before a hit or the box's exit: a miss
+                seek = true;
+            }
+        }
+    }
+    flops += (unsigned long long)it * de_iter_flops(KIND);
+    wave_add_u64(stat_slot(Wv.stats, 2), steps);
+    wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
+    wave_add_u64(stat_slot(Wv.stats, kStatMarchFlops + KIND - 1), flops);
 }
 
 template <bool MULTI>
@@ -1405,6 +1560,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B,
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0, visits = 0, exhausted = 0;
+    unsigned long long flops = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
@@ -1413,12 +1569,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B,
         uint32_t ref = kMiss;
         bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack, visits);
         for (int s = 0; !blocked && s < S.nsdf; ++s)
-            blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted) > 0.0f;
+            blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted, flops) > 0.0f;
         if (!blocked) add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
     wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
     wave_add_u64(stat_slot(Wv.stats, 12), visits);
+    wave_add_u64(stat_slot(Wv.stats, kStatShadowMarchFlops), flops);
 }
 
 // Persistent any-hit kernel for the shadow queue; see the persistent traversal
@@ -1431,6 +1588,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0, exhausted = 0;
+    unsigned long long flops = 0;
     int i = -1, sp = 0, node = -1, best = -1, path = 0;
     unsigned visits = 0;
     bool blocked = false;
@@ -1478,7 +1636,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
                 blocked = tp > kEps && tp < t;
             }
             for (int s = 0; !blocked && s < S.nsdf; ++s)
-                blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted) > 0.0f;
+                blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted, flops) > 0.0f;
             if (!blocked) add_xyz(Wv, path, ld3(Wv.s_xyz, i));
             i = -1;
         }
@@ -1486,6 +1644,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
     wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
     wave_add_u64(stat_slot(Wv.stats, 12), visits);
+    wave_add_u64(stat_slot(Wv.stats, kStatShadowMarchFlops), flops);
 }
 
 // Per pixel, in sample order: add each path's XYZ to the running sum.
@@ -1939,7 +2098,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     // shadow org/dir/xyz 48, sums 12
     size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 12;
     // + 256-byte rounding of 22 arrays + the counters and cursors, one line each
-    int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + 2 * kSeg) * kCntStride * 4);
+    int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
     char* p = static_cast<char*>(st->mem.p);
     ExtWave& W = st->wave;
@@ -1952,7 +2111,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     W.aov0 = carve_e<float4>(p, n); W.aov1 = carve_e<float4>(p, n);
     W.sum = carve_e<float>(p, 3 * n);
     W.counts = carve_e<int>(p, (size_t)N_CNT * kCntStride);
-    W.cursor = carve_e<int>(p, (size_t)2 * kSeg * kCntStride);
+    W.cursor = carve_e<int>(p, (size_t)N_CUR * kSeg * kCntStride);
     W.stats = ctx->d_stats;
     W.cap = cap;
     return PT_OK;
@@ -1984,7 +2143,10 @@ struct ExtPlan {
 
 template <int KIND, bool MULTI>
 static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int grid) {
-    launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
+    if (ctx->cfg.march == 1)
+        launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
+    else
+        launch(ctx, name, k_ext_march_pt<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
 }
 
 // One bounce b (extension queue `cur`). Scenes without SDF objects trace the

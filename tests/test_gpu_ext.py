@@ -203,6 +203,75 @@ def test_counters_and_no_nans(gpu_ctx):
     assert c["bvh_nodes"] >= c["segments"] and c["shadow_bvh_nodes"] > 0
 
 
+def lit_sdf_scene(kind: str) -> str:
+    """The mandelbulb + menger scene with one of the two removed and a 6500 K
+    quad light above, so that shadow rays march the SDF too."""
+    d = scenes.sdf_scene_dict(seed=3)
+    d["sdf"] = [s for s in d["sdf"] if s["kind"] == kind]
+    d["quad"] = [{"position": [-0.5, 4.0, -0.5], "edge1": [1.0, 0.0, 0.0], "edge2": [0.0, 0.0, 1.0],
+                  "materialID": 1, "lightID": 1}]
+    d["light"] = [{"emission": {"temperature": 6500.0, "luminosity": 12.0}}]
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("kind", ["menger", "mandelbulb"])
+def test_march_flop_counters(gpu_ctx, kind):
+    """The sphere-tracing FP32 operation counters against the per-step cost:
+    every step of a menger of 4 levels costs 10 + 16 + 33 * 4; a mandelbulb
+    step costs 10 + 10 + 65 per iteration executed, between 1 and 8."""
+    gpu_ctx.reset_counters()
+    gpu_render(gpu_ctx, lit_sdf_scene(kind), 64, 48, 4, 6)
+    c = gpu_ctx.counters()
+    steps = c["sdf_steps"]
+    assert steps > 0 and c["shadow_segments"] > 0 and c["shadow_march_flops"] > 0
+    own = "march_flops_" + kind
+    for k in ("mandelbulb", "menger", "blob", "terrain"):
+        if "march_flops_" + k != own:
+            assert c["march_flops_" + k] == 0, k
+    total = c[own] + c["shadow_march_flops"]
+    if kind == "menger":
+        assert total == steps * (10 + 16 + 33 * 4)
+    else:
+        iters, rem = divmod(total - steps * 20, 65)
+        assert rem == 0 and steps <= iters <= 8 * steps
+        assert iters > 2 * steps  # most evaluations near the surface run past the first iterations
+
+
+def four_kind_sdf_scene() -> str:
+    """Every SDF kind, two mengers (so a ray marches several SDFs of one kind),
+    and a quad light so that shadow rays march them too."""
+    d = json.loads(lit_sdf_scene("mandelbulb"))
+    d["sdf"] += [
+        {"kind": "menger", "iterations": 4, "position": [1.4, 1.0, 0.0], "boundingSize": [2.02, 2.02, 2.02],
+         "material": 2.0},
+        {"kind": "menger", "iterations": 3, "position": [0.2, 0.5, 2.5], "boundingSize": [2.02, 2.02, 2.02],
+         "material": 1.0},
+        {"kind": "blob", "position": [-0.2, 0.9, -1.4], "boundingSize": [2.2, 3.2, 2.2], "material": 2.0},
+        {"kind": "terrain", "position": [0.0, 1.5, 7.0], "boundingSize": [10.0, 3.0, 2.0], "material": 1.0},
+    ]
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("name", ["sdf", "four_kinds"])
+def test_persistent_march_is_invisible(name):
+    """Sphere tracing with lanes refilled from the queue (the default) and one
+    ray per lane give identical images, step counts, exhausted marches and
+    operation counts."""
+    text, depth = {"sdf": (scenes.sdf_scene(seed=5), 8), "four_kinds": (four_kind_sdf_scene(), 6)}[name]
+    imgs, cnts = [], []
+    for march in (0, 1):
+        with pt.Context(0, march=march, sdf_max_steps=96) as c:
+            imgs.append(gpu_render(c, text, 64, 48, 4, depth))
+            cnts.append(c.counters())
+    assert np.array_equal(imgs[0], imgs[1])
+    assert cnts[0]["sdf_steps"] > 0 and cnts[0]["sdf_exhausted"] > 0
+    for k in ("segments", "shadow_segments", "sdf_steps", "sdf_exhausted", "lambert_hits", "march_flops_mandelbulb",
+              "march_flops_menger", "march_flops_blob", "march_flops_terrain", "shadow_march_flops"):
+        assert cnts[0][k] == cnts[1][k], k
+    if name == "four_kinds":
+        assert all(cnts[0]["march_flops_" + k] > 0 for k in ("mandelbulb", "menger", "blob", "terrain"))
+
+
 @pytest.mark.parametrize("name", ["spheres", "sdf", "cornell"])
 def test_persistent_traversal_is_invisible(scene_texts, name):
     """Persistent lanes refilled from the queue visit the same nodes per ray as
@@ -219,5 +288,5 @@ def test_persistent_traversal_is_invisible(scene_texts, name):
             cnts.append(c.counters())
     assert np.array_equal(imgs[0], imgs[1])
     for k in ("segments", "shadow_segments", "sdf_steps", "lambert_hits", "dielectric_hits", "conductor_hits",
-              "bvh_nodes", "shadow_bvh_nodes"):
+              "bvh_nodes", "shadow_bvh_nodes", "march_flops_mandelbulb", "march_flops_menger", "shadow_march_flops"):
         assert cnts[0][k] == cnts[1][k], k
