@@ -3,15 +3,16 @@
 Workloads (--workload):
   spheres (default)  Cornell box plus 4096 random spheres (60 % Lambert, 25 %
                      dispersive dielectric, 15 % conductor), 1920x1080, path
-                     depth 12, 4 hero wavelengths, 2 spp per step.
+                     depth 12, 4 hero wavelengths, 16 spp per step (one wave
+                     of 33 M paths).
   cornell            the Cornell box alone (5 quads, a Lambert and a Cauchy
                      dielectric sphere, 6500 K quad light), 512x512, depth 8,
-                     one wavelength per path, 16 spp per step: 4 steps are the
+                     one wavelength per path, 64 spp per step: a step is the
                      whole 64-spp render.
   sdf                power-8 mandelbulb (8 iterations) and level-4 menger
                      sponge over a ground plane under a uniform white sky,
                      sphere tracing of at most 256 steps to 1e-4, 1920x1080,
-                     depth 8, 4 hero wavelengths, 2 spp per step.
+                     depth 8, 4 hero wavelengths, 16 spp per step.
 A step is one dispatch tracing `--spp` samples of every pixel (one wave of
 paths); the timed region is exactly `--steps` dispatches bracketed by stream
 synchronisation, with the scene, BVH and path buffers resident in HBM.
@@ -79,18 +80,18 @@ def workload(args):
     from pathtracer_amd import scenes
     if args.workload == "cornell":
         text = scenes.cornell(wavelengths=1, seed=0)
-        dflt = (512, 512, 8, 16)
+        dflt = (512, 512, 8, 64)
         what = ("Cornell box (5 quads with 3-Gaussian spectra, Lambert sphere, Cauchy dielectric sphere "
                 "A=1.5 B=0.01, 6500 K quad light), {w}x{h}, depth {d}, 1 wavelength per path in 380-780 nm")
     elif args.workload == "sdf":
         text = scenes.sdf_scene(seed=0)
-        dflt = (1920, 1080, 8, 2)
+        dflt = (1920, 1080, 8, 16)
         what = ("mandelbulb (power 8, 8 iterations, bailout 16) + level-4 menger sponge over a ground plane, "
                 "uniform white environment, sphere tracing <= 256 steps to 1e-4, {w}x{h}, depth {d}, "
                 "4 hero wavelengths")
     else:
         text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
-        dflt = (1920, 1080, 12, 2)
+        dflt = (1920, 1080, 12, 16)
         what = ("Cornell box + " + str(args.spheres) + " random spheres (60% Lambert / 25% dielectric / "
                 "15% conductor), {w}x{h}, depth {d}, 4 hero wavelengths")
     w = args.width or dflt[0]
@@ -116,6 +117,8 @@ def parse_args():
     ap.add_argument("--bvh-leaf", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--traversal", type=int, default=0, help="0 one ray per lane, 1 persistent lanes refilled from the queue")
+    ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane")
+    ap.add_argument("--march-blocks", type=int, default=0, help="blocks per CU of the persistent march (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
@@ -158,9 +161,10 @@ def copy_probe(mib: int, steps: int):
     print(json.dumps({"copy_bytes": n * 4, "steps": steps, "GB/s_read+write": 2 * n * 4 * steps / dt / 1e9}))
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc summary."""
-    for f in sorted((ROOT / "profiles").glob(f"pmc_{kernel}*.txt")):
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from a committed rocprofv3
+    --pmc summary (profiles/pmc_<kernel>_<workload>.txt)."""
+    for f in sorted((ROOT / "profiles").glob(f"pmc_{kernel}_{workload}.txt")):
         for line in f.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 try:
@@ -182,10 +186,10 @@ def cpu_baseline(args, scene_text: str, W: int, H: int, depth: int, full_spp: in
         return {"value": None, "unit": "segments/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     # time of one sample per pixel over the frame, from 8 bands spread over its height
-    band = max(1, H // 64)
+    band = max(1, H // 16)
     probe_s = 0.0
     for k in range(8):
-        r0 = min(H - band, (2 * k + 1) * H // 16)
+        r0 = min(H - band, k * H // 8)
         probe_s += ptext.render(scene_text, W, H, spp=1, depth=depth, threads=threads, rows=(r0, r0 + band))["seconds"]
     frame_s = probe_s * H / (8 * band)
     spp = max(1, int(round(args.cpu_seconds / max(frame_s, 1e-9))))
@@ -537,8 +541,10 @@ def trace_bench(args):
     import pathtracer_amd as pt
 
     text, W, H, depth, spp, what = workload(args)
+    # name of this workload's profile summaries under profiles/
+    tag = {"spheres": f"spheres{args.spheres}_{W}x{H}", "sdf": f"sdf_{W}x{H}", "cornell": f"cornell_{W}x{H}"}[args.workload]
     ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal,
-                     bvh_leaf=args.bvh_leaf)
+                     bvh_leaf=args.bvh_leaf, march=args.march, march_blocks_per_cu=args.march_blocks)
     scene = ctx.scene_from_json(text)
     r = pt.Renderer(ctx, scene, W, H, spf=spp, path_length=depth, mode=pt.MODE_EXT)
     for _ in range(args.warmup):
@@ -587,7 +593,7 @@ def trace_bench(args):
                 "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                "traffic": traffic_from_profiles(name),
+                "traffic": traffic_from_profiles(name, tag),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_flops_per_launch": round(flops / launches),
@@ -601,12 +607,12 @@ def trace_bench(args):
                 "peak": HBM_PEAK_GBS,
                 "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic_from_profiles(name),
+                "traffic": traffic_from_profiles(name, tag),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_bytes_per_launch": round(total_bytes / launches),
             }
-    full = spp * args.steps if args.workload == "cornell" else 0
+    full = spp if args.workload == "cornell" else 0
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, text, W, H, depth, full_spp=full)
     config = {
         "workload": what,

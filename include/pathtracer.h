@@ -123,7 +123,8 @@ typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
 
 /* Launch/tuning knobs (0 = library default). */
 typedef struct pt_config {
-    int32_t wave_paths;    /* paths in flight per wave (queue capacity) */
+    int32_t wave_paths;    /* paths in flight per wave (queue capacity); 0 = 2^25 for pt_dispatch (allocated as a
+                              dispatch needs, about 208 B per path) and 2^22 for a server's batched waves */
     int32_t block_size;    /* threads per block of the queue kernels: 0 or 256 (they are compiled for 256
                               lanes, 4 wavefronts; other values are rejected) */
     int32_t bvh_width;     /* 2, 4 or 8 (0 = 2) */
@@ -136,6 +137,8 @@ typedef struct pt_config {
     int32_t blocks_per_cu; /* extended mode: grid of the queue kernels, blocks per compute unit, 1..64 (0 = 8) */
     int32_t march;         /* extended mode, SDF sphere tracing: 0 = persistent lanes refilled from the queue,
                               1 = one ray per lane */
+    int32_t march_blocks_per_cu; /* extended mode: grid of the persistent march kernels, blocks per compute unit,
+                                    1..64 (0 = the default) */
 } pt_config;
 
 typedef struct pt_counters {

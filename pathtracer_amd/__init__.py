@@ -103,6 +103,7 @@ class Config(C.Structure):
         ("graph_mode", C.c_int32),
         ("blocks_per_cu", C.c_int32),
         ("march", C.c_int32),
+        ("march_blocks_per_cu", C.c_int32),
     ]
 
 
@@ -339,9 +340,9 @@ class Context:
 
     def __init__(self, device: int = 0, wave_paths: int = 0, block_size: int = 0, bvh_width: int = 0,
                  sdf_max_steps: int = 0, hbm_budget: int = 0, traversal: int = 0, bvh_leaf: int = 0,
-                 graph_mode: int = 0, blocks_per_cu: int = 0, march: int = 0):
+                 graph_mode: int = 0, blocks_per_cu: int = 0, march: int = 0, march_blocks_per_cu: int = 0):
         cfg = Config(wave_paths, block_size, bvh_width, sdf_max_steps, hbm_budget, traversal, bvh_leaf, graph_mode,
-                     blocks_per_cu, march)
+                     blocks_per_cu, march, march_blocks_per_cu)
         h = C.c_void_p()
         _check(lib.pt_create(C.byref(h), device, C.byref(cfg)))
         self.h = h

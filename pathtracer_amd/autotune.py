@@ -6,8 +6,10 @@ returns the fastest configuration:
 
   bvh_width x bvh_leaf   BVH arity (2, 4, 8) and primitives per leaf (1, 2, 4)
   traversal              one ray per lane, or persistent lanes refilled from the queue
-  wave_paths             paths in flight per wave (queue capacity)
+  wave_paths             paths in flight per wave (queue capacity); each
+                         candidate is measured on dispatches that fill its wave
   blocks_per_cu          grid shape of the queue kernels (blocks of 256 lanes per CU)
+  march_blocks_per_cu    grid shape of the persistent SDF march kernels (SDF scenes)
   sdf_max_steps          the smallest sphere-tracing budget at which at most
                          `sdf_tolerance` of the traced rays (extension and
                          shadow) have a march run out of steps (an exhausted
@@ -38,8 +40,9 @@ from pathtracer_amd import scenes
 
 WIDTHS = (2, 4, 8)
 LEAVES = (1, 2, 4)
-WAVES = (1 << 20, 1 << 21, 1 << 22, 1 << 23)
+WAVES = (1 << 22, 1 << 23, 1 << 24, 1 << 25)
 BLOCKS_PER_CU = (4, 8, 16)
+MARCH_BLOCKS_PER_CU = (2, 4, 8, 16)
 SDF_BUDGETS = (32, 48, 64, 96, 128, 192, 256)
 
 
@@ -48,8 +51,9 @@ class Choice:
     bvh_width: int = 2
     bvh_leaf: int = 1
     traversal: int = 0
-    wave_paths: int = 1 << 22
+    wave_paths: int = 1 << 25
     blocks_per_cu: int = 8
+    march_blocks_per_cu: int = 8
     sdf_max_steps: int = 256
 
     def context(self, **over) -> pt.Context:
@@ -106,9 +110,7 @@ def measure(choice: Choice, text: str, w: int, h: int, depth: int, spp: int, rep
 def autotune(text: str, width: int, height: int, depth: int, spp: int = 0, repeats: int = 3,
              sdf_tolerance: float = 1e-4, log=None) -> Result:
     """Coordinate descent over the parameters above for this workload; spp
-    defaults to enough samples per pixel for about 4M paths per dispatch."""
-    if spp <= 0:
-        spp = max(1, (1 << 22) // (width * height))
+    defaults to the samples per pixel that fill each candidate's wave."""
     has_sdf = '"sdf"' in text and '"kind"' in text
     res = Result(Choice())
     seen: dict[str, Trial] = {}
@@ -117,7 +119,8 @@ def autotune(text: str, width: int, height: int, depth: int, spp: int = 0, repea
         key = json.dumps(asdict(c), sort_keys=True)
         if key in seen:
             return seen[key]
-        t = measure(c, text, width, height, depth, spp, repeats)
+        t = measure(c, text, width, height, depth, spp if spp > 0 else max(1, c.wave_paths // (width * height)),
+                    repeats)
         seen[key] = t
         res.trials.append(t)
         if log:
@@ -148,6 +151,8 @@ def autotune(text: str, width: int, height: int, depth: int, spp: int = 0, repea
     descend("traversal", (0, 1))
     descend("wave_paths", WAVES)
     descend("blocks_per_cu", BLOCKS_PER_CU)
+    if has_sdf:
+        descend("march_blocks_per_cu", MARCH_BLOCKS_PER_CU)
     return res
 
 
