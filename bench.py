@@ -139,6 +139,10 @@ def parse_args():
                          "one-at-a-time ones (one SRPT run per factor)")
     ap.add_argument("--denoiser", action="store_true",
                     help="denoiser workload instead: forward passes at --width x --height and training steps")
+    ap.add_argument("--denoiser-eval", action="store_true",
+                    help="the shipped denoiser on held-out 1080p scenes against a plain --eval-ref-spp render")
+    ap.add_argument("--eval-ref", type=int, default=16384, help="samples per pixel of the --denoiser-eval reference")
+    ap.add_argument("--weights", type=str, default="", help="denoiser parameters (.npy) instead of the shipped ones")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run --steps device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -368,6 +372,65 @@ def dn_forward_flops(n_pixels: int) -> float:
     return sum(2.0 * 9 * ci * co * n_pixels / div for ci, co, div in DN_LAYERS)
 
 
+def denoiser_eval(args):
+    """The learned denoiser at its benchmark size: held-out scenes of the
+    serving families (one mandelbulb + menger scene, one random-sphere box),
+    first-hit features of an spp-sample render at --width x --height
+    (default 1920 x 1080) denoised by the shipped network, against a plain
+    render of --eval-ref samples per pixel (default 16384) from disjoint
+    samples. Prints the relative RMSE of the plain mean and of the denoised
+    image at each input sample count, with the reference's own relative
+    standard error (its noise floor)."""
+    import numpy as np
+    import pathtracer_amd as pt
+    from pathtracer_amd import denoise_train as T, scenes
+
+    W, H = ((args.width or 1920) + 3) // 4 * 4, ((args.height or 1080) + 3) // 4 * 4
+    ref_spp = args.eval_ref
+    spps = [4, 16, 64]
+    held_out = [
+        ("sdf", scenes.sdf_scene(jitter=0.1, scene_seed=2_000_001, wavelengths=4, seed=2_000_001), 8),
+        ("spheres1000", scenes.random_spheres(n=1000, scene_seed=2_000_002, wavelengths=4, seed=2_000_002), 12),
+    ]
+    out = []
+    t0 = time.perf_counter()
+    with pt.Context(0) as ctx, pt.Denoiser(ctx, 1, H, W) as dn:
+        if args.weights:
+            T.load_weights(dn, args.weights)
+        else:
+            T.load_weights(dn)
+        for name, text, depth in held_out:
+            sc = ctx.scene_from_json(text)
+            ctx.sync()
+            tr = time.perf_counter()
+            ref = T.render_mean(ctx, sc, W, H, ref_spp, depth, first=max(spps), chunk=16)
+            ref_s = time.perf_counter() - tr
+            # noise floor of the reference: its mean's standard error, from the
+            # per-pixel spread of 64 samples scaled to ref_spp
+            f64 = T.render_features(ctx, sc, W, H, 64, depth, first=max(spps) + ref_spp)
+            se_y = f64[..., 10].astype(np.float64) * np.sqrt(64.0 / ref_spp)
+            floor = float(np.sqrt(np.sum(se_y ** 2) * 3.0) / np.sqrt(np.sum(ref.astype(np.float64) ** 2)))
+            row = {"scene": name, "depth": depth, "ref_spp": ref_spp, "ref_seconds": round(ref_s, 2),
+                   "ref_rel_se_estimate": round(floor, 5)}
+            for spp in spps:
+                feat = T.render_features(ctx, sc, W, H, spp, depth)
+                ctx.sync()
+                td = time.perf_counter()
+                den = T.denoise_image(dn, feat)
+                dt = time.perf_counter() - td
+                row[f"plain_{spp}spp"] = round(T.rel_rmse(feat[..., :3], ref), 5)
+                row[f"denoised_{spp}spp"] = round(T.rel_rmse(den, ref), 5)
+                row["denoise_seconds"] = round(dt, 4)
+            sc.release()
+            out.append(row)
+            print(json.dumps(row), flush=True)
+    print(json.dumps({
+        "metric": "denoiser relative RMSE against a plain render",
+        "config": {"workload": f"held-out serving-family scenes, {W}x{H}, inputs of 4/16/64 spp "
+                               f"(XYZ, albedo, normal, depth, standard error of Y), reference {ref_spp} spp"},
+        "results": out, "seconds": round(time.perf_counter() - t0, 1)}))
+
+
 def denoiser_bench(args):
     """Denoiser: --steps forward passes of the shipped network over the
     first-hit features of a 4-sample render at --width x --height (batch 1),
@@ -533,6 +596,9 @@ def main():
         return
     if args.denoiser:
         denoiser_bench(args)
+        return
+    if args.denoiser_eval:
+        denoiser_eval(args)
         return
     trace_bench(args)
 

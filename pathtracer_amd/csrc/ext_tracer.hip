@@ -309,6 +309,10 @@ __device__ __forceinline__ void ray_inv(f3 o, f3 d, f3& inv, f3& oi) {
 // pushes and pops touch the LDS stack, so most visits' dependent chain is one
 // node fetch. `hit` becomes the leaf slot of the closest hit so far. With ANY,
 // returns true at the first hit, and children are taken in node order.
+// Child count of a sphere-BVH leaf holding one sphere, stored in the parent's
+// record: lo_x, lo_y, lo_z, hi_x are its centre and radius, child its leaf slot.
+constexpr int kInlineSphere = -2;
+
 template <int W, typename SE, bool ANY, bool QUAD>
 __device__ __forceinline__ bool bvh_step(const BvhDev& B, f3 o, f3 d, f3 inv, f3 oi, float& tbest, int& hit,
                                          int& node, int& sp, SE* stack) {
@@ -339,7 +343,14 @@ __device__ __forceinline__ bool bvh_step(const BvhDev& B, f3 o, f3 d, f3 inv, f3
     // Leaves first (they can shrink tbest), then internal children.
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        if (tn[j] < big && cnt[j] > 0) {
+        if (!QUAD && cnt[j] == kInlineSphere) {
+            float t;
+            if (isect_sphere(make_float4(lo_x[j], lo_y[j], lo_z[j], hi_x[j]), o, d, tbest, t)) {
+                tbest = t;
+                hit = kid[j];
+                if (ANY) return true;
+            }
+        } else if (tn[j] < big && cnt[j] > 0) {
             for (int k = kid[j]; k < kid[j] + cnt[j]; ++k) {
                 float t;
                 bool h = QUAD ? isect_quad(B.prims + 4 * k, o, d, tbest, t) : isect_sphere(B.prims[k], o, d, tbest, t);
@@ -1986,6 +1997,25 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     for (size_t k = 0; k < sb.prim_refs.size(); ++k) {
         sleaf[k] = sph[sb.prim_refs[k]];
         sids[k] = (int)sb.prim_refs[k];
+    }
+    // A leaf child of one sphere keeps that sphere in its parent's record
+    // (lo_x, lo_y, lo_z, hi_x = centre, radius; count kInlineSphere), so the
+    // traversal tests it without fetching the sphere record after the node.
+    for (int n = 0; n < sb.n_nodes; ++n) {
+        uint32_t* w = sb.words.data() + (size_t)n * 8 * width;
+        for (int j = 0; j < width; ++j) {
+            int32_t cnt, child;
+            std::memcpy(&cnt, &w[7 * width + j], 4);
+            std::memcpy(&child, &w[6 * width + j], 4);
+            if (cnt != 1 || child < 0 || (size_t)child >= sleaf.size()) continue;
+            const float4 c = sleaf[(size_t)child];
+            std::memcpy(&w[0 * width + j], &c.x, 4);
+            std::memcpy(&w[1 * width + j], &c.y, 4);
+            std::memcpy(&w[2 * width + j], &c.z, 4);
+            std::memcpy(&w[3 * width + j], &c.w, 4);
+            const int32_t inl = kInlineSphere;
+            std::memcpy(&w[7 * width + j], &inl, 4);
+        }
     }
     const bool quad_bvh = quad_ml.size() > (size_t)kQuadBrute;
     pt::WideBvh qb;

@@ -80,12 +80,12 @@ def training_scene(i: int) -> tuple[str, int]:
     return scenes.sdf_scene(jitter=0.1, scene_seed=seed, wavelengths=4, seed=seed), 8
 
 
-def make_pair(ctx: pt.Context, text: str, depth: int, w: int, h: int, spp_in: int, spp_ref: int):
+def make_pair(ctx: pt.Context, text: str, depth: int, w: int, h: int, spp_in: int, spp_ref: int, chunk: int = 64):
     """-> (features (h, w, FEATURES), target (h, w, 3)) from independent samples."""
     sc = ctx.scene_from_json(text)
     try:
         feat = render_features(ctx, sc, w, h, spp_in, depth)
-        target = render_mean(ctx, sc, w, h, spp_ref, depth, first=spp_in)
+        target = render_mean(ctx, sc, w, h, spp_ref, depth, first=spp_in, chunk=chunk)
     finally:
         sc.release()
     return feat, target
@@ -149,24 +149,31 @@ def train_two_phase(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: in
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--first", type=int, default=0,
+                    help="index of the first training scene (scenes 100 and 101 are the gain tests' held-out ones)")
     ap.add_argument("--size", type=int, default=192)
     ap.add_argument("--spp-in", type=str, default="4,16,64,256,1024")
     ap.add_argument("--spp-ref", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=96)
+    ap.add_argument("--chunk", type=int, default=512, help="samples per pixel per dispatch of the targets")
     ap.add_argument("--out", type=str, required=True)
     a = ap.parse_args(argv)
     spp_in = [int(v) for v in a.spp_in.split(",")]
     t0 = time.perf_counter()
     with pt.Context(0) as ctx:
         pairs = []
-        for i in range(a.scenes):
+        for i in range(a.first, a.first + a.scenes):
+            if a.first <= 101 < a.first + a.scenes and i in (100, 101):
+                continue  # held out for the gain tests
             text, depth = training_scene(i)
-            feat, target = make_pair(ctx, text, depth, a.size, a.size, spp_in[i % len(spp_in)], a.spp_ref)
+            feat, target = make_pair(ctx, text, depth, a.size, a.size, spp_in[i % len(spp_in)], a.spp_ref,
+                                     chunk=a.chunk)
             pairs.append((feat, target))
-            print(f"pair {i}: spp {spp_in[i % len(spp_in)]} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
-                  f"({time.perf_counter() - t0:.0f} s)", flush=True)
+            if len(pairs) % 50 == 1 or a.scenes <= 100:
+                print(f"pair {i}: spp {spp_in[i % len(spp_in)]} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
+                      f"({time.perf_counter() - t0:.0f} s)", flush=True)
         with pt.Denoiser(ctx, a.batch, a.crop, a.crop, seed=1) as dn:
             train_two_phase(dn, pairs, a.steps, a.batch, a.crop, log_every=500)
             np.save(a.out, dn.params())
