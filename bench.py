@@ -132,6 +132,9 @@ def parse_args():
                          "denoiser, against independent plain renders of 4096-8192 samples per pixel")
     ap.add_argument("--serve-graphs", type=int, default=1,
                     help="1: batched waves replayed from captured hipGraphs; 0: kernels launched one by one")
+    ap.add_argument("--serve-wave", type=int, default=0,
+                    help="paths per batched wave of the server's context, both systems (0 = the library's 2^22)")
+    ap.add_argument("--serve-max-active", type=int, default=0, help="requests the SRPT server traces together (0 = 16)")
     ap.add_argument("--serve-denoise", action="store_true",
                     help="the SRPT server may finish requests with the shipped denoiser's image")
     ap.add_argument("--serve-speedup", type=str, default="1",
@@ -235,7 +238,7 @@ def serve_bench(args):
     from pathtracer_amd import scenes
 
     reqs = [scenes.serving_request(i) for i in range(args.serve)]
-    ctx = pt.Context(0, graph_mode=0 if args.serve_graphs else 1)
+    ctx = pt.Context(0, graph_mode=0 if args.serve_graphs else 1, wave_paths=args.serve_wave)
     t_start = time.perf_counter()
 
     def submit(srv, r, t):
@@ -275,7 +278,7 @@ def serve_bench(args):
 
     def srpt_run(speedup):
         done, segments = {}, 0
-        with pt.Server(ctx, policy=pt.SERVE_SRPT) as srv:
+        with pt.Server(ctx, policy=pt.SERVE_SRPT, max_active=args.serve_max_active) as srv:
             if dn is not None:
                 srv.set_denoiser(dn)
             base = srv.now()
@@ -345,7 +348,8 @@ def serve_bench(args):
                                "depth 8); resolution U{256^2, 512^2, 1024x768, 1920x1080}; "
                                "spp budget log-uniform 16..1024; Poisson arrivals",
                    "requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
-                   "arrival_rate_per_s": rate, "srpt_denoise": bool(args.serve_denoise)},
+                   "arrival_rate_per_s": rate, "srpt_denoise": bool(args.serve_denoise),
+                   "wave_paths": args.serve_wave or (1 << 22), "srpt_max_active": args.serve_max_active or 16},
         "srpt": srpt,
         "srpt_runs": runs,
         "throughput_ratio_at_equal_p99": equal_p99,

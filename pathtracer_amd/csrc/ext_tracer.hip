@@ -1212,17 +1212,76 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
     wave_add_u64(stat_slot(Wv.stats, kStatMarchFlops + KIND - 1), flops);
 }
 
+// Material queues from the hits of queue `cur`, in two passes over the block's
+// entries: the first counts each class, so the block reserves its slots with one
+// device atomic per class for all its rounds (atomics on one address serialise
+// across the chip, and per-round reservations of a 2^25-path wave cost about
+// 0.4 ms a launch); the second places the entries round by round from
+// block-local running offsets.
 template <bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
+    constexpr int NW = kBlk / 64;
+    __shared__ int s_base[N_MATQ];
+    __shared__ int s_n[N_MATQ][NW];
     const int n = *qlen(Wv, cur);
     const int stride = gridDim.x * blockDim.x;
     const int rounds = (n + stride - 1) / stride;
+    const int lane = (int)__lane_id(), wv = (int)threadIdx.x >> 6;
+    if ((int)threadIdx.x < N_MATQ) s_base[threadIdx.x] = 0;
+    __syncthreads();
+    int cnt[N_MATQ] = {};
     for (int r = 0; r < rounds; ++r) {
-        int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        bool active = idx < n;
-        int p = active ? Wv.q[cur][idx] : 0;
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
-        classify_append(S, Wv, active, p, active ? __float_as_uint(Wv.hit[p].y) : kMiss);
+        const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        if (idx < n) {
+            const int p = Wv.q[cur][idx];
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+            const int cls = hit_class(S, __float_as_uint(Wv.hit[p].y));
+#pragma unroll
+            for (int q = 0; q < N_MATQ; ++q) cnt[q] += cls == q ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < N_MATQ; ++q) {
+        int v = cnt[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane == 0 && v) atomicAdd(&s_base[q], v);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < N_MATQ) {
+        const int q = (int)threadIdx.x;
+        const int tot = s_base[q];
+        s_base[q] = tot ? atomicAdd(qlen(Wv, CNT_MAT + q), tot) : 0;
+    }
+    __syncthreads();
+    for (int r = 0; r < rounds; ++r) {
+        const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        int p = 0, cls = -1;
+        if (idx < n) {
+            p = Wv.q[cur][idx];
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+            cls = hit_class(S, __float_as_uint(Wv.hit[p].y));
+        }
+        uint64_t m[N_MATQ];
+#pragma unroll
+        for (int q = 0; q < N_MATQ; ++q) {
+            m[q] = __ballot(cls == q);
+            if (lane == 0) s_n[q][wv] = (int)__popcll(m[q]);
+        }
+        __syncthreads();
+        if (cls >= 0) {
+            int off = s_base[cls];
+            for (int w = 0; w < wv; ++w) off += s_n[cls][w];
+            Wv.mq[cls][off + (int)lanes_below(m[cls])] = p;
+        }
+        __syncthreads();  // every thread has read s_base before it advances
+        if ((int)threadIdx.x < N_MATQ) {
+            const int q = (int)threadIdx.x;
+            int tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) tot += s_n[q][w];
+            s_base[q] += tot;
+        }
+        __syncthreads();  // s_n and s_base are rewritten by the next round
     }
 }
 
