@@ -43,9 +43,13 @@ FP32_PEAK_TFLOPS = 157.3
 # the BVH and scene tables that stay in L2. A kernel's bytes per launch are the
 # sum over its terms of (bytes per unit x units traced in the timed region)
 # divided by its launches.
-#   extension ray (segment), in k_ext_trace / k_ext_intersect:
+#   extension ray (segment), in k_ext_trace (which also classifies):
 #     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8
 #     + primitive material/light lookup 8 + material-queue entry 4      = 48 B
+#   in k_ext_intersect (hit only; k_ext_classify appends the material queues):
+#     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8    = 36 B
+#   in k_ext_classify: queue index 4 + hit 8 + material/light lookup 8
+#     + material-queue entry 4                                          = 24 B
 #   shadow ray, in k_ext_trace / k_ext_shadow:
 #     record (origin 12, direction 12, tmax 4, XYZ 12, path 4) 44 + path XYZ read+write 24 = 68 B
 #   Lambert vertex, in k_ext_shade_lambert:
@@ -53,7 +57,8 @@ FP32_PEAK_TFLOPS = 157.3
 #     + writes: ray 24, throughput 16, pdf 4, flags 4, shadow record 44, next-queue 4 = 160 B
 BYTES_PER_UNIT = {
     "k_ext_trace": [(48, "segments"), (68, "shadow_segments")],
-    "k_ext_intersect": [(48, "segments")],
+    "k_ext_intersect": [(36, "segments")],
+    "k_ext_classify": [(24, "segments")],
     "k_ext_shadow": [(68, "shadow_segments")],
     "k_ext_shade_lambert": [(160, "lambert_hits")],
 }
@@ -116,7 +121,7 @@ def parse_args():
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--bvh-leaf", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
-    ap.add_argument("--traversal", type=int, default=0, help="0 one ray per lane, 1 persistent lanes refilled from the queue")
+    ap.add_argument("--traversal", type=int, default=0, help="0 default (persistent), 1 persistent lanes refilled from the queue, 2 one ray per lane")
     ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane")
     ap.add_argument("--march-blocks", type=int, default=0, help="blocks per CU of the persistent march (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")

@@ -130,7 +130,9 @@ typedef struct pt_config {
     int32_t bvh_width;     /* 2, 4 or 8 (0 = 2) */
     int32_t sdf_max_steps; /* sphere-tracing budget of the extended mode */
     int64_t hbm_budget;    /* bytes this context may allocate (0 = no limit) */
-    int32_t traversal;     /* extended mode: 0 = one ray per lane, 1 = persistent lanes refilled from the queue */
+    int32_t traversal;     /* extended mode, single-scene dispatches: 0 = the default (1), 1 = persistent lanes
+                              refilled from the queue, 2 = one ray per lane. Batched waves and scenes with a
+                              parallelogram BVH take one ray per lane. */
     int32_t bvh_leaf;      /* most primitives per BVH leaf, 1..16 (0 = 1) */
     int32_t graph_mode;    /* batched serving waves: 0 = captured once per launch shape into a hipGraph and
                               replayed, 1 = kernels launched one by one */

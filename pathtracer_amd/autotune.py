@@ -5,7 +5,7 @@ segments per second of short renders under candidate pt_config values and
 returns the fastest configuration:
 
   bvh_width x bvh_leaf   BVH arity (2, 4, 8) and primitives per leaf (1, 2, 4)
-  traversal              one ray per lane, or persistent lanes refilled from the queue
+  traversal              persistent lanes refilled from the queue (1), or one ray per lane (2)
   wave_paths             paths in flight per wave (queue capacity); each
                          candidate is measured on dispatches that fill its wave
   blocks_per_cu          grid shape of the queue kernels (blocks of 256 lanes per CU)
@@ -50,7 +50,7 @@ SDF_BUDGETS = (32, 48, 64, 96, 128, 192, 256)
 class Choice:
     bvh_width: int = 2
     bvh_leaf: int = 1
-    traversal: int = 0
+    traversal: int = 1
     wave_paths: int = 1 << 25
     blocks_per_cu: int = 8
     march_blocks_per_cu: int = 8
@@ -148,7 +148,7 @@ def autotune(text: str, width: int, height: int, depth: int, spp: int = 0, repea
 
     descend("bvh_width", WIDTHS)
     descend("bvh_leaf", LEAVES)
-    descend("traversal", (0, 1))
+    descend("traversal", (1, 2))
     descend("wave_paths", WAVES)
     descend("blocks_per_cu", BLOCKS_PER_CU)
     if has_sdf:

@@ -83,7 +83,7 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
     if (ctx->cfg.wave_paths < 0 || (ctx->cfg.block_size != 0 && ctx->cfg.block_size != 256) ||
         ctx->cfg.blocks_per_cu < 0 || ctx->cfg.blocks_per_cu > 64 || ctx->cfg.sdf_max_steps < 0 ||
         (ctx->cfg.bvh_width != 0 && ctx->cfg.bvh_width != 2 && ctx->cfg.bvh_width != 4 && ctx->cfg.bvh_width != 8) ||
-        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 1 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16 ||
+        ctx->cfg.traversal < 0 || ctx->cfg.traversal > 2 || ctx->cfg.bvh_leaf < 0 || ctx->cfg.bvh_leaf > 16 ||
         ctx->cfg.graph_mode < 0 || ctx->cfg.graph_mode > 1 || ctx->cfg.march < 0 || ctx->cfg.march > 1 ||
         ctx->cfg.march_blocks_per_cu < 0 || ctx->cfg.march_blocks_per_cu > 64) {
         delete ctx;

@@ -41,7 +41,7 @@ def test_sdf_budget_choice():
     text = scenes.sdf_scene(seed=1)
     res = A.autotune(text, 64, 48, 6, spp=16, repeats=1, sdf_tolerance=1e-4)
     fr = {t.params["sdf_max_steps"]: t.sdf_exhausted_frac for t in res.trials[:len(A.SDF_BUDGETS)]
-          if t.params["bvh_width"] == 2 and t.params["bvh_leaf"] == 1 and t.params["traversal"] == 0}
+          if t.params["bvh_width"] == 2 and t.params["bvh_leaf"] == 1 and t.params["traversal"] == 1}
     print(f"\nexhausted fraction by budget: {fr}; chosen {res.best.sdf_max_steps}")
     b = res.best.sdf_max_steps
     assert fr[b] <= 1e-4 or b == A.SDF_BUDGETS[-1]

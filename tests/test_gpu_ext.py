@@ -282,7 +282,7 @@ def test_persistent_traversal_is_invisible(scene_texts, name):
         "cornell": (scenes.cornell(4, seed=7), 8),
     }[name]
     imgs, cnts = [], []
-    for tr in (0, 1):
+    for tr in (2, 1):  # one ray per lane, persistent
         with pt.Context(0, traversal=tr) as c:
             imgs.append(gpu_render(c, text, 64, 36, 4, depth))
             cnts.append(c.counters())
