@@ -3,8 +3,8 @@
 Workloads (--workload):
   spheres (default)  Cornell box plus 4096 random spheres (60 % Lambert, 25 %
                      dispersive dielectric, 15 % conductor), 1920x1080, path
-                     depth 12, 4 hero wavelengths, 16 spp per step (one wave
-                     of 33 M paths).
+                     depth 12, 4 hero wavelengths, 32 spp per step (one wave
+                     of 66 M paths; 32 steps are the 1024-spp render).
   cornell            the Cornell box alone (5 quads, a Lambert and a Cauchy
                      dielectric sphere, 6500 K quad light), 512x512, depth 8,
                      one wavelength per path, 64 spp per step: a step is the
@@ -12,7 +12,7 @@ Workloads (--workload):
   sdf                power-8 mandelbulb (8 iterations) and level-4 menger
                      sponge over a ground plane under a uniform white sky,
                      sphere tracing of at most 256 steps to 1e-4, 1920x1080,
-                     depth 8, 4 hero wavelengths, 16 spp per step.
+                     depth 8, 4 hero wavelengths, 32 spp per step.
 A step is one dispatch tracing `--spp` samples of every pixel (one wave of
 paths); the timed region is exactly `--steps` dispatches bracketed by stream
 synchronisation, with the scene, BVH and path buffers resident in HBM.
@@ -90,13 +90,13 @@ def workload(args):
                 "A=1.5 B=0.01, 6500 K quad light), {w}x{h}, depth {d}, 1 wavelength per path in 380-780 nm")
     elif args.workload == "sdf":
         text = scenes.sdf_scene(seed=0)
-        dflt = (1920, 1080, 8, 16)
+        dflt = (1920, 1080, 8, 32)
         what = ("mandelbulb (power 8, 8 iterations, bailout 16) + level-4 menger sponge over a ground plane, "
                 "uniform white environment, sphere tracing <= 256 steps to 1e-4, {w}x{h}, depth {d}, "
                 "4 hero wavelengths")
     else:
         text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
-        dflt = (1920, 1080, 12, 16)
+        dflt = (1920, 1080, 12, 32)
         what = ("Cornell box + " + str(args.spheres) + " random spheres (60% Lambert / 25% dielectric / "
                 "15% conductor), {w}x{h}, depth {d}, 4 hero wavelengths")
     w = args.width or dflt[0]
