@@ -177,7 +177,8 @@ def main(argv=None) -> int:
         text = scenes.random_spheres(n=a.spheres)
     r = autotune(text, a.width, a.height, a.depth, a.spp, a.repeats, log=lambda s: print(s, flush=True))
     out = r.as_dict()
-    print(json.dumps({"best": out["best"], "segments_per_s": max(t.segments_per_s for t in r.trials)}))
+    chosen = [t.segments_per_s for t in r.trials if t.params == out["best"]]
+    print(json.dumps({"best": out["best"], "segments_per_s": chosen[0] if chosen else None}))
     if a.out:
         with open(a.out, "w") as f:
             json.dump(out, f, indent=1)
