@@ -5,6 +5,12 @@ Workloads (--workload):
                      dispersive dielectric, 15 % conductor), 1920x1080, path
                      depth 12, 4 hero wavelengths, 32 spp per step (one wave
                      of 66 M paths; 32 steps are the 1024-spp render).
+  ref                the reference's own scene file scene1.json (3 spheres, a
+                     plane, 2 rotated boxes, the BK7 camera lens) in reference
+                     mode: the shader's primitives, sequential PCG chain, 4
+                     rotated wavelengths and quirks, bit-identical to the FP32
+                     CPU restatement of the shader, which is timed beside it;
+                     512x512, path length 8, 8 spp per step.
   cornell            the Cornell box alone (5 quads, a Lambert and a Cauchy
                      dielectric sphere, 6500 K quad light), 512x512, depth 8,
                      one wavelength per path, 64 spp per step: a step is the
@@ -61,6 +67,15 @@ BYTES_PER_UNIT = {
     "k_ext_classify": [(24, "segments")],
     "k_ext_shadow": [(68, "shadow_segments")],
     "k_ext_shade_lambert": [(160, "lambert_hits")],
+    # reference mode (SoA scalar records):
+    #   k_ref_intersect: queue index 4 + ray 24 + hit (t, normal, material, light) 24 = 52 B per segment
+    #   k_ref_shade: reads queue 4, hit 24, ray 24, throughput 16, MIS weight 4, seed 4, hero
+    #     wavelength 4; writes ray 24, throughput 16, weight 4, seed 4, next-queue 4 = 132 B per
+    #     segment; + shadow record (path 4, ray 24, light object 4, contribution 16) 48 B per shadow ray
+    #   k_ref_shadow: reads ray 24 + light object 4, writes hit distance 4 + object 4 = 36 B per shadow ray
+    "k_ref_intersect": [(52, "segments")],
+    "k_ref_shade": [(132, "segments"), (48, "shadow_segments")],
+    "k_ref_shadow": [(36, "shadow_segments")],
 }
 
 # Sphere-tracing kernels are bound by FP32 VALU work, not HBM. Their algorithmic
@@ -83,7 +98,12 @@ FLOPS_COUNTER = {
 def workload(args):
     """(scene JSON, width, height, depth, spp per step, description) of --workload."""
     from pathtracer_amd import scenes
-    if args.workload == "cornell":
+    if args.workload == "ref":
+        text = (ROOT / "tests" / "golden" / "scenes" / "scene1.json").read_text()
+        dflt = (512, 512, 8, 8)
+        what = ("the reference's scene1.json (3 spheres, plane, 2 rotated boxes, BK7 lens) in reference mode "
+                "(the shader's method, 4 rotated wavelengths), {w}x{h}, path length {d}")
+    elif args.workload == "cornell":
         text = scenes.cornell(wavelengths=1, seed=0)
         dflt = (512, 512, 8, 64)
         what = ("Cornell box (5 quads with 3-Gaussian spectra, Lambert sphere, Cauchy dielectric sphere "
@@ -111,7 +131,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=32)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("spheres", "cornell", "sdf"), default="spheres",
+    ap.add_argument("--workload", choices=("spheres", "cornell", "sdf", "ref"), default="spheres",
                     help="scene of the tracer bench (module docstring)")
     ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (default: the workload's)")
     ap.add_argument("--width", type=int, default=0)
@@ -215,6 +235,60 @@ def cpu_baseline(args, scene_text: str, W: int, H: int, depth: int, full_spp: in
         "kind": "port",
         "sample": f"FP64 CPU oracle with its own BVH, the full {W}x{H} frame x {spp} spp, "
                   f"depth {depth}: {run['segments']} segments in {run['seconds']:.2f} s",
+    }
+
+
+def ref_cpu_baseline(args, ctx, text: str, W: int, H: int, depth: int):
+    """The FP32 CPU restatement of the reference shader (oracle/ptref.c) on the
+    same scene in reference mode, on at most 16 host threads: one dispatch of the
+    number of samples per pixel that takes about --cpu-seconds. Also the
+    per-pixel parity of a 2-spp frame between it and the GPU."""
+    import numpy as np
+    import pathtracer_amd as pt
+    try:
+        from oracle import ptref
+    except Exception as e:  # oracle not built
+        return {"value": None, "unit": "segments/s", "cores": 0, "kind": "port", "sample": f"unavailable: {e}"}
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+
+    def packed(spf):
+        pc = pt.push_constants(W, H, spf=spf, path_length=depth)
+        ubo, pc, sdfs = pt.pack_json(text, 1, pc)
+        pc.frame = pc.currentSamples = spf
+        return ubo, pc, sdfs
+
+    # parity: the same 2-spp frame on the GPU and the CPU
+    ubo, pc, sdfs = packed(2)
+    sc = ctx.scene_from_json(text)
+    acc = ctx.accum(W, H)
+    ctx.dispatch(sc, acc, pc, pt.MODE_REF)
+    gpu = acc.read().copy()
+    acc.release()
+    sc.release()
+    cpu = np.zeros((H, W, 4), np.float32)
+    ptref.dispatch(bytes(ubo), bytes(pc), b"", 0, cpu, threads=threads)
+    g, c = gpu[..., :3].astype(np.float64), cpu[..., :3].astype(np.float64)
+    rel = np.linalg.norm(g - c, axis=-1) / np.maximum(np.linalg.norm(c, axis=-1), 1e-12)
+    within = float(np.mean(rel <= 1e-3))
+    exact = float(np.mean(np.all(gpu[..., :3] == cpu[..., :3], axis=-1)))
+    # timing: one spp, then the sample
+    ubo, pc, sdfs = packed(1)
+    t0 = time.perf_counter()
+    ptref.dispatch(bytes(ubo), bytes(pc), b"", 0, np.zeros((H, W, 4), np.float32), threads=threads)
+    t1 = time.perf_counter() - t0
+    spf = max(1, int(round(args.cpu_seconds / max(t1, 1e-9))))
+    ubo, pc, sdfs = packed(spf)
+    t0 = time.perf_counter()
+    cnt = ptref.dispatch(bytes(ubo), bytes(pc), b"", 0, np.zeros((H, W, 4), np.float32), threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": cnt["segments"] / dt,
+        "unit": "segments/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"FP32 CPU restatement of the reference shader (oracle/ptref.c), the full {W}x{H} frame x {spf} spp, "
+                  f"path length {depth}: {cnt['segments']} segments in {dt:.2f} s; per-pixel parity with the GPU on a "
+                  f"2-spp frame: {within * 100:.3f}% of pixels within 1e-3, {exact * 100:.3f}% bit-identical",
     }
 
 
@@ -617,11 +691,13 @@ def trace_bench(args):
 
     text, W, H, depth, spp, what = workload(args)
     # name of this workload's profile summaries under profiles/
-    tag = {"spheres": f"spheres{args.spheres}_{W}x{H}", "sdf": f"sdf_{W}x{H}", "cornell": f"cornell_{W}x{H}"}[args.workload]
+    tag = {"spheres": f"spheres{args.spheres}_{W}x{H}", "sdf": f"sdf_{W}x{H}", "cornell": f"cornell_{W}x{H}",
+           "ref": f"ref_scene1_{W}x{H}"}[args.workload]
+    mode = pt.MODE_REF if args.workload == "ref" else pt.MODE_EXT
     ctx = pt.Context(0, wave_paths=args.wave_paths, bvh_width=args.bvh_width, traversal=args.traversal,
                      bvh_leaf=args.bvh_leaf, march=args.march, march_blocks_per_cu=args.march_blocks)
     scene = ctx.scene_from_json(text)
-    r = pt.Renderer(ctx, scene, W, H, spf=spp, path_length=depth, mode=pt.MODE_EXT)
+    r = pt.Renderer(ctx, scene, W, H, spf=spp, path_length=depth, mode=mode)
     for _ in range(args.warmup):
         r.render_frame()
     ctx.sync()
@@ -688,7 +764,12 @@ def trace_bench(args):
                 "algorithmic_bytes_per_launch": round(total_bytes / launches),
             }
     full = spp if args.workload == "cornell" else 0
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args, text, W, H, depth, full_spp=full)
+    if args.no_cpu_baseline:
+        cpu = None
+    elif args.workload == "ref":
+        cpu = ref_cpu_baseline(args, ctx, text, W, H, depth)
+    else:
+        cpu = cpu_baseline(args, text, W, H, depth, full_spp=full)
     config = {
         "workload": what,
         "spp_per_step": spp,
@@ -697,7 +778,9 @@ def trace_bench(args):
         "shadow_segments": cnt["shadow_segments"],
         "segments_per_path": cnt["segments"] / max(1, cnt["paths"]),
     }
-    if args.workload == "sdf":
+    if args.workload == "ref":
+        pass
+    elif args.workload == "sdf":
         config["sdf_steps_per_segment"] = cnt["sdf_steps"] / max(1, cnt["segments"] + cnt["shadow_segments"])
         config["sdf_exhausted"] = cnt["sdf_exhausted"]
         config["march_tflops"] = {k: round(pcnt[v] / 1e12, 4) for k, v in FLOPS_COUNTER.items() if pcnt.get(v)}
