@@ -160,6 +160,8 @@ def parse_args():
     ap.add_argument("--serve-wave", type=int, default=0,
                     help="paths per batched wave of the server's context, both systems (0 = the library's 2^22)")
     ap.add_argument("--serve-max-active", type=int, default=0, help="requests the SRPT server traces together (0 = 16)")
+    ap.add_argument("--serve-profile", action="store_true",
+                    help="GPU time by kernel family in both systems (HIP events per launch, graphs off)")
     ap.add_argument("--serve-denoise", action="store_true",
                     help="the SRPT server may finish requests with the shipped denoiser's image")
     ap.add_argument("--serve-speedup", type=str, default="1",
@@ -323,8 +325,26 @@ def serve_bench(args):
     def submit(srv, r, t):
         return srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target, t_arrive=t)
 
+    def kernel_split(prof):
+        """GPU ms by kernel family from pt_profile_read totals (--serve-profile)."""
+        fam = {"trace": 0.0, "denoise": 0.0, "moments+estimate": 0.0, "other": 0.0}
+        for name, (_, ms) in prof.items():
+            if name.startswith(("k_ext_moments", "k_ext_rel", "k_ext_features", "k_dn_assemble", "k_dn_cross",
+                                "k_dn_scale")):
+                fam["moments+estimate"] += ms
+            elif name.startswith("k_ext_") or name.startswith("k_bump"):
+                fam["trace"] += ms
+            elif name.startswith(("k_conv", "k_pool", "k_up", "k_relu", "k_prep", "k_head", "k_dn", "k_wcopy")):
+                fam["denoise"] += ms
+            else:
+                fam["other"] += ms
+        return {k: round(v, 1) for k, v in fam.items()}
+
     # 1. service times alone
     service = {}
+    if args.serve_profile:
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
     with pt.Server(ctx, policy=pt.SERVE_FIFO, max_active=1) as srv:
         ids = {submit(srv, r, 0.0): r["id"] for r in reqs}
         while len(service) < len(reqs):
@@ -339,6 +359,7 @@ def serve_bench(args):
                         print(f"one at a time: {len(service)}/{len(reqs)} done "
                               f"({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
     s_alone = np.array([service[r["id"]]["gpu_seconds"] for r in reqs])
+    alone_kernels = kernel_split(ctx.profile_read(reset=True)) if args.serve_profile else None
     alone_converged = sum(service[r["id"]]["converged"] for r in reqs)
 
     # 2. arrivals
@@ -357,6 +378,8 @@ def serve_bench(args):
 
     def srpt_run(speedup):
         done, segments = {}, 0
+        if args.serve_profile:
+            ctx.profile_read(reset=True)
         with pt.Server(ctx, policy=pt.SERVE_SRPT, max_active=args.serve_max_active) as srv:
             if dn is not None:
                 srv.set_denoiser(dn)
@@ -393,7 +416,8 @@ def serve_bench(args):
                 "segments_per_s": segments / makespan,
                 "denoised": int(sum(done[r["id"]]["denoised"] for r in reqs)),
                 "mean_spp": float(np.mean([done[r["id"]]["spp"] for r in reqs])),
-                "gpu_seconds": float(sum(done[r["id"]]["gpu_seconds"] for r in reqs))}
+                "gpu_seconds": float(sum(done[r["id"]]["gpu_seconds"] for r in reqs)),
+                "kernel_ms": kernel_split(ctx.profile_read(reset=True)) if args.serve_profile else None}
 
     runs = []
     for k in args.serve_speedup.split(","):
@@ -437,7 +461,7 @@ def serve_bench(args):
                           "p50_response_s": percentile(fifo_resp, 50), "p99_response_s": percentile(fifo_resp, 99),
                           "mean_service_s": float(s_alone.mean()),
                           "mean_spp": float(np.mean([service[r["id"]]["spp"] for r in reqs])),
-                          "gpu_seconds": float(s_alone.sum())},
+                          "gpu_seconds": float(s_alone.sum()), "kernel_ms": alone_kernels},
         "bench_seconds": time.perf_counter() - t_start,
     }
     print(json.dumps(line))
