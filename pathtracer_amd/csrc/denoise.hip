@@ -254,59 +254,81 @@ __global__ void __launch_bounds__(kThreads) k_conv_wgrad(WgradArgs a) {
 }
 
 // ---------------------------------------------------------------------------------
-// Elementwise kernels (NHWC BF16).
+// Elementwise kernels (NHWC BF16). Pooling and upsampling move 8 channels (one
+// 16-byte vector) per thread: C is a multiple of 8, and the tensors hold fewer
+// than 2^31 such groups (the denoiser's size limits), so index arithmetic is
+// 32-bit. Each channel's arithmetic is the one-channel formula, in the same order.
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+__device__ __forceinline__ uint32_t bf_pack(float lo, float hi) {
+    return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+}
+// per channel: scale * (((a + b) + c) + d)
+__device__ __forceinline__ uint32_t sum4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, float scale) {
+    return bf_pack(scale * (((bf_lo(a) + bf_lo(b)) + bf_lo(c)) + bf_lo(d)),
+                   scale * (((bf_hi(a) + bf_hi(b)) + bf_hi(c)) + bf_hi(d)));
+}
+__device__ __forceinline__ uint4 sum4(uint4 a, uint4 b, uint4 c, uint4 d, float scale) {
+    return make_uint4(sum4(a.x, b.x, c.x, d.x, scale), sum4(a.y, b.y, c.y, d.y, scale),
+                      sum4(a.z, b.z, c.z, d.z, scale), sum4(a.w, b.w, c.w, d.w, scale));
+}
+__device__ __forceinline__ uint32_t scale2(uint32_t a, float scale) {
+    return bf_pack(scale * bf_lo(a), scale * bf_hi(a));
+}
+
 __global__ void k_pool2(const u16* x, u16* y, int N, int H, int W, int C) {  // 2x2 average
-    const int Ho = H / 2, Wo = W / 2;
-    const size_t n = (size_t)N * Ho * Wo * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        size_t r = i / C;
-        const int xo = (int)(r % Wo);
-        r /= Wo;
-        const int yo = (int)(r % Ho), b = (int)(r / Ho);
-        const size_t q = ((size_t)b * H + 2 * yo) * W + 2 * xo;
-        float s = bf2f(x[q * C + c]) + bf2f(x[(q + 1) * C + c]) + bf2f(x[(q + W) * C + c]) + bf2f(x[(q + W + 1) * C + c]);
-        y[i] = f2bf(0.25f * s);
+    const uint32_t Ho = H / 2, Wo = W / 2, C8 = C >> 3;
+    const uint32_t n = (uint32_t)N * Ho * Wo * C8;
+    const uint4* xv = reinterpret_cast<const uint4*>(x);
+    uint4* yv = reinterpret_cast<uint4*>(y);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % C8, r = i / C8;
+        const uint32_t xo = r % Wo, r2 = r / Wo;
+        const uint32_t yo = r2 % Ho, b = r2 / Ho;
+        const uint32_t q = (b * (uint32_t)H + 2 * yo) * (uint32_t)W + 2 * xo;
+        yv[i] = sum4(xv[q * C8 + g], xv[(q + 1) * C8 + g], xv[(q + W) * C8 + g], xv[(q + W + 1) * C8 + g], 0.25f);
     }
 }
-// gradient of the 2x2 average: dX[q] += 0.25 dY (dX written, not accumulated)
+// gradient of the 2x2 average: dX[q] = 0.25 dY (dX written, not accumulated)
 __global__ void k_pool2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) {
-    const size_t n = (size_t)N * H * W * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        size_t r = i / C;
-        const int x = (int)(r % W);
-        r /= W;
-        const int y = (int)(r % H), b = (int)(r / H);
-        const size_t o = (((size_t)b * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c;
-        dx[i] = f2bf(0.25f * bf2f(dy[o]));
+    const uint32_t C8 = C >> 3;
+    const uint32_t n = (uint32_t)N * H * W * C8;
+    const uint4* dyv = reinterpret_cast<const uint4*>(dy);
+    uint4* dxv = reinterpret_cast<uint4*>(dx);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % C8, r = i / C8;
+        const uint32_t x = r % W, r2 = r / W;
+        const uint32_t y = r2 % H, b = r2 / H;
+        const uint4 v = dyv[((b * (uint32_t)(H / 2) + y / 2) * (uint32_t)(W / 2) + x / 2) * C8 + g];
+        dxv[i] = make_uint4(scale2(v.x, 0.25f), scale2(v.y, 0.25f), scale2(v.z, 0.25f), scale2(v.w, 0.25f));
     }
 }
 // nearest 2x upsample: y [N, 2H, 2W, C] from x [N, H, W, C]
 __global__ void k_up2(const u16* x, u16* y, int N, int H, int W, int C) {
-    const size_t n = (size_t)N * 2 * H * 2 * W * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        size_t r = i / C;
-        const int xo = (int)(r % (2 * W));
-        r /= 2 * W;
-        const int yo = (int)(r % (2 * H)), b = (int)(r / (2 * H));
-        y[i] = x[(((size_t)b * H + yo / 2) * W + xo / 2) * C + c];
+    const uint32_t C8 = C >> 3, Ho = 2 * H, Wo = 2 * W;
+    const uint32_t n = (uint32_t)N * Ho * Wo * C8;
+    const uint4* xv = reinterpret_cast<const uint4*>(x);
+    uint4* yv = reinterpret_cast<uint4*>(y);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % C8, r = i / C8;
+        const uint32_t xo = r % Wo, r2 = r / Wo;
+        const uint32_t yo = r2 % Ho, b = r2 / Ho;
+        yv[i] = xv[((b * (uint32_t)H + yo / 2) * (uint32_t)W + xo / 2) * C8 + g];
     }
 }
 // gradient of the upsample: dX = sum of the 4 dY it fed
 __global__ void k_up2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) {
-    const size_t n = (size_t)N * H * W * C;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        size_t r = i / C;
-        const int x = (int)(r % W);
-        r /= W;
-        const int y = (int)(r % H), b = (int)(r / H);
-        const size_t q = ((size_t)b * 2 * H + 2 * y) * 2 * W + 2 * x;
-        float s = bf2f(dy[q * C + c]) + bf2f(dy[(q + 1) * C + c]) + bf2f(dy[(q + 2 * W) * C + c]) +
-                  bf2f(dy[(q + 2 * W + 1) * C + c]);
-        dx[i] = f2bf(s);
+    const uint32_t C8 = C >> 3;
+    const uint32_t n = (uint32_t)N * H * W * C8;
+    const uint4* dyv = reinterpret_cast<const uint4*>(dy);
+    uint4* dxv = reinterpret_cast<uint4*>(dx);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t g = i % C8, r = i / C8;
+        const uint32_t x = r % W, r2 = r / W;
+        const uint32_t y = r2 % H, b = r2 / H;
+        const uint32_t q = (b * 2 * (uint32_t)H + 2 * y) * 2 * (uint32_t)W + 2 * x;
+        dxv[i] = sum4(dyv[q * C8 + g], dyv[(q + 1) * C8 + g], dyv[(q + 2 * W) * C8 + g],
+                      dyv[(q + 2 * W + 1) * C8 + g], 1.0f);
     }
 }
 // dZ = dY * (Y > 0): gradient through the ReLU that produced Y; a second
@@ -560,16 +582,16 @@ void forward(pt_denoiser* d, int N, int H, int W) {
     ptk::launch(ctx, "k_dn_prep", k_prep, ew_grid(P), dim3(256), (const float*)d->feat, d->x0, (size_t)P);
     conv(d, d->L[0], d->x0, nullptr, d->e0a, N, H, W, false);
     conv(d, d->L[1], d->e0a, nullptr, d->e0, N, H, W, false);
-    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P1 * 32), dim3(256), (const u16*)d->e0, d->p1, N, H, W, 32);
+    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P1 * 32 / 8), dim3(256), (const u16*)d->e0, d->p1, N, H, W, 32);
     conv(d, d->L[2], d->p1, nullptr, d->e1a, N, H1, W1, false);
     conv(d, d->L[3], d->e1a, nullptr, d->e1, N, H1, W1, false);
-    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P2 * 64), dim3(256), (const u16*)d->e1, d->p2, N, H1, W1, 64);
+    ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P2 * 64 / 8), dim3(256), (const u16*)d->e1, d->p2, N, H1, W1, 64);
     conv(d, d->L[4], d->p2, nullptr, d->ba, N, H2, W2, false);
     conv(d, d->L[5], d->ba, nullptr, d->bb, N, H2, W2, false);
-    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P1 * 128), dim3(256), (const u16*)d->bb, d->u1, N, H2, W2, 128);
+    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P1 * 128 / 8), dim3(256), (const u16*)d->bb, d->u1, N, H2, W2, 128);
     conv(d, d->L[6], d->u1, d->e1, d->d1a, N, H1, W1, false);
     conv(d, d->L[7], d->d1a, nullptr, d->d1, N, H1, W1, false);
-    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P * 64), dim3(256), (const u16*)d->d1, d->u0, N, H1, W1, 64);
+    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P * 64 / 8), dim3(256), (const u16*)d->d1, d->u0, N, H1, W1, 64);
     conv(d, d->L[8], d->u0, d->e0, d->d0a, N, H, W, false);
     conv(d, d->L[9], d->d0a, nullptr, d->d0, N, H, W, false);
     conv(d, d->L[10], d->d0, nullptr, d->out, N, H, W, false);
@@ -592,7 +614,7 @@ void backward(pt_denoiser* d, int N, int H, int W) {
     relu_bwd(d->ga, nullptr, d->d0a, d->gb, (size_t)P * 32);           // dZ8
     wgrad(d, d->L[8], d->gb, d->u0, d->e0, N, H, W);
     conv(d, d->L[8], d->gb, nullptr, d->gc, N, H, W, true, d->gs0, 64);  // dL/d u0 (64) | dL/d e0 via skip (32)
-    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P1 * 64), dim3(256), (const u16*)d->gc, d->ga, N, H1, W1,
+    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P1 * 64 / 8), dim3(256), (const u16*)d->gc, d->ga, N, H1, W1,
                 64);                                                   // dL/d d1 (64)
     relu_bwd(d->ga, nullptr, d->d1, d->gb, (size_t)P1 * 64);           // dZ7
     wgrad(d, d->L[7], d->gb, d->d1a, nullptr, N, H1, W1);
@@ -600,7 +622,7 @@ void backward(pt_denoiser* d, int N, int H, int W) {
     relu_bwd(d->ga, nullptr, d->d1a, d->gb, (size_t)P1 * 64);          // dZ6
     wgrad(d, d->L[6], d->gb, d->u1, d->e1, N, H1, W1);
     conv(d, d->L[6], d->gb, nullptr, d->gc, N, H1, W1, true, d->gs1, 128);  // dL/d u1 (128) | dL/d e1 via skip (64)
-    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P2 * 128), dim3(256), (const u16*)d->gc, d->ga, N, H2,
+    ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P2 * 128 / 8), dim3(256), (const u16*)d->gc, d->ga, N, H2,
                 W2, 128);                                              // dL/d bb (128)
     relu_bwd(d->ga, nullptr, d->bb, d->gb, (size_t)P2 * 128);          // dZ5
     wgrad(d, d->L[5], d->gb, d->ba, nullptr, N, H2, W2);
@@ -608,7 +630,7 @@ void backward(pt_denoiser* d, int N, int H, int W) {
     relu_bwd(d->ga, nullptr, d->ba, d->gb, (size_t)P2 * 128);          // dZ4
     wgrad(d, d->L[4], d->gb, d->p2, nullptr, N, H2, W2);
     conv(d, d->L[4], d->gb, nullptr, d->ga, N, H2, W2, true);         // dL/d p2 (64)
-    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P1 * 64), dim3(256), (const u16*)d->ga, d->gc, N, H1,
+    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P1 * 64 / 8), dim3(256), (const u16*)d->ga, d->gc, N, H1,
                 W1, 64);                                               // dL/d e1 via pool (64)
     relu_bwd(d->gc, d->gs1, d->e1, d->gb, (size_t)P1 * 64);            // dZ3 (pool + skip)
     wgrad(d, d->L[3], d->gb, d->e1a, nullptr, N, H1, W1);
@@ -616,7 +638,7 @@ void backward(pt_denoiser* d, int N, int H, int W) {
     relu_bwd(d->ga, nullptr, d->e1a, d->gb, (size_t)P1 * 64);          // dZ2
     wgrad(d, d->L[2], d->gb, d->p1, nullptr, N, H1, W1);
     conv(d, d->L[2], d->gb, nullptr, d->ga, N, H1, W1, true);         // dL/d p1 (32)
-    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P * 32), dim3(256), (const u16*)d->ga, d->gc, N, H, W,
+    ptk::launch(ctx, "k_dn_pool_bwd", k_pool2_bwd, ew_grid((size_t)P * 32 / 8), dim3(256), (const u16*)d->ga, d->gc, N, H, W,
                 32);                                                   // dL/d e0 via pool (32)
     relu_bwd(d->gc, d->gs0, d->e0, d->gb, (size_t)P * 32);             // dZ1 (pool + skip)
     wgrad(d, d->L[1], d->gb, d->e0a, nullptr, N, H, W);
