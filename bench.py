@@ -3,8 +3,8 @@
 Workloads (--workload):
   spheres (default)  Cornell box plus 4096 random spheres (60 % Lambert, 25 %
                      dispersive dielectric, 15 % conductor), 1920x1080, path
-                     depth 12, 4 hero wavelengths, 32 spp per step (one wave
-                     of 66 M paths; 32 steps are the 1024-spp render).
+                     depth 12, 4 hero wavelengths, 64 spp per step (one wave
+                     of 133 M paths; 16 steps are the 1024-spp render).
   ref                the reference's own scene file scene1.json (3 spheres, a
                      plane, 2 rotated boxes, the BK7 camera lens) in reference
                      mode: the shader's primitives, sequential PCG chain, 4
@@ -18,7 +18,7 @@ Workloads (--workload):
   sdf                power-8 mandelbulb (8 iterations) and level-4 menger
                      sponge over a ground plane under a uniform white sky,
                      sphere tracing of at most 256 steps to 1e-4, 1920x1080,
-                     depth 8, 4 hero wavelengths, 32 spp per step.
+                     depth 8, 4 hero wavelengths, 64 spp per step.
 A step is one dispatch tracing `--spp` samples of every pixel (one wave of
 paths); the timed region is exactly `--steps` dispatches bracketed by stream
 synchronisation, with the scene, BVH and path buffers resident in HBM.
@@ -110,13 +110,13 @@ def workload(args):
                 "A=1.5 B=0.01, 6500 K quad light), {w}x{h}, depth {d}, 1 wavelength per path in 380-780 nm")
     elif args.workload == "sdf":
         text = scenes.sdf_scene(seed=0)
-        dflt = (1920, 1080, 8, 32)
+        dflt = (1920, 1080, 8, 64)
         what = ("mandelbulb (power 8, 8 iterations, bailout 16) + level-4 menger sponge over a ground plane, "
                 "uniform white environment, sphere tracing <= 256 steps to 1e-4, {w}x{h}, depth {d}, "
                 "4 hero wavelengths")
     else:
         text = scenes.random_spheres(n=args.spheres, scene_seed=1, wavelengths=4, seed=0)
-        dflt = (1920, 1080, 12, 32)
+        dflt = (1920, 1080, 12, 64)
         what = ("Cornell box + " + str(args.spheres) + " random spheres (60% Lambert / 25% dielectric / "
                 "15% conductor), {w}x{h}, depth {d}, 4 hero wavelengths")
     w = args.width or dflt[0]
@@ -129,8 +129,9 @@ def workload(args):
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=16,
+                    help="timed steps (16 steps of the default workload are its whole 1024-spp render)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=("spheres", "cornell", "sdf", "ref"), default="spheres",
                     help="scene of the tracer bench (module docstring)")
     ap.add_argument("--spp", type=int, default=0, help="samples per pixel per step (default: the workload's)")

@@ -40,7 +40,7 @@ from pathtracer_amd import scenes
 
 WIDTHS = (2, 4, 8)
 LEAVES = (1, 2, 4)
-WAVES = (1 << 23, 1 << 24, 1 << 25, 1 << 26)
+WAVES = (1 << 24, 1 << 25, 1 << 26, 1 << 27)
 BLOCKS_PER_CU = (4, 8, 16)
 MARCH_BLOCKS_PER_CU = (2, 4, 8, 16)
 SDF_BUDGETS = (32, 48, 64, 96, 128, 192, 256)
@@ -51,7 +51,7 @@ class Choice:
     bvh_width: int = 2
     bvh_leaf: int = 1
     traversal: int = 1
-    wave_paths: int = 1 << 26
+    wave_paths: int = 1 << 27
     blocks_per_cu: int = 8
     march_blocks_per_cu: int = 8
     sdf_max_steps: int = 256

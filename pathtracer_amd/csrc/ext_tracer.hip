@@ -2403,13 +2403,14 @@ struct ExtSink {
     float* feat = nullptr;  // features sink: sums of XYZ, first-hit features and Y^2, 12 per pixel
 };
 
-// Default paths per wave of a single-scene dispatch: 2^26 paths, 14 GB of wave
+// Default paths per wave of a single-scene dispatch: 2^27 paths, 28 GB of wave
 // state. Each bounce launch ends with the latency chain of its slowest ray (a
 // long BVH traversal or a 256-step march), which a wave of more paths amortises;
 // measured on the 1080p benchmark scenes, 2^22 -> 2^25 paths per wave raised
-// traced segments per second by 1.3x (spheres) and 2.4x (SDFs), and 2^26 by a
-// further 6 % (spheres, persistent traversal).
-constexpr int64_t kDispatchWave = (int64_t)1 << 26;
+// traced segments per second by 1.3x (spheres) and 2.4x (SDFs), 2^26 by a
+// further 6 % (spheres, persistent traversal) and 2^27 by 1.4 % (spheres) and
+// 8.7 % (SDFs).
+constexpr int64_t kDispatchWave = (int64_t)1 << 27;
 
 static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, const ExtSink& sink) {
     int st = ext_scene_build(ctx, const_cast<pt_scene*>(s));

@@ -19,7 +19,7 @@ def fake_measure(optimum, exhausted_at):
 
 
 def test_descent_finds_the_separable_optimum(monkeypatch):
-    opt = {"bvh_width": 4, "bvh_leaf": 2, "traversal": 2, "wave_paths": 1 << 23, "blocks_per_cu": 16}
+    opt = {"bvh_width": 4, "bvh_leaf": 2, "traversal": 2, "wave_paths": 1 << 25, "blocks_per_cu": 16}
     m, calls = fake_measure(opt, {})
     monkeypatch.setattr(A, "measure", m)
     r = A.autotune('{"sphere": []}', 64, 64, 8)
