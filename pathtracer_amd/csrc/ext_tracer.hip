@@ -134,6 +134,7 @@ struct ExtWave {
     float4* s_xyz; // unoccluded contribution
     float4* aov0;  // first hit: albedo (xyz), distance (w)
     float4* aov1;  // first hit: shading normal (xyz)
+    uint32_t* smask;  // SDF scenes: bit k set when the ray enters the box of an SDF of kind k before its analytic hit
     float* sum;
     int* counts;   // N_CNT queue lengths, kCntStride apart
     int* cursor;   // 2 x kSeg persistent-kernel cursors, kCntStride apart
@@ -550,18 +551,36 @@ template <int KIND>
 __device__ __forceinline__ float ext_de(const ExtSdf& s, f3 p, unsigned& it) {
     return ext_de_at(KIND ? KIND : s.kind, s.c, s.iters, s.bailout, p, it);
 }
+// The ray's entry and exit distances of an SDF's box, clipped to (kEps, tmax);
+// false when it does not enter before tmax. Every march and the intersect
+// kernels' SDF box mask use this one test, so they agree bit for bit.
+__device__ __forceinline__ bool sdf_box(const ExtSdf& s, f3 o, f3 d, float tmax, float& tin, float& tout) {
+    f3 inv = mk3(hw_rcp(d.x), hw_rcp(d.y), hw_rcp(d.z));
+    f3 t0 = (s.lo - o) * inv, t1 = (s.hi - o) * inv;
+    f3 a = min3(t0, t1), b = max3(t0, t1);
+    tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
+    tout = fminf(fminf(fminf(b.x, b.y), b.z), tmax);
+    return tin < tout;
+}
+// Bit k: the ray enters the box of an SDF of kind k before tmax (its analytic
+// hit). A march bounded by a nearer hit can only enter fewer boxes.
+__device__ __forceinline__ uint32_t sdf_mask(const ExtSceneDev& S, f3 o, f3 d, float tmax) {
+    uint32_t m = 0;
+    for (int i = 0; i < S.nsdf; ++i) {
+        float tin, tout;
+        if (sdf_box(S.sdf[i], o, d, tmax, tin, tout)) m |= 1u << S.sdf[i].kind;
+    }
+    return m;
+}
+
 // Sphere tracing of one SDF object inside its box, up to tmax. Returns the hit
 // distance or a negative value. Adds its steps to `steps` and their FP32
 // operations (de_step_flops, de_iter_flops) to `flops`.
 template <int KIND>
 __device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_steps, unsigned& steps,
                            unsigned& exhausted, unsigned long long& flops) {
-    f3 inv = mk3(hw_rcp(d.x), hw_rcp(d.y), hw_rcp(d.z));
-    f3 t0 = (s.lo - o) * inv, t1 = (s.hi - o) * inv;
-    f3 a = min3(t0, t1), b = max3(t0, t1);
-    float tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
-    float tout = fminf(fminf(fminf(b.x, b.y), b.z), tmax);
-    if (!(tin < tout)) return -1.0f;
+    float tin, tout;
+    if (!sdf_box(s, o, d, tmax, tin, tout)) return -1.0f;
     float t = tin, res = -1.0f;
     unsigned n = 0, it = 0;
     bool open = true;  // neither a hit nor past the box's exit yet
@@ -870,6 +889,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
             float t = 1e30f;
             scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, visits);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+            if (S.nsdf > 0) Wv.smask[p] = sdf_mask(S, o, d, t);
         }
         if (CLASSIFY) classify_append(S, Wv, active, p, ref);
     }
@@ -1007,6 +1027,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
                 }
             }
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+            if (S.nsdf > 0) Wv.smask[p] = sdf_mask(S, o, d, t);
             p = -1;
         }
     }
@@ -1125,8 +1146,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
         const uint64_t idle = __ballot(p < 0);
         if (!chunk.drained() && __popcll(idle) >= kMarchRefill) {
             const int e = wave_refill(idle, chunk);
-            if (e >= 0) {
-                p = q[e];
+            // rays that enter no box of this kind (the intersect kernel's mask)
+            // go back to idle without loading their ray
+            const int pe = e >= 0 ? q[e] : -1;
+            if (pe >= 0 && ((Wv.smask[pe] >> KIND) & 1u)) {
+                p = pe;
                 const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
                 sdfs = S.sdf;
                 nsdf = S.nsdf;
@@ -1151,12 +1175,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
             for (; ns < nsdf; ++ns) {
                 const ExtSdf& e = sdfs[ns];
                 if (e.kind != KIND) continue;
-                f3 inv = mk3(hw_rcp(d.x), hw_rcp(d.y), hw_rcp(d.z));
-                f3 t0 = (e.lo - o) * inv, t1 = (e.hi - o) * inv;
-                f3 a = min3(t0, t1), b = max3(t0, t1);
-                tin = fmaxf(fmaxf(fmaxf(a.x, a.y), a.z), kEps);
-                tx = fminf(fminf(fminf(b.x, b.y), b.z), t);
-                if (tin < tx) {
+                if (sdf_box(e, o, d, t, tin, tx)) {
                     found = true;
                     break;
                 }
@@ -2195,7 +2214,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     size_t n = (size_t)cap;
     // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
     // shadow org/dir/xyz 48, sums 12
-    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 12;
+    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12;
     // + 256-byte rounding of 22 arrays + the counters and cursors, one line each
     int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
@@ -2208,6 +2227,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
     W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
     W.aov0 = carve_e<float4>(p, n); W.aov1 = carve_e<float4>(p, n);
+    W.smask = carve_e<uint32_t>(p, n);
     W.sum = carve_e<float>(p, 3 * n);
     W.counts = carve_e<int>(p, (size_t)N_CNT * kCntStride);
     W.cursor = carve_e<int>(p, (size_t)N_CUR * kSeg * kCntStride);
