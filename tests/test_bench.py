@@ -1,0 +1,71 @@
+"""bench.py's contract on the CPU: one GPU only, the workloads' shapes, and the
+roofline tables naming kernels and counters that exist, so that a renamed
+launch cannot silently drop the roofline of the dominant kernel."""
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+import bench  # noqa: E402
+import pathtracer_amd as pt  # noqa: E402
+
+
+def args(*argv):
+    old = sys.argv
+    sys.argv = ["bench.py", *argv]
+    try:
+        return bench.parse_args()
+    finally:
+        sys.argv = old
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("workload,shape", [
+    ("spheres", (1920, 1080, 12, 64)),
+    ("sdf", (1920, 1080, 8, 64)),
+    ("cornell", (512, 512, 8, 64)),
+    ("ref", (512, 512, 8, 8)),
+])
+def test_workload_shapes(workload, shape):
+    text, w, h, depth, spp, what = bench.workload(args("--workload", workload))
+    assert (w, h, depth, spp) == shape
+    assert text.lstrip().startswith("{") and f"{w}x{h}" in what
+
+
+def test_default_run_is_the_whole_1024_spp_render():
+    a = args()
+    _, w, h, depth, spp, _ = bench.workload(a)
+    assert a.workload == "spheres" and a.gpus == 1
+    assert a.steps * spp == 1024
+
+
+def launch_names():
+    src = "".join(p.read_text() for p in (ROOT / "pathtracer_amd" / "csrc").glob("*.hip"))
+    return set(re.findall(r'launch(?:_march<[^>]*>)?\(\s*ctx,\s*"(k_\w+)"', src))
+
+
+def test_roofline_tables_name_launched_kernels():
+    names = launch_names()
+    assert "k_ext_intersect" in names and "k_ext_march_mandelbulb" in names
+    for k in list(bench.BYTES_PER_UNIT) + list(bench.FLOPS_COUNTER):
+        assert k in names, k
+
+
+def test_roofline_units_are_counters():
+    fields = {f for f, _ in pt.Counters._fields_}
+    for terms in bench.BYTES_PER_UNIT.values():
+        for bytes_per_unit, unit in terms:
+            assert bytes_per_unit > 0 and unit in fields, unit
+    for counter in bench.FLOPS_COUNTER.values():
+        assert counter in fields, counter
