@@ -374,7 +374,8 @@ def serve_bench(args):
         from pathtracer_amd import denoise_train
         hmax = max(r["height"] for r in reqs)
         wmax = max(r["width"] for r in reqs)
-        dn = pt.Denoiser(ctx, 1, (hmax + 3) // 4 * 4, (wmax + 3) // 4 * 4)
+        # room for a batch of 2: the error estimate's two half images in one pass
+        dn = pt.Denoiser(ctx, 2, (hmax + 3) // 4 * 4, (wmax + 3) // 4 * 4)
         denoise_train.load_weights(dn)
 
     def srpt_run(speedup):

@@ -684,6 +684,17 @@ void dn_half(pt_denoiser* d, const DnRequest& q, int which, float inv_scale) {
     ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(P), dim3(256), (const float*)d->feat, (const u16*)d->out,
                 (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, P, 0);
 }
+// Both halves in one batch of 2: D(even half) into d->den, D(odd half) into
+// d->den + 3 P. Each image's pixels are computed as in a batch of one, so the
+// results equal two dn_half calls; a small image's two passes share the launches.
+void dn_halves(pt_denoiser* d, const DnRequest& q, float inv_scale) {
+    const size_t P = (size_t)q.W * q.H;
+    ptk::launch(d->ctx, "k_dn_assemble", k_dn_assemble, ew_grid(P), dim3(256), q, 1, inv_scale, d->feat);
+    ptk::launch(d->ctx, "k_dn_assemble", k_dn_assemble, ew_grid(P), dim3(256), q, 2, inv_scale, d->feat + P * kFeat);
+    forward(d, 2, q.H, q.W);
+    ptk::launch(d->ctx, "k_dn_head", k_head, ew_grid(2 * P), dim3(256), (const float*)d->feat, (const u16*)d->out,
+                (const float*)nullptr, d->den, (u16*)nullptr, (float*)nullptr, 2 * P, 0);
+}
 }  // namespace
 
 int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, int spp, double* d_lines, double* rel) {
@@ -694,15 +705,24 @@ int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, int spp
     hipError_t e = hipMemsetAsync(d_lines, 0, 2 * 64 * 32 * sizeof(double), ctx->stream);
     if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
     const dim3 grid((unsigned)std::min<size_t>((P + 255) / 256, 2048));
-    dn_half(d, q, 1, inv);  // D(even half) against the odd half
-    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)nullptr, 2,
-                inv, d_lines);
-    // keep D(A) (the training-target buffer is free during inference)
-    e = hipMemcpyAsync(d->tgt, d->den, P * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
-    if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
-    dn_half(d, q, 2, inv);  // D(odd half) against the even half, and against D(even half)
-    ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)d->tgt, 1,
-                inv, d_lines + 64 * 32);
+    if (d->maxN >= 2) {
+        dn_halves(d, q, inv);
+        // D(even half) against the odd half; D(odd half) against the even half and D(even half)
+        ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)nullptr, 2,
+                    inv, d_lines);
+        ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)(d->den + P * 3),
+                    (const float*)d->den, 1, inv, d_lines + 64 * 32);
+    } else {
+        dn_half(d, q, 1, inv);  // D(even half) against the odd half
+        ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)nullptr, 2,
+                    inv, d_lines);
+        // keep D(A) (the training-target buffer is free during inference)
+        e = hipMemcpyAsync(d->tgt, d->den, P * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
+        if (e != hipSuccess) return dn_fail(d, ptk::hip_fail(ctx, e, "denoiser estimate"), ctx->err);
+        dn_half(d, q, 2, inv);  // D(odd half) against the even half, and against D(even half)
+        ptk::launch(ctx, "k_dn_cross", k_dn_cross, grid, dim3(256), q, (const float*)d->den, (const float*)d->tgt, 1,
+                    inv, d_lines + 64 * 32);
+    }
     std::vector<double> h((size_t)2 * 64 * 32);
     e = hipMemcpyAsync(h.data(), d_lines, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -136,9 +136,9 @@ def test_bad_requests_are_rejected(gpu_ctx):
         assert srv.step() is None
 
 
-def attach_shipped_denoiser(ctx, srv, size=128):
+def attach_shipped_denoiser(ctx, srv, size=128, batch=1):
     from pathtracer_amd import denoise_train as T
-    dn = pt.Denoiser(ctx, 1, size, size)
+    dn = pt.Denoiser(ctx, batch, size, size)
     T.load_weights(dn)
     srv.set_denoiser(dn)
     return dn
@@ -146,13 +146,15 @@ def attach_shipped_denoiser(ctx, srv, size=128):
 
 def test_denoised_result_is_the_same_alone_and_in_the_mix():
     """A request that may finish denoised, run to its sample budget: its image
-    (denoised or, as fallback, plain) depends only on its own sums."""
+    (denoised or, as fallback, plain) depends only on its own sums. A denoiser
+    with room for a batch of 2 runs both half images of the error estimate in
+    one pass, which must not change the estimate either."""
     text = scenes.sdf_scene(jitter=0.05, scene_seed=4, seed=12)
     others = [(scenes.cornell(4, seed=1), 24, 24, 6), (scenes.random_spheres(n=64, scene_seed=9, seed=3), 40, 20, 12)]
     outs = []
-    for wave, mix, graph in ((1 << 22, False, 0), (20000, True, 0), (30000, True, 1)):
+    for wave, mix, graph, batch in ((1 << 22, False, 0, 1), (20000, True, 0, 2), (30000, True, 1, 2)):
         with pt.Context(0, wave_paths=wave, graph_mode=graph) as ctx, pt.Server(ctx) as srv:
-            dn = attach_shipped_denoiser(ctx, srv, 64)
+            dn = attach_shipped_denoiser(ctx, srv, 64, batch)
             rid = srv.submit(text, 48, 32, spp_max=40, path_length=6, target=1e-6, denoise=True)
             extra = [srv.submit(t, w, h, spp_max=24, path_length=d, target=1e-6) for t, w, h, d in others] if mix else []
             st = run_until_done(srv, [rid] + extra)[rid]
