@@ -37,6 +37,9 @@ namespace ptk {
 
 using namespace ptd;
 
+// in namespace ptk: the header holds device helpers only, no includes
+#include "wave_queue.h"
+
 // ---------------------------------------------------------------------------------
 // Device scene.
 enum : uint32_t { REF_SPHERE = 0u, REF_QUAD = 1u, REF_PLANE = 2u, REF_SDF = 3u };
@@ -45,15 +48,14 @@ enum : int { Q_MISS = 0, Q_EMIT = 1, Q_LAMBERT = 2, Q_DIELECTRIC = 3, Q_CONDUCTO
 constexpr int kQuadBrute = 16;  // at most this many parallelograms are tested without a BVH
 // Queue lengths: [0..1] extension queues, [2] shadow queue, [3..7] material
 // queues, each counter on its own 256-byte line (device-scope atomics on one
-// line serialise across the chip).
-constexpr int kCntStride = 64;
+// line serialise across the chip); kCntStride is in wave_queue.h.
 // The shadow queue's length alternates between two counters by bounce parity:
 // bounce b appends to CNT_SHADOW[b & 1], which the next bounce's trace reads
 // while bounce b + 1 appends to the other one.
 enum : int { CNT_SHADOW0 = 2, CNT_MAT = 3, CNT_SHADOW1 = 3 + N_MATQ, N_CNT = 4 + N_MATQ };
 constexpr int shadow_cnt(int b) { return (b & 1) ? CNT_SHADOW1 : CNT_SHADOW0; }
-// Persistent kernels' queue cursors: kSeg segments per queue, one line each.
-constexpr int kSeg = 32;
+// Persistent kernels' queue cursors: kSeg segments per queue, one line each
+// (wave_queue.h).
 // Cursor sets: persistent intersect, persistent shadow, and one per SDF kind for
 // the persistent march kernels (CUR_MARCH + kind - 1).
 enum : int { CUR_ISECT = 0, CUR_SHADOW = 1, CUR_MARCH = 2, N_CUR = 6 };
@@ -137,7 +139,7 @@ struct ExtWave {
     uint32_t* smask;  // SDF scenes: bit k set when the ray enters the box of an SDF of kind k before its analytic hit
     float* sum;
     int* counts;   // N_CNT queue lengths, kCntStride apart
-    int* cursor;   // 2 x kSeg persistent-kernel cursors, kCntStride apart
+    int* cursor;   // N_CUR x kSeg persistent-kernel cursors, kCntStride apart
     unsigned long long* stats;
     int cap;
 };
@@ -909,64 +911,6 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
 // them to its lanes from registers. Consecutive waves have different home
 // segments, so the blocks of one XCD (blockIdx.x mod 8) share a few segments.
 constexpr int kRefill = 16;
-constexpr int kChunkMax = 64;
-
-// A wave's share of a segmented queue; wave-uniform.
-struct WaveChunk {
-    int lo = 0, hi = 0;          // entries taken but not handed out yet
-    int seg_lo = 0, seg_hi = 0;  // the home segment
-    int chunk = 1;
-    int* cur = nullptr;          // the home segment's cursor
-    bool exhausted = false;      // the home segment has no entries left to take
-    __device__ __forceinline__ bool drained() const { return exhausted && lo >= hi; }
-};
-
-__device__ __forceinline__ WaveChunk wave_chunk_init(int* cursors, int n) {
-    WaveChunk c;
-    const int nw = (int)(blockDim.x >> 6);
-    const int waves = (int)gridDim.x * nw;
-    // every segment needs at least one home wave: a grid of fewer than kSeg
-    // waves uses as many segments as it has waves
-    const int nseg = min(kSeg, waves);
-    const int seg = ((int)blockIdx.x * nw + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) % nseg;
-    const int len = (n + nseg - 1) / nseg;
-    c.seg_lo = min(seg * len, n);
-    c.seg_hi = min(c.seg_lo + len, n);
-    c.chunk = max(1, min(kChunkMax, n / max(1, waves)));
-    c.cur = cursors + seg * kCntStride;
-    c.exhausted = c.seg_lo >= c.seg_hi;
-    return c;
-}
-
-// Gives the idle lanes the next entries of the wave's segment; returns this
-// lane's entry or -1. Call from converged control flow.
-__device__ __forceinline__ int wave_refill(uint64_t idle, WaveChunk& c) {
-    const int lane = (int)__lane_id();
-    const int k = (int)__popcll(idle);
-    const int avail = c.hi - c.lo;
-    int nlo = 0, nhi = 0;
-    if (avail < k && !c.exhausted) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(c.cur, c.chunk);
-        base = __builtin_amdgcn_readfirstlane(base);  // lane 0's value, in a scalar register
-        nlo = min(c.seg_lo + base, c.seg_hi);
-        nhi = min(nlo + c.chunk, c.seg_hi);
-        if (nhi >= c.seg_hi) c.exhausted = true;
-    }
-    const int r = (int)lanes_below(idle);
-    int e = -1;
-    if ((idle >> lane) & 1ull) {
-        if (r < avail) e = c.lo + r;
-        else if (r - avail < nhi - nlo) e = nlo + (r - avail);
-    }
-    if (k <= avail) {
-        c.lo += k;
-    } else {
-        c.lo = nlo + min(k - avail, nhi - nlo);
-        c.hi = nhi;
-    }
-    return e;
-}
 
 // Persistent closest-hit kernel (parallelograms tested directly, spheres through
 // their BVH, then planes); see the persistent traversal notes above. Writes the
