@@ -135,10 +135,21 @@ struct Parser {
         }
         std::string tok(s, p);
         if (!real) {
+            // nlohmann: a negative literal as int64, a non-negative one as uint64,
+            // and a double when it fits neither
             errno = 0;
-            long long ll = std::strtoll(tok.c_str(), nullptr, 10);
-            if (errno == ERANGE) real = true;  // nlohmann falls back to a double too
-            else { v.kind = Kind::Int; v.i = ll; v.d = (double)ll; return true; }
+            if (tok[0] == '-') {
+                long long ll = std::strtoll(tok.c_str(), nullptr, 10);
+                if (errno != ERANGE) { v.kind = Kind::Int; v.i = ll; v.d = (double)ll; return true; }
+            } else {
+                unsigned long long ull = std::strtoull(tok.c_str(), nullptr, 10);
+                if (errno != ERANGE) {
+                    if (ull <= (unsigned long long)INT64_MAX) { v.kind = Kind::Int; v.i = (int64_t)ull; }
+                    else { v.kind = Kind::Uint; v.u = ull; }
+                    v.d = (double)ull;
+                    return true;
+                }
+            }
         }
         v.kind = Kind::Real;
         v.d = std::strtod(tok.c_str(), nullptr);  // correctly rounded in glibc
@@ -215,6 +226,7 @@ static std::runtime_error type_error(const char* what, const char* want) {
 
 double as_double(const Value& v, const char* what) {
     if (v.kind == Kind::Int) return (double)v.i;
+    if (v.kind == Kind::Uint) return (double)v.u;
     if (v.kind == Kind::Real) return v.d;
     if (v.kind == Kind::Bool) return v.b ? 1.0 : 0.0;
     throw type_error(what, "a number");
@@ -222,6 +234,7 @@ double as_double(const Value& v, const char* what) {
 
 float as_float(const Value& v, const char* what) {
     if (v.kind == Kind::Int) return static_cast<float>(v.i);
+    if (v.kind == Kind::Uint) return static_cast<float>(v.u);  // directly, not through double
     if (v.kind == Kind::Real) return static_cast<float>(v.d);
     if (v.kind == Kind::Bool) return v.b ? 1.0f : 0.0f;
     throw type_error(what, "a number");
@@ -229,6 +242,7 @@ float as_float(const Value& v, const char* what) {
 
 int as_int(const Value& v, const char* what) {
     if (v.kind == Kind::Int) return static_cast<int>(v.i);
+    if (v.kind == Kind::Uint) return static_cast<int>(v.u);
     if (v.kind == Kind::Real) return static_cast<int>(v.d);
     if (v.kind == Kind::Bool) return v.b ? 1 : 0;
     throw type_error(what, "a number");
@@ -237,6 +251,7 @@ int as_int(const Value& v, const char* what) {
 bool as_bool(const Value& v, const char* what) {
     if (v.kind == Kind::Bool) return v.b;
     if (v.kind == Kind::Int) return v.i != 0;
+    if (v.kind == Kind::Uint) return true;
     if (v.kind == Kind::Real) return v.d != 0.0;
     throw type_error(what, "a boolean");
 }
@@ -274,6 +289,7 @@ static void dump_rec(std::string& o, const Value& v, int indent, int level) {
         case Kind::Null: o += "null"; break;
         case Kind::Bool: o += v.b ? "true" : "false"; break;
         case Kind::Int: o += std::to_string(v.i); break;
+        case Kind::Uint: o += std::to_string(v.u); break;
         case Kind::Real: {
             if (!std::isfinite(v.d)) { o += "null"; break; }  // as nlohmann::json does
             // the shortest %g text that reads back as the same double (or float, for f32)

@@ -1,0 +1,175 @@
+"""The scene loader against the JSON library the reference itself uses.
+
+oracle/_ref/json_probe is built (by oracle/Makefile, where the reference tree is
+present) from a probe of this repository and the single header the reference
+vendors, nlohmann-json 3.11.3. It prints, per number or bool leaf, the binary32
+that get<float>() returns and the int that get<int>() returns: what
+UpdateFromJSON (src/pathtracer.cpp:2576-2722) stores in its float and int
+fields. The uniform block and the camera push constants packed by pt.pack_json
+must hold exactly those values, on the 11 shipped scenes and on scenes whose
+literals are hard for a decimal parser: long mantissas, binary32 ties reached
+through the binary64 parse, denormals, and integers beyond int64 and uint64."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import pathtracer_amd as pt
+from tests.conftest import SCENES
+from tests.test_scene import py_pack
+
+ROOT = Path(__file__).resolve().parent.parent
+PROBE = ROOT / "oracle" / "_ref" / "json_probe"
+
+pytestmark = pytest.mark.skipif(not PROBE.exists(), reason="oracle/_ref/json_probe not built (needs the reference tree)")
+
+# fields that UpdateFromJSON stores in ints (src/pathtracer.cpp:110-182)
+INT_KEYS = {"materialID", "lightID", "ISO", "numShots"}
+
+
+def nlohmann_values(text: str, tmp_path: Path) -> dict:
+    f = tmp_path / "scene.json"
+    f.write_text(text)
+    r = subprocess.run([str(PROBE), str(f)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    for line in r.stdout.splitlines():
+        ptr, bits, i = line.split(" ")
+        out[ptr] = (np.uint32(int(bits, 16)).view(np.float32), None if i == "x" else int(i))
+    return out
+
+
+def as_read_by_reference(node, vals: dict, ptr: str = ""):
+    """The scene with each number or bool leaf replaced by what the reference's
+    field of that name holds."""
+    if isinstance(node, dict):
+        return {k: as_read_by_reference(v, vals, f"{ptr}/{k}") for k, v in node.items()}
+    if isinstance(node, list):
+        return [as_read_by_reference(v, vals, f"{ptr}/{i}") for i, v in enumerate(node)]
+    if isinstance(node, (bool, int, float)):
+        f, i = vals[ptr]
+        if ptr.rsplit("/", 1)[-1] in INT_KEYS:
+            assert i is not None, ptr
+            return i
+        return f
+    return node
+
+
+def check_against_nlohmann(text: str, tmp_path: Path):
+    vals = nlohmann_values(text, tmp_path)
+    scene = as_read_by_reference(json.loads(text), vals)
+    ubo, _, _ = pt.pack_json(text)
+    got = np.frombuffer(bytes(ubo), np.float32)[:2774]
+    want = py_pack(scene)[:2774]
+    bad = np.flatnonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, [(int(k), float(got[k]), float(want[k])) for k in bad[:8]]
+    cam = scene["camera"]
+    for shot in range(1, cam["numShots"] + 1):
+        _, pc, _ = pt.pack_json(text, shot)
+        pos, ang = cam["position"][shot - 1], cam["angle"][shot - 1]
+        # cameraAngle = (-angle.y, angle.x) (src/pathtracer.cpp:3821)
+        got_cam = np.array([pc.cameraPosX, pc.cameraPosY, pc.cameraPosZ, *pc.cameraAngle], np.float32)
+        want_cam = np.array([*pos, -ang[1], ang[0]], np.float32)
+        assert np.array_equal(got_cam.view(np.uint32), want_cam.view(np.uint32)), (shot, got_cam, want_cam)
+        assert pc.ISO == cam["ISO"]
+        for name, key in (("apertureDist", "apertureDistance"), ("lensFocalLength", "lensFocalLength")):
+            assert np.float32(getattr(pc, name)).view(np.uint32) == np.float32(cam[key]).view(np.uint32), name
+
+
+@pytest.mark.parametrize("path", SCENES, ids=lambda p: p.stem)
+def test_shipped_scenes_read_as_nlohmann_reads_them(path, tmp_path):
+    check_against_nlohmann(path.read_text(), tmp_path)
+
+
+HARD_REALS = [
+    "0.1000000000000000055511151231257827",
+    "0.333333333333333314829616256247390992939472198486328125",
+    "2.5000000000000004",
+    # just above the binary32 tie between 1 and 1 + 2^-23: binary64 rounds it onto
+    # the tie, which binary32 then breaks to even (1.0)
+    "1.00000005960464477539062500000000001",
+    "1e-46",
+    "7.006492321624085e-46",
+    "1.401298464324817e-45",
+    "1.1754942106924411e-38",
+    "3.4028234663852886e38",
+    "4.9406564584124654e-324",
+    "-0.0",
+    "-0",
+    "0e0",
+    "1E+2",
+    "-2.5e-3",
+    "123456789012345678901234567890",
+    "16777217",
+    "16777219",
+    "-16777217",
+    "9223372036854775807",
+    "9223372036854775808",
+    # 2^63 + 2^39 + 1: as an unsigned 64-bit integer it rounds up to 2^63 + 2^40 in
+    # binary32; through binary64 it lands on the tie and rounds to 2^63
+    "9223372586610589697",
+    "18446744073709551615",
+    "18446744073709551616",
+    "-9223372036854775808",
+    "-9223372036854775809",
+    "-9223372586610589697",
+]
+
+
+def with_literals(scene: dict, slots: list, literals: list) -> str:
+    """scene as JSON text with the leaves at `slots` (key paths) replaced by the
+    given literals, verbatim."""
+    for n, path in enumerate(slots):
+        node = scene
+        for k in path[:-1]:
+            node = node[k]
+        node[path[-1]] = f"@@{n}@@"
+    text = json.dumps(scene, indent=1)
+    for n, lit in enumerate(literals):
+        text = text.replace(f'"@@{n}@@"', lit)
+    return text
+
+
+def float_slots(scene: dict) -> list:
+    slots = []
+    for i, s in enumerate(scene["sphere"]):
+        slots += [("sphere", i, "position", k) for k in range(3)] + [("sphere", i, "radius")]
+    for i, _ in enumerate(scene["plane"]):
+        slots += [("plane", i, "position", k) for k in range(3)]
+    for i, _ in enumerate(scene["material"]):
+        slots += [("material", i, "reflection", "peakWavelength"), ("material", i, "reflection", "sigma")]
+    for i, _ in enumerate(scene["light"]):
+        slots += [("light", i, "emission", "temperature"), ("light", i, "emission", "luminosity")]
+    for k in range(3):
+        slots.append(("camera", "position", 0, k))
+    slots += [("camera", "angle", 0, 0), ("camera", "angle", 0, 1), ("camera", "apertureDistance"),
+              ("camera", "lensFocalLength")]
+    return slots
+
+
+@pytest.mark.parametrize("shift", range(4))
+def test_hard_literals_read_as_nlohmann_reads_them(scene_texts, tmp_path, shift):
+    base = json.loads(scene_texts["scene0"])
+    slots = float_slots(base)
+    lits = [HARD_REALS[(n + shift * 7) % len(HARD_REALS)] for n in range(len(slots))]
+    # every literal lands somewhere across the four shifts
+    check_against_nlohmann(with_literals(base, slots, lits), tmp_path)
+
+
+def test_real_literals_in_int_fields(scene_texts, tmp_path):
+    base = json.loads(scene_texts["scene0"])
+    slots = [("sphere", 0, "materialID"), ("sphere", 1, "materialID"), ("sphere", 0, "lightID"),
+             ("camera", "ISO"), ("plane", 0, "materialID")]
+    lits = ["1.0", "0.99999", "0e0", "1600.7", "1E0"]
+    check_against_nlohmann(with_literals(base, slots, lits), tmp_path)
+
+
+def test_probe_sees_the_double_rounding_case(tmp_path):
+    """The probe itself: an integer literal beyond int64 converts straight to
+    binary32 (rounding up), while the same value written as a real literal goes
+    through binary64 and lands on the tie (rounding to even)."""
+    vals = nlohmann_values('{"a": 9223372586610589697, "b": 9223372586610589697.0}', tmp_path)
+    assert vals["/a"][0] == np.float32(2.0 ** 63 + 2.0 ** 40)
+    assert vals["/b"][0] == np.float32(2.0 ** 63)
