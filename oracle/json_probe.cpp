@@ -9,6 +9,9 @@
 // Output, one line per leaf: <JSON pointer> <f32 bits of get<float>(), hex>
 // <get<int>(), or x when the value is outside int's range>.
 //
+// With --dump, prints instead the document as the reference's SaveScene writes
+// it: scene.dump(4, ' ', true) (src/pathtracer.cpp:3465-3470).
+//
 // Built by oracle/Makefile into oracle/_ref/ from the header where it lies in
 // the reference tree; never part of the product.
 #include <cmath>
@@ -48,19 +51,23 @@ static void walk(const std::string& ptr, const Json& j) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 2) {
-        std::fprintf(stderr, "usage: %s scene.json\n", argv[0]);
+    const bool dump = argc == 3 && std::strcmp(argv[1], "--dump") == 0;
+    if (argc != 2 && !dump) {
+        std::fprintf(stderr, "usage: %s [--dump] scene.json\n", argv[0]);
         return 2;
     }
-    std::ifstream in(argv[1], std::ios::binary);
+    const char* path = argv[argc - 1];
+    std::ifstream in(path, std::ios::binary);
     if (!in) {
-        std::fprintf(stderr, "cannot open %s\n", argv[1]);
+        std::fprintf(stderr, "cannot open %s\n", path);
         return 2;
     }
     std::stringstream ss;
     ss << in.rdbuf();
     try {
-        walk("", Json::parse(ss.str()));
+        const Json j = Json::parse(ss.str());
+        if (dump) std::fputs(j.dump(4, ' ', true).c_str(), stdout);
+        else walk("", j);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "parse error: %s\n", e.what());
         return 1;

@@ -261,23 +261,95 @@ const std::string& as_string(const Value& v, const char* what) {
     return v.s;
 }
 
+// Strings as the reference saves them, nlohmann-json's dump(4, ' ', true)
+// (src/pathtracer.cpp:3469): the two-character escapes for " \\ \b \f \n \r \t,
+// and with ensure_ascii every other code point below 0x20 or from 0x7F up as
+// \uXXXX in lower-case hex (a UTF-16 surrogate pair above U+FFFF). A byte that
+// does not start a valid UTF-8 sequence is written as it is.
 static void dump_string(std::string& o, const std::string& s) {
+    auto u16 = [&](unsigned cp) {
+        char b[8];
+        std::snprintf(b, sizeof b, "\\u%04x", cp);
+        o += b;
+    };
     o.push_back('"');
-    for (unsigned char c : s) {
+    const size_t n = s.size();
+    for (size_t i = 0; i < n; ++i) {
+        const unsigned char c = (unsigned char)s[i];
         switch (c) {
-            case '"': o += "\\\""; break;
-            case '\\': o += "\\\\"; break;
-            case '\n': o += "\\n"; break;
-            case '\r': o += "\\r"; break;
-            case '\t': o += "\\t"; break;
-            case '\b': o += "\\b"; break;
-            case '\f': o += "\\f"; break;
-            default:
-                if (c < 0x20) { char b[8]; std::snprintf(b, sizeof b, "\\u%04x", c); o += b; }
-                else o.push_back((char)c);
+            case '"': o += "\\\""; continue;
+            case '\\': o += "\\\\"; continue;
+            case '\n': o += "\\n"; continue;
+            case '\r': o += "\\r"; continue;
+            case '\t': o += "\\t"; continue;
+            case '\b': o += "\\b"; continue;
+            case '\f': o += "\\f"; continue;
+            default: break;
         }
+        if (c < 0x20 || c == 0x7F) { u16(c); continue; }
+        if (c < 0x80) { o.push_back((char)c); continue; }
+        // a multi-byte UTF-8 sequence: its length, lead bits and minimum value
+        const int len = c >= 0xF0 && c < 0xF5 ? 4 : c >= 0xE0 ? (c < 0xF0 ? 3 : 0) : c >= 0xC2 ? 2 : 0;
+        unsigned cp = len == 4 ? (c & 0x07u) : len == 3 ? (c & 0x0Fu) : (c & 0x1Fu);
+        bool ok = len > 0 && i + (size_t)len <= n;
+        for (int k = 1; ok && k < len; ++k) {
+            const unsigned char t = (unsigned char)s[i + (size_t)k];
+            ok = (t & 0xC0u) == 0x80u;
+            cp = (cp << 6) | (t & 0x3Fu);
+        }
+        ok = ok && !(len == 3 && (cp < 0x800 || (cp >= 0xD800 && cp <= 0xDFFF))) && !(len == 4 && (cp < 0x10000 || cp > 0x10FFFF));
+        if (!ok) { o.push_back((char)c); continue; }
+        if (cp <= 0xFFFF) u16(cp);
+        else { u16(0xD800 + ((cp - 0x10000) >> 10)); u16(0xDC00 + ((cp - 0x10000) & 0x3FF)); }
+        i += (size_t)len - 1;
     }
     o.push_back('"');
+}
+
+// A double as nlohmann-json's serializer writes it (the reference saves scenes
+// through it): the shortest digit string that reads back as the same value (as
+// the same binary32 for `f32`), laid out as plain digits with ".0" when the
+// decimal point falls after the last digit and within 15 places, as d.ddd when
+// it falls inside, as 0.000ddd down to 10^-4, and otherwise as d.ddde+XX with a
+// signed exponent of at least two digits; zero is 0.0 or -0.0.
+static void dump_real(std::string& o, double x, bool f32) {
+    if (std::signbit(x)) { o.push_back('-'); x = -x; }
+    if (x == 0.0) { o += "0.0"; return; }
+    char b[40];
+    for (int prec = 1; prec <= 17; ++prec) {
+        std::snprintf(b, sizeof b, "%.*e", prec - 1, x);
+        const double r = std::strtod(b, nullptr);
+        if (f32 ? (float)r == (float)x : r == x) break;
+    }
+    // b is "d.ddde[+-]XX": value = 0.DIGITS x 10^n, n = XX + 1
+    std::string dig;
+    const char* q = b;
+    for (; *q && *q != 'e'; ++q)
+        if (*q >= '0' && *q <= '9') dig.push_back(*q);
+    while (dig.size() > 1 && dig.back() == '0') dig.pop_back();
+    const int k = (int)dig.size();
+    const int n = std::atoi(q + 1) + 1;
+    if (k <= n && n <= 15) {
+        o += dig;
+        o.append((size_t)(n - k), '0');
+        o += ".0";
+    } else if (0 < n && n <= 15) {
+        o.append(dig, 0, (size_t)n);
+        o.push_back('.');
+        o.append(dig, (size_t)n, std::string::npos);
+    } else if (-4 < n && n <= 0) {
+        o += "0.";
+        o.append((size_t)(-n), '0');
+        o += dig;
+    } else {
+        o.push_back(dig[0]);
+        if (k > 1) {
+            o.push_back('.');
+            o.append(dig, 1, std::string::npos);
+        }
+        std::snprintf(b, sizeof b, "e%c%02d", n - 1 < 0 ? '-' : '+', std::abs(n - 1));
+        o += b;
+    }
 }
 
 static void dump_rec(std::string& o, const Value& v, int indent, int level) {
@@ -290,20 +362,10 @@ static void dump_rec(std::string& o, const Value& v, int indent, int level) {
         case Kind::Bool: o += v.b ? "true" : "false"; break;
         case Kind::Int: o += std::to_string(v.i); break;
         case Kind::Uint: o += std::to_string(v.u); break;
-        case Kind::Real: {
-            if (!std::isfinite(v.d)) { o += "null"; break; }  // as nlohmann::json does
-            // the shortest %g text that reads back as the same double (or float, for f32)
-            char b[40];
-            for (int p = 1; p <= 17; ++p) {
-                std::snprintf(b, sizeof b, "%.*g", p, v.d);
-                double r = std::strtod(b, nullptr);
-                if (v.f32 ? (float)r == (float)v.d : r == v.d) break;
-            }
-            std::string t = b;
-            if (t.find_first_of(".eEn") == std::string::npos) t += ".0";
-            o += t;
+        case Kind::Real:
+            if (!std::isfinite(v.d)) o += "null";  // as nlohmann::json does
+            else dump_real(o, v.d, v.f32);
             break;
-        }
         case Kind::String: dump_string(o, v.s); break;
         case Kind::Array:
             if (v.arr.empty()) { o += "[]"; break; }
