@@ -176,6 +176,8 @@ def parse_args():
     ap.add_argument("--weights", type=str, default="", help="denoiser parameters (.npy) instead of the shipped ones")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run --steps device-to-device copies of MIB MiB (calibrates the HBM byte counters)")
+    ap.add_argument("--peak-probe", action="store_true",
+                    help="only measure the device peaks: streaming copy (GB/s) and unpacked FP32 FMA loop (TFLOP/s)")
     return ap.parse_args()
 
 
@@ -196,6 +198,24 @@ def copy_probe(mib: int, steps: int):
     print(json.dumps({"copy_bytes": n * 4, "steps": steps, "GB/s_read+write": 2 * n * 4 * steps / dt / 1e9}))
 
 
+def peak_probe(steps: int):
+    """Measured peaks beside the published ones: a streaming copy of 2 x 4 GiB
+    and the FP32 FMA loop (8 independent chains per thread, no packed FP32),
+    each launched `steps` times after a warm-up and timed with HIP events."""
+    import pathtracer_amd as pt
+    with pt.Context(0) as ctx:
+        bw, nbytes, copy_ms = ctx.peak_probe(0, 1 << 30, max(1, steps))
+        fl, nflop, fma_ms = ctx.peak_probe(1, 1 << 16, max(1, steps))
+    print(json.dumps({
+        "copy": {"kernel": "k_peak_copy", "bytes_per_launch": nbytes, "ms": round(copy_ms, 4),
+                 "GB/s": round(bw / 1e9, 1), "published_GB/s": HBM_PEAK_GBS,
+                 "frac_of_published": round(bw / 1e9 / HBM_PEAK_GBS, 4)},
+        "fp32_fma": {"kernel": "k_peak_fma", "flops_per_launch": nflop, "ms": round(fma_ms, 4),
+                     "TFLOP/s": round(fl / 1e12, 2), "published_TFLOP/s": FP32_PEAK_TFLOPS,
+                     "frac_of_published": round(fl / 1e12 / FP32_PEAK_TFLOPS, 4)},
+        "steps": steps}))
+
+
 def traffic_from_profiles(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` from a committed rocprofv3
     --pmc summary (profiles/pmc_<kernel>_<workload>.txt)."""
@@ -206,6 +226,20 @@ def traffic_from_profiles(kernel: str, workload: str):
                     return float(line.split(":", 1)[1].split()[0])
                 except ValueError:
                     return None
+    return None
+
+
+def measured_peaks():
+    """Device peaks measured by `bench.py --peak-probe` (streaming copy GB/s,
+    unpacked FP32 FMA loop TFLOP/s) from the committed summary
+    profiles/peaks_copy_fma_mi355x.txt, or None when it is absent."""
+    f = ROOT / "profiles" / "peaks_copy_fma_mi355x.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith('{"copy"'):
+            j = json.loads(line)
+            return {"GB/s": j["copy"]["GB/s"], "TFLOP/s": j["fp32_fma"]["TFLOP/s"]}
     return None
 
 
@@ -697,6 +731,9 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe, args.steps)
         return
+    if args.peak_probe:
+        peak_probe(args.steps)
+        return
     if args.serve_check:
         serve_check(args)
         return
@@ -756,6 +793,7 @@ def trace_bench(args):
         ctx.profile(False)
 
     roofline = None
+    mp = measured_peaks()
     if prof:
         dom = max(prof.items(), key=lambda kv: kv[1][1])
         name, (launches, total_ms) = dom
@@ -771,6 +809,9 @@ def trace_bench(args):
                 "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
                 "traffic": traffic_from_profiles(name, tag),
+                # beside the published peak: the FMA loop measured on this device
+                "peak_measured": mp and mp["TFLOP/s"],
+                "frac_measured": mp and round(achieved / mp["TFLOP/s"], 5),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_flops_per_launch": round(flops / launches),
@@ -785,6 +826,9 @@ def trace_bench(args):
                 "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic_from_profiles(name, tag),
+                # beside the published peak: the streaming copy measured on this device
+                "peak_measured": mp and mp["GB/s"],
+                "frac_measured": mp and round(achieved / mp["GB/s"], 5),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_bytes_per_launch": round(total_bytes / launches),

@@ -253,6 +253,7 @@ SIGNATURES = {
         C.c_int,
         [_P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t],
     ),
+    "pt_peak_probe": (C.c_int, [_P, C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
 }
 
 
@@ -427,6 +428,22 @@ class Context:
         self.check(lib.pt_math_eval_device(self.h, fn, _fptr(x), _fptr(yy) if yy is not None else None,
                                            _fptr(out), x.size))
         return out
+
+    def peak_probe(self, what: int, n: int, reps: int) -> tuple:
+        """Measured device peak: what 0 streams n floats (bytes/s), what 1 runs n
+        rounds of the FP32 FMA loop (flop/s). Returns (rate per second, units per
+        launch, average launch ms) from the HIP events around each timed launch."""
+        units = C.c_double()
+        self.profile(True)
+        self.profile_read(reset=True)
+        try:
+            self.check(lib.pt_peak_probe(self.h, what, n, reps, C.byref(units)))
+            stats = self.profile_read(reset=True)
+        finally:
+            self.profile(False)
+        launches, total_ms = stats["k_peak_copy" if what == 0 else "k_peak_fma"]
+        ms = total_ms / launches
+        return units.value / (ms * 1e-3), units.value, ms
 
 
 class Scene:

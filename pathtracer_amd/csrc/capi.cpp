@@ -557,4 +557,12 @@ pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* 
     return ptk::math_eval_device(ctx, fn, x, y, out, n);
 }
 
+pt_status pt_peak_probe(pt_ctx* ctx, int what, size_t n, int reps, double* units_out) {
+    if (!ctx || !units_out || what < 0 || what > 1 || reps < 1 || reps > 1000 || n == 0 ||
+        (what == 0 && (n % 4 != 0 || n > ((size_t)1 << 30))) || (what == 1 && n > ((size_t)1 << 20)))
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_peak_probe: bad argument");
+    (void)hipSetDevice(ctx->device);
+    return ptk::peak_probe(ctx, what, n, reps, units_out);
+}
+
 }  // extern "C"

@@ -1,5 +1,6 @@
 // resolve.hip — display resolve of the XYZ accumulator and device-side
 // diagnostics.
+#include <algorithm>
 #include <vector>
 
 #include "tracer.hpp"
@@ -98,6 +99,66 @@ int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float*
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     dev_free(ctx, buf);
     if (e != hipSuccess) return hip_fail(ctx, e, "pt_math_eval_device");
+    return PT_OK;
+}
+
+// Measured device peaks for the rooflines. k_peak_copy streams n4 float4s from
+// src to dst (grid-stride, 32 bytes of HBM traffic per element). k_peak_fma runs
+// `iters` rounds of 8 independent fused multiply-adds per thread (explicit fmaf:
+// this library is built without FP contraction, and without packed FP32, so the
+// loop measures what unpacked FP32 code can issue) and stores one sum per thread
+// so that nothing is dead.
+__global__ void __launch_bounds__(256) k_peak_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_peak_fma(float m, float c, int iters, float* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float a[8];
+    for (int k = 0; k < 8; ++k) a[k] = (float)(t + k) * 1e-9f;
+#pragma unroll 4
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = __builtin_fmaf(a[k], m, c);
+    }
+    out[t] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+int peak_probe(pt_ctx* ctx, int what, size_t n, int reps, double* units) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0)
+        cus = 256;
+    DevAlloc buf;
+    hipError_t e = hipSuccess;
+    if (what == 0) {
+        const size_t n4 = n / 4;
+        int st = dev_alloc(ctx, buf, n * 8);
+        if (st != PT_OK) return st;
+        const float4* src = static_cast<const float4*>(buf.p);
+        float4* dst = static_cast<float4*>(buf.p) + n4;
+        e = hipMemsetAsync(buf.p, 0, n * 8, ctx->stream);
+        const size_t want = (n4 + 255) / 256;
+        const unsigned blocks = (unsigned)std::min<size_t>(want, (size_t)cus * 64);
+        for (int r = 0; e == hipSuccess && r <= reps; ++r) {  // the first launch warms up
+            launch(ctx, r == 0 ? "k_peak_copy_warmup" : "k_peak_copy", k_peak_copy, dim3(blocks), dim3(256), src, dst, n4);
+            e = hipGetLastError();
+        }
+        *units = 8.0 * (double)n4 * 4.0;
+    } else {
+        const unsigned blocks = (unsigned)cus * 16;
+        int st = dev_alloc(ctx, buf, (size_t)blocks * 256 * 4);
+        if (st != PT_OK) return st;
+        for (int r = 0; e == hipSuccess && r <= reps; ++r) {
+            launch(ctx, r == 0 ? "k_peak_fma_warmup" : "k_peak_fma", k_peak_fma, dim3(blocks), dim3(256), 0.999999f,
+                   1e-7f, (int)n, static_cast<float*>(buf.p));
+            e = hipGetLastError();
+        }
+        *units = 16.0 * (double)n * (double)blocks * 256.0;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, buf);
+    if (e != hipSuccess) return hip_fail(ctx, e, "pt_peak_probe");
     return PT_OK;
 }
 

@@ -232,6 +232,7 @@ void ext_wave_release(pt_ctx* ctx);
 // Display resolve and diagnostics (resolve.hip).
 int resolve_srgb8(pt_ctx* ctx, pt_accum* acc, int tonemap, uint8_t* host_rgb);
 int math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* y, float* out, size_t n);
+int peak_probe(pt_ctx* ctx, int what, size_t n, int reps, double* units);
 int count_dispatch(pt_ctx* ctx);
 
 // Kernel launch on the context's stream; with profiling on, bracketed by two

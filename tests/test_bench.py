@@ -69,3 +69,14 @@ def test_roofline_units_are_counters():
             assert bytes_per_unit > 0 and unit in fields, unit
     for counter in bench.FLOPS_COUNTER.values():
         assert counter in fields, counter
+
+
+def test_measured_peaks_read_from_the_committed_profile():
+    mp = bench.measured_peaks()
+    assert mp is not None, "profiles/peaks_copy_fma_mi355x.txt is missing its bench line"
+    assert 1000.0 < mp["GB/s"] < bench.HBM_PEAK_GBS
+    assert 10.0 < mp["TFLOP/s"] < bench.FP32_PEAK_TFLOPS
+
+
+def test_peak_probe_flag():
+    assert args("--peak-probe").peak_probe and not args().peak_probe

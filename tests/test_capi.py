@@ -48,6 +48,7 @@ def test_argument_validation_without_device():
     assert pt.lib.pt_dispatch(None, None, None, None, 0) == pt.PT_ERR_INVALID_ARG
     assert pt.lib.pt_sdf_recognize(None, 0, None) == pt.PT_ERR_INVALID_ARG
     assert pt.lib.pt_pack_json(None, 0, 1, None, None, None, 0, None) == pt.PT_ERR_INVALID_ARG
+    assert pt.lib.pt_peak_probe(None, 0, 1024, 1, None) == pt.PT_ERR_INVALID_ARG
 
 
 def test_create_reports_missing_device_or_succeeds():
