@@ -1,6 +1,7 @@
 // json.cpp — see json.hpp.
 #include "json.hpp"
 
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -78,6 +79,31 @@ struct Parser {
         }
         return true;
     }
+    // A raw multi-byte UTF-8 sequence whose lead byte `c` was just read: checked
+    // against the well-formed byte ranges of RFC 3629 (no overlong forms, no
+    // encoded surrogates, nothing above U+10FFFF), as nlohmann-json's lexer
+    // checks them, and appended to s.
+    bool utf8_tail(unsigned char c, std::string& s) {
+        int n = 0;
+        unsigned char lo = 0x80, hi = 0xBF;  // range of the first continuation byte
+        if (c >= 0xC2 && c <= 0xDF) n = 1;
+        else if (c == 0xE0) { n = 2; lo = 0xA0; }
+        else if ((c >= 0xE1 && c <= 0xEC) || c == 0xEE || c == 0xEF) n = 2;
+        else if (c == 0xED) { n = 2; hi = 0x9F; }
+        else if (c == 0xF0) { n = 3; lo = 0x90; }
+        else if (c >= 0xF1 && c <= 0xF3) n = 3;
+        else if (c == 0xF4) { n = 3; hi = 0x8F; }
+        else { fail("ill-formed UTF-8 byte"); return false; }
+        if (end - p < n) { fail("ill-formed UTF-8 byte"); return false; }
+        for (int k = 0; k < n; ++k) {
+            const unsigned char t = (unsigned char)p[k];
+            if (t < (k == 0 ? lo : 0x80) || t > (k == 0 ? hi : 0xBF)) { fail("ill-formed UTF-8 byte"); return false; }
+        }
+        s.push_back((char)c);
+        s.append(p, (size_t)n);
+        p += n;
+        return true;
+    }
     bool string(std::string& s) {
         ++p;  // opening quote
         while (true) {
@@ -85,6 +111,10 @@ struct Parser {
             char c = *p++;
             if (c == '"') return true;
             if ((unsigned char)c < 0x20) { fail("control character in string"); return false; }
+            if ((unsigned char)c >= 0x80) {
+                if (!utf8_tail((unsigned char)c, s)) return false;
+                continue;
+            }
             if (c != '\\') { s.push_back(c); continue; }
             if (p >= end) { fail("unterminated escape"); return false; }
             char e = *p++;
@@ -100,6 +130,7 @@ struct Parser {
                 case 'u': {
                     uint32_t cp;
                     if (!hex4(cp)) return false;
+                    if (cp >= 0xDC00 && cp <= 0xDFFF) { fail("lone low surrogate"); return false; }
                     if (cp >= 0xD800 && cp < 0xDC00) {
                         uint32_t lo;
                         if (!(lit("\\u") && hex4(lo)) || lo < 0xDC00 || lo > 0xDFFF) {
@@ -153,6 +184,9 @@ struct Parser {
         }
         v.kind = Kind::Real;
         v.d = std::strtod(tok.c_str(), nullptr);  // correctly rounded in glibc
+        // beyond binary64's range: nlohmann rejects it (out_of_range 406); an
+        // underflow to zero or a denormal is kept
+        if (!std::isfinite(v.d)) { p = s; fail("number overflow"); return false; }
         return true;
     }
     bool value(Value& v, int depth) {
@@ -175,7 +209,11 @@ struct Parser {
                 ++p;
                 Value child;
                 if (!value(child, depth + 1)) return false;
-                v.obj.emplace_back(std::move(key), std::move(child));
+                // a repeated key keeps its first position and takes the last
+                // value, as nlohmann::ordered_json's parser does
+                auto same = std::find_if(v.obj.begin(), v.obj.end(), [&](const auto& kv) { return kv.first == key; });
+                if (same != v.obj.end()) same->second = std::move(child);
+                else v.obj.emplace_back(std::move(key), std::move(child));
                 ws();
                 if (p < end && *p == ',') { ++p; continue; }
                 if (p < end && *p == '}') { ++p; return true; }
@@ -213,6 +251,8 @@ struct Parser {
 
 bool parse(const char* text, size_t len, Value& out, std::string& err) {
     Parser ps{text, text + len, text, {}};
+    // a leading UTF-8 byte order mark is skipped, as nlohmann-json does
+    if (len >= 3 && std::memcmp(text, "\xEF\xBB\xBF", 3) == 0) ps.p += 3;
     out = Value{};
     if (!ps.value(out, 0)) { err = ps.err; return false; }
     ps.ws();

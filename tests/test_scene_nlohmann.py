@@ -234,3 +234,102 @@ def test_strings_escaped_as_the_reference_saves_them(scene_texts, tmp_path):
     assert ours.isascii() and "\\ud834\\udd1e" in ours and "\\u007f" in ours
     assert ours == reference_save(ours, tmp_path)
     assert json.loads(ours)["sdf"][0]["glsl"] == s["sdf"][0]["glsl"]
+
+
+def wrap_extra_key(base: bytes, val: bytes) -> bytes:
+    """A valid scene with one more top-level key holding `val` verbatim (both
+    the reference and the loader ignore unknown keys)."""
+    i = base.index(b"{")
+    return base[:i + 1] + b'"x": ' + val + b"," + base[i + 1:]
+
+
+# name -> a value placed under an extra top-level key of a valid scene
+EDGE_VALUES = {
+    "lone_low_surrogate": b'"\\udc00"',
+    "lone_high_surrogate": b'"\\ud800x"',
+    "high_then_ascii_escape": b'"\\ud800\\u0041"',
+    "surrogate_pair": b'"\\ud834\\udd1e"',
+    "raw_byte_ff": b'"a\xffb"',
+    "overlong_nul": b'"\xc0\x80"',
+    "encoded_surrogate": b'"\xed\xa0\x80"',
+    "above_u10ffff": b'"\xf4\x90\x80\x80"',
+    "four_byte_ok": '"\U0001d11e"'.encode(),
+    "truncated_utf8": b'"\xe2\x82"',
+    "stray_continuation": b'"\x80"',
+    "del": b'"\x7f"',
+    "escaped_nul": b'"\\u0000"',
+    "raw_control": b'"\x01"',
+    "leading_zero": b"01",
+    "minus_only": b"-",
+    "trailing_dot": b"1.",
+    "leading_plus": b"+1",
+    "trailing_comma": b"[1,2,]",
+    "comment": b"/* c */ 1",
+    "nan": b"NaN",
+    "overflow": b"1e400",
+    "negative_overflow": b"-1e400",
+    "underflow": b"1e-400",
+    "huge_integer": b"123456789012345678901234567890",
+    "formfeed_whitespace": b"\x0c1",
+    "upper_hex_escape": b'"\\u00E9"',
+    "escaped_slash": b'"\\/"',
+    "bad_escape": b'"\\a"',
+    "capital_true": b"True",
+    "null": b"null",
+}
+
+
+def probe_accepts(doc: bytes, tmp_path: Path) -> bool:
+    f = tmp_path / "doc.json"
+    f.write_bytes(doc)
+    r = subprocess.run([str(PROBE), str(f)], capture_output=True, timeout=60)
+    assert r.returncode in (0, 1), r.stderr
+    return r.returncode == 0
+
+
+def loader_accepts(doc: bytes) -> bool:
+    import ctypes as C
+    n = C.c_size_t()
+    st = pt.lib.pt_json_rewrite(doc, len(doc), None, 0, C.byref(n))
+    assert st in (pt.PT_OK, pt.PT_ERR_JSON), st
+    return st == pt.PT_OK
+
+
+def edge_documents(base: bytes) -> dict:
+    docs = {k: wrap_extra_key(base, v) for k, v in EDGE_VALUES.items()}
+    docs.update({"byte_order_mark": b"\xef\xbb\xbf" + base, "trailing_garbage": base + b" x",
+                 "trailing_whitespace": base + b" \n\t\r ", "two_documents": base + base, "empty": b"",
+                 "whitespace_only": b"  "})
+    return docs
+
+
+def test_accepts_and_rejects_what_nlohmann_does(tmp_path):
+    """Strings (escapes, surrogates, RFC 3629 UTF-8), numbers (forms, overflow),
+    a byte order mark and trailing content: the loader accepts a document
+    exactly when the reference's parser does."""
+    base = (ROOT / "tests" / "golden" / "scenes" / "scene0.json").read_bytes()
+    diff = {name: (probe_accepts(doc, tmp_path), loader_accepts(doc))
+            for name, doc in edge_documents(base).items()}
+    assert all(a == b for a, b in diff.values()), {k: v for k, v in diff.items() if v[0] != v[1]}
+    assert diff["byte_order_mark"] == (True, True) and diff["overflow"] == (False, False)
+
+
+def test_repeated_key_takes_the_last_value_in_the_first_place(scene_texts, tmp_path):
+    text = scene_texts["scene0"]
+    assert '"ISO": 1600' in text
+    doubled = text.replace('"ISO": 1600', '"ISO": 100, "ISO": 1600', 1)
+    _, pc, _ = pt.pack_json(doubled)
+    assert pc.ISO == 1600
+    check_against_nlohmann(doubled, tmp_path)
+    ours = pt.json_rewrite(doubled)
+    assert ours == reference_save(ours, tmp_path)
+
+
+def test_nesting_limit_is_the_one_deliberate_difference(tmp_path):
+    """nlohmann parses any depth; the loader stops at 256 levels so that a
+    hostile document cannot exhaust the calling thread's stack."""
+    base = (ROOT / "tests" / "golden" / "scenes" / "scene0.json").read_bytes()
+    ok = wrap_extra_key(base, b"[" * 200 + b"]" * 200)
+    deep = wrap_extra_key(base, b"[" * 300 + b"]" * 300)
+    assert probe_accepts(ok, tmp_path) and loader_accepts(ok)
+    assert probe_accepts(deep, tmp_path) and not loader_accepts(deep)
