@@ -12,6 +12,11 @@
 // With --dump, prints instead the document as the reference's SaveScene writes
 // it: scene.dump(4, ' ', true) (src/pathtracer.cpp:3465-3470).
 //
+// The default parse is ordered_json::parse of the whole text, as the startup
+// scene is read (src/pathtracer.cpp:2573). With --stream the file is read as
+// LoadScene reads one, `file >> json` (ReadJSON, src/pathtracer.cpp:893-908),
+// which parses the first value and leaves what follows it unread.
+//
 // Built by oracle/Makefile into oracle/_ref/ from the header where it lies in
 // the reference tree; never part of the product.
 #include <cmath>
@@ -52,8 +57,9 @@ static void walk(const std::string& ptr, const Json& j) {
 
 int main(int argc, char** argv) {
     const bool dump = argc == 3 && std::strcmp(argv[1], "--dump") == 0;
-    if (argc != 2 && !dump) {
-        std::fprintf(stderr, "usage: %s [--dump] scene.json\n", argv[0]);
+    const bool stream = argc == 3 && std::strcmp(argv[1], "--stream") == 0;
+    if (argc != 2 && !dump && !stream) {
+        std::fprintf(stderr, "usage: %s [--dump | --stream] scene.json\n", argv[0]);
         return 2;
     }
     const char* path = argv[argc - 1];
@@ -62,9 +68,15 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "cannot open %s\n", path);
         return 2;
     }
-    std::stringstream ss;
-    ss << in.rdbuf();
     try {
+        if (stream) {
+            Json j;
+            in >> j;
+            walk("", j);
+            return 0;
+        }
+        std::stringstream ss;
+        ss << in.rdbuf();
         const Json j = Json::parse(ss.str());
         if (dump) std::fputs(j.dump(4, ' ', true).c_str(), stdout);
         else walk("", j);

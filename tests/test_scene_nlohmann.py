@@ -333,3 +333,26 @@ def test_nesting_limit_is_the_one_deliberate_difference(tmp_path):
     deep = wrap_extra_key(base, b"[" * 300 + b"]" * 300)
     assert probe_accepts(ok, tmp_path) and loader_accepts(ok)
     assert probe_accepts(deep, tmp_path) and not loader_accepts(deep)
+
+
+def test_trailing_content_follows_the_strict_parse(tmp_path):
+    """The reference reads its startup scene with ordered_json::parse
+    (src/pathtracer.cpp:2573), which rejects content after the document, and a
+    scene the user loads with `file >> json` (ReadJSON, :893-908), which stops
+    after the first value. The loader follows the strict parse; the two reference
+    paths differ on exactly these documents."""
+    base = (ROOT / "tests" / "golden" / "scenes" / "scene0.json").read_bytes()
+    f = tmp_path / "doc.json"
+    for doc in (base + b" x", base + base, base + b"}"):
+        f.write_bytes(doc)
+        strict = subprocess.run([str(PROBE), str(f)], capture_output=True, timeout=60).returncode
+        stream = subprocess.run([str(PROBE), "--stream", str(f)], capture_output=True, timeout=60).returncode
+        assert (strict, stream) == (1, 0)
+        assert not loader_accepts(doc)
+    for name, doc in edge_documents(base).items():
+        if name in ("trailing_garbage", "two_documents"):
+            continue
+        f.write_bytes(doc)
+        strict = subprocess.run([str(PROBE), str(f)], capture_output=True, timeout=60).returncode
+        stream = subprocess.run([str(PROBE), "--stream", str(f)], capture_output=True, timeout=60).returncode
+        assert strict == stream, name
