@@ -308,7 +308,7 @@ const std::string& as_string(const Value& v, const char* what) {
 // does not start a valid UTF-8 sequence is written as it is.
 static void dump_string(std::string& o, const std::string& s) {
     auto u16 = [&](unsigned cp) {
-        char b[8];
+        char b[16];
         std::snprintf(b, sizeof b, "\\u%04x", cp);
         o += b;
     };
