@@ -1962,7 +1962,8 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
         e.kind = sd.desc.kind;
         e.iters = sd.desc.iterations;
         e.bailout = sd.desc.bailout > 0.0f ? sd.desc.bailout : 16.0f;
-        e.mat = (int)std::floor(sd.desc.material[0]);
+        const float m0 = sd.desc.material[0];  // an id far outside the table is rejected below
+        e.mat = (m0 > -1e9f && m0 < 1e9f) ? (int)std::floor(m0) : -1000000000;
         e.c = v3(sd.pos);
         e.lo = v3(sd.pos) - v3(sd.size) * 0.5f;
         e.hi = v3(sd.pos) + v3(sd.size) * 0.5f;

@@ -273,17 +273,27 @@ double as_double(const Value& v, const char* what) {
 }
 
 float as_float(const Value& v, const char* what) {
-    if (v.kind == Kind::Int) return static_cast<float>(v.i);
-    if (v.kind == Kind::Uint) return static_cast<float>(v.u);  // directly, not through double
-    if (v.kind == Kind::Real) return static_cast<float>(v.d);
-    if (v.kind == Kind::Bool) return v.b ? 1.0f : 0.0f;
-    throw type_error(what, "a number");
+    float f;
+    if (v.kind == Kind::Int) f = static_cast<float>(v.i);
+    else if (v.kind == Kind::Uint) f = static_cast<float>(v.u);  // directly, not through double
+    else if (v.kind == Kind::Real) f = static_cast<float>(v.d);
+    else if (v.kind == Kind::Bool) return v.b ? 1.0f : 0.0f;
+    else throw type_error(what, "a number");
+    // beyond binary32's range the reference's field would hold an infinity that
+    // no later stage can use or save; such a value is an error here
+    if (!std::isfinite(f)) throw type_error(what, "a number in float's range");
+    return f;
 }
 
 int as_int(const Value& v, const char* what) {
     if (v.kind == Kind::Int) return static_cast<int>(v.i);
     if (v.kind == Kind::Uint) return static_cast<int>(v.u);
-    if (v.kind == Kind::Real) return static_cast<int>(v.d);
+    if (v.kind == Kind::Real) {
+        // nlohmann's get<int>() casts without a check (undefined beyond int's
+        // range); such a value is an error here
+        if (!(v.d > -2147483649.0 && v.d < 2147483648.0)) throw type_error(what, "a number in int's range");
+        return static_cast<int>(v.d);
+    }
     if (v.kind == Kind::Bool) return v.b ? 1 : 0;
     throw type_error(what, "a number");
 }

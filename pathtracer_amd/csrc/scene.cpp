@@ -392,7 +392,11 @@ int camera_to_push(const SceneModel& s, int shot, pt_push_constants& pc, std::st
 // built-in functions they call and verified against the exact sequence of
 // numeric literals each built-in kernel implements; the free parameters
 // (iterations, escape radius, material values) are read from that sequence.
-// Anything else is rejected.
+// Then the whole program must be the built-in one token for token (a 64-bit
+// fingerprint of its functions with the free parameters masked), so that a
+// valid program computing another shape is not rendered as the built-in one.
+// Whitespace, comments and the spelling of numbers do not count. Anything else
+// is rejected.
 namespace {
 
 struct Tok { std::string t; bool num; };
@@ -448,7 +452,7 @@ std::vector<Tok> tokenize(const std::string& src) {
     return out;
 }
 
-struct Func { std::string name; std::vector<Tok> body; };
+struct Func { std::string name; std::vector<Tok> head, body; };
 
 // Splits `float name(...) { ... }` definitions.
 bool split_functions(const std::vector<Tok>& t, std::vector<Func>& out) {
@@ -468,6 +472,7 @@ bool split_functions(const std::vector<Tok>& t, std::vector<Func>& out) {
                 else if (t[k].t == "}") { if (--depth == 0) break; }
             }
             if (k >= t.size()) return false;
+            f.head.assign(t.begin() + (long)i, t.begin() + (long)j);
             f.body.assign(t.begin() + (long)j + 1, t.begin() + (long)k);
             out.push_back(std::move(f));
             i = k + 1;
@@ -507,6 +512,45 @@ bool match_literals(const std::vector<double>& have, const std::vector<double>& 
 }
 
 const double P = std::nan("");  // placeholder
+
+// Token fingerprints (see fingerprint() below) of the reference's four SDF
+// programs and three kinds of sdfmaterial, as shipped in sdfs/*.glsl and in
+// the scenes that embed them (scenes/scene3..10.json): each kind has one.
+#define SDF_FP_MANDELBULB 0x2d1eecbac9e0ac6dull
+#define SDF_FP_MENGER 0x087bb4cf520f90bdull
+#define SDF_FP_BLOB 0x788e0e00fa30ff89ull
+#define SDF_FP_TERRAIN 0x25b927ec8dc4614cull
+#define SDFMAT_FP_CONSTANT 0x23a4b3641549a38eull
+#define SDFMAT_FP_RADIAL 0x0c5b2fb52fbaf614ull
+#define SDFMAT_FP_BLOB 0xfa8a08568569156dull
+
+// FNV-1a (64-bit) of a function's tokens: its header, then its body, numbers
+// written as their value ("1.0" and "1." are one token) and the literals at the
+// NaN positions of `pattern` (the free parameters) as "#". Literal positions
+// count the body's numeric tokens in order, as match_literals does.
+void fingerprint(const Func& f, const std::vector<double>& pattern, uint64_t& h) {
+    auto mix = [&](const std::string& t) {
+        for (unsigned char c : t) h = (h ^ c) * 1099511628211ull;
+        h = (h ^ 0x1Fu) * 1099511628211ull;  // token separator
+    };
+    char b[40];
+    for (const Tok& k : f.head) {
+        if (!k.num) { mix(k.t); continue; }
+        std::snprintf(b, sizeof b, "%.17g", std::strtod(k.t.c_str(), nullptr));
+        mix(b);
+    }
+    mix("{");
+    size_t lit = 0;
+    for (const Tok& k : f.body) {
+        if (!k.num) { mix(k.t); continue; }
+        const bool free = lit < pattern.size() && pattern[lit] != pattern[lit];
+        ++lit;
+        if (free) { mix("#"); continue; }
+        std::snprintf(b, sizeof b, "%.17g", std::strtod(k.t.c_str(), nullptr));
+        mix(b);
+    }
+    mix("}");
+}
 
 }  // namespace
 
@@ -548,19 +592,23 @@ int recognize_sdf(const std::string& glsl, pt_sdf_desc& d, std::string& err) {
     if (helpers.empty() && calls(sdf->body, "pow") && calls(sdf->body, "inversesqrt") && calls(sdf->body, "log") &&
         match_literals(lit, kMandel, got)) {
         d.kind = PT_SDF_MANDELBULB;
-        d.iterations = (int)got[0];
-        d.bailout = (float)got[1];
-        if (got[0] != (double)d.iterations || d.iterations < 0 || d.iterations > 64) {
+        if (!(got[0] >= 0.0 && got[0] <= 64.0) || got[0] != std::floor(got[0])) {
             err = "mandelbulb iteration count must be an integer in [0, 64]";
             return PT_ERR_UNSUPPORTED_SDF;
         }
+        if (!(got[1] > 0.0 && got[1] <= 3.4e38)) {
+            err = "mandelbulb escape radius must be a positive number in float's range";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+        d.iterations = (int)got[0];
+        d.bailout = (float)got[1];
     } else if (helpers.empty() && calls(sdf->body, "mod") && match_literals(lit, kMenger, got)) {
         d.kind = PT_SDF_MENGER;
-        d.iterations = (int)got[0];
-        if (got[0] != (double)d.iterations || d.iterations < 0 || d.iterations > 16) {
+        if (!(got[0] >= 0.0 && got[0] <= 16.0) || got[0] != std::floor(got[0])) {
             err = "menger level count must be an integer in [0, 16]";
             return PT_ERR_UNSUPPORTED_SDF;
         }
+        d.iterations = (int)got[0];
     } else if (helpers.empty() && count_calls(sdf->body, "smin") == 3 && match_literals(lit, kBlob, got)) {
         d.kind = PT_SDF_BLOB;
     } else if (helpers.size() == 1 && count_calls(helpers[0]->body, "sin") == 11 &&
@@ -572,7 +620,31 @@ int recognize_sdf(const std::string& glsl, pt_sdf_desc& d, std::string& err) {
         return PT_ERR_UNSUPPORTED_SDF;
     }
 
+    // The whole program, not only its literals and calls, must be the built-in
+    // one: every function but sdfmaterial, tokens in order, free parameters
+    // masked. A valid program that computes another shape (a sphere dropped from
+    // the blob, another operand somewhere) is rejected, not rendered as the
+    // built-in shape.
+    {
+        const std::vector<double> none;
+        const std::vector<double>& pat = d.kind == PT_SDF_MANDELBULB ? kMandel : d.kind == PT_SDF_MENGER ? kMenger : none;
+        uint64_t h = 14695981039346656037ull;
+        for (const Func& f : fns)
+            if (&f != mat) fingerprint(f, &f == sdf ? pat : none, h);
+        // one per kind: the reference's sdfs/*.glsl and the scenes that embed them
+        static const uint64_t kProgram[5] = {0, SDF_FP_MANDELBULB, SDF_FP_MENGER, SDF_FP_BLOB, SDF_FP_TERRAIN};
+        if (h != kProgram[d.kind]) {
+            err = "SDF program differs from the built-in one it resembles (its tokens, not only its constants, must match)";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+    }
+
     std::vector<double> mlit = literals(mat->body);
+    for (double v : mlit)
+        if (!(std::fabs(v) <= 3.4e38)) {
+            err = "sdfmaterial() values must be numbers in float's range";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
     if (d.kind == PT_SDF_BLOB) {
         if (count_calls(mat->body, "smin") != 3 || !match_literals(mlit, kBlobMat, got)) {
             err = "unrecognised blob sdfmaterial()";
@@ -590,6 +662,19 @@ int recognize_sdf(const std::string& glsl, pt_sdf_desc& d, std::string& err) {
         err = "unrecognised sdfmaterial() (supported: constant, radial mix, blob)";
         return PT_ERR_UNSUPPORTED_SDF;
     }
+    // sdfmaterial likewise, its material values masked
+    {
+        static const std::vector<double> kConst = {P}, kRadial = {P, P, P};
+        const std::vector<double>& pat =
+            d.material_kind == PT_SDFMAT_BLOB ? kBlobMat : d.material_kind == PT_SDFMAT_RADIAL ? kRadial : kConst;
+        uint64_t h = 14695981039346656037ull;
+        fingerprint(*mat, pat, h);
+        static const uint64_t kMaterial[3] = {SDFMAT_FP_CONSTANT, SDFMAT_FP_RADIAL, SDFMAT_FP_BLOB};
+        if (h != kMaterial[d.material_kind]) {
+            err = "sdfmaterial() differs from the built-in one it resembles (its tokens, not only its constants, must match)";
+            return PT_ERR_UNSUPPORTED_SDF;
+        }
+    }
     return PT_OK;
 }
 
@@ -603,6 +688,9 @@ namespace {
 // reference's 32-bit cast overflows above |x| = 21474.8).
 double round_decimal(double x) {
     const double p = 1e5;
+    // beyond int64 (and for NaN or infinity) there is nothing to round: the
+    // value is written as it is
+    if (!(std::fabs(x) * p < 9.2e18)) return x;
     double n = x >= 0.0 ? (double)(int64_t)(x * p + 0.5) : (double)(int64_t)(x * p - 0.5);
     return n / p;
 }

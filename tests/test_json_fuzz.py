@@ -1,9 +1,10 @@
-"""Mutation fuzzing of the scene JSON reader and writer on the CPU under
-AddressSanitizer and UndefinedBehaviorSanitizer (tests/json_fuzz.cpp, built
-with g++ from pathtracer_amd/csrc/json.cpp alone): no memory or undefined-
-behaviour error on mutants of the shipped scenes, every rejection carries a
-message, and for every accepted mutant the writer's text is a fixed point of
-parse-then-write."""
+"""Mutation fuzzing of the scene JSON reader and writer, the scene loader,
+packer and writer, and the SDF recogniser, on the CPU under AddressSanitizer
+and UndefinedBehaviorSanitizer (tests/json_fuzz.cpp, built with g++ from
+pathtracer_amd/csrc/json.cpp and scene.cpp): no memory or undefined-behaviour
+error on mutants of the shipped scenes and their SDF texts, every rejection
+carries a message, and for every accepted mutant the written text (JSON
+document and scene) is a fixed point of read-then-write."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -20,9 +21,10 @@ def fuzzer(tmp_path_factory):
         pytest.skip("g++ not found")
     exe = tmp_path_factory.mktemp("fuzz") / "json_fuzz"
     flags = ["-O1", "-g", "-std=c++17", "-Wall", "-Werror", "-fno-sanitize-recover=all",
-             "-fsanitize=address,undefined"]
+             "-fsanitize=address,undefined,float-cast-overflow"]
     r = subprocess.run([gxx, *flags, "-I", str(ROOT / "pathtracer_amd" / "csrc"), str(ROOT / "tests" / "json_fuzz.cpp"),
-                        str(ROOT / "pathtracer_amd" / "csrc" / "json.cpp"), "-o", str(exe)],
+                        str(ROOT / "pathtracer_amd" / "csrc" / "json.cpp"), str(ROOT / "pathtracer_amd" / "csrc" / "scene.cpp"),
+                        "-o", str(exe)],
                        capture_output=True, text=True, timeout=300)
     if r.returncode != 0 and "sanitize" in r.stderr:
         pytest.skip("sanitizer runtime not available: " + r.stderr[-200:])

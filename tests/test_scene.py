@@ -3,6 +3,7 @@ recogniser, against an independent restatement of UpdateFromJSON +
 UpdateUniformBuffer (src/pathtracer.cpp:2576-2722, 3642-3834) written here
 with Python's json module, on the reference's 11 shipped scenes."""
 import json
+import re
 
 import numpy as np
 import pytest
@@ -170,3 +171,62 @@ def test_integer_and_real_literals_convert_like_nlohmann(scene_texts):
     text2 = json.dumps(s).replace('"radius": 1.0', '"radius": 0.1000000000000000055511151231257827')
     ubo2, _, _ = pt.pack_json(text2)
     assert ubo2.objects[3] == np.float32(0.1)
+
+
+SHAPE_CHANGES = [
+    # valid GLSL computing another shape: each must be rejected, not rendered as the built-in
+    ("scene3", "smin(s1, s2)", "smin(s1, s1)"),                  # a blob sphere dropped
+    ("scene3", "), s3), s4)", "), s1), s4)"),                    # another one
+    ("scene4", "cost2 = cost * cost", "cost2 = cost * cosp"),    # another operand
+    ("scene4", "dot(vec2(z.x, z.z), vec2(z.x, z.z))", "dot(vec2(z.x, z.x), vec2(z.x, z.z))"),
+    ("scene10", "vec3 q = abs(p)", "vec3 q = abs(-p)"),
+    ("scene8", "float sdf(in vec3 p)", "float sdf(in vec3 q)"),  # the parameter renamed, the body still reads p
+]
+
+
+@pytest.mark.parametrize("scene,find,replace", SHAPE_CHANGES)
+def test_recogniser_rejects_other_programs(scene_texts, scene, find, replace):
+    glsl = json.loads(scene_texts[scene])["sdf"][0]["glsl"]
+    assert find in glsl
+    with pytest.raises(pt.PTError) as e:
+        pt.recognize_sdf(glsl.replace(find, replace, 1))
+    assert e.value.status == pt.PT_ERR_UNSUPPORTED_SDF
+
+
+def test_recogniser_ignores_layout_comments_and_number_spelling(scene_texts):
+    mb = json.loads(scene_texts["scene4"])["sdf"][0]["glsl"]
+    d = pt.recognize_sdf(mb.replace("vec3 z = p;", "vec3   z=p; // the orbit\n/* block */"))
+    assert (d.kind, d.iterations) == (pt.SDF_MANDELBULB, 4)
+    mg = json.loads(scene_texts["scene10"])["sdf"][0]["glsl"]
+    assert "vec3(1.0)" in mg
+    assert pt.recognize_sdf(mg.replace("vec3(1.0)", "vec3(1.)")).kind == pt.SDF_MENGER
+
+
+def code_positions(text: str) -> list:
+    """Indices of the digits of `text` outside comments."""
+    masked = re.sub(r"//[^\n]*|/\*.*?\*/", lambda m: " " * len(m.group(0)), text, flags=re.S)
+    return [i for i, c in enumerate(masked) if c.isdigit()]
+
+
+def desc_key(d):
+    return (d.kind, d.material_kind, d.iterations, d.bailout, tuple(d.material))
+
+
+@pytest.mark.parametrize("scene", ["scene3", "scene4", "scene6", "scene8", "scene10"])
+def test_every_digit_change_is_a_parameter_or_rejected(scene_texts, scene):
+    """Changing any one digit of the program (outside comments) either changes
+    the recognised parameters (a free parameter) or is rejected: no change is
+    accepted as the same built-in shape."""
+    glsl = json.loads(scene_texts[scene])["sdf"][0]["glsl"]
+    base = desc_key(pt.recognize_sdf(glsl))
+    silent = []
+    for i in code_positions(glsl):
+        for c in "0123456789":
+            if c == glsl[i]:
+                continue
+            try:
+                if desc_key(pt.recognize_sdf(glsl[:i] + c + glsl[i + 1:])) == base:
+                    silent.append((i, glsl[max(0, i - 20):i + 5]))
+            except pt.PTError:
+                pass
+    assert not silent, silent[:5]
