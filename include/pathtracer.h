@@ -287,11 +287,10 @@ pt_status pt_math_eval_device(pt_ctx* ctx, int fn, const float* x, const float* 
  * at most 2^30; 8 n bytes of HBM traffic per launch; kernel k_peak_copy).
  * what = 1: an FP32 fused multiply-add loop of n rounds (at most 2^20) over 8
  * independent chains per thread, in a grid of 16 blocks of 256 threads per CU
- * (kernel k_peak_fma). Launches one
- * warm-up (k_peak_*_warmup) and then `reps` timed launches on the context's
- * stream and returns when they are done; with profiling on (pt_profile_enable)
- * pt_profile_read gives their durations. *units_out: the bytes (what 0) or
- * flops (what 1) of one launch. */
+ * (kernel k_peak_fma). Launches one warm-up (k_peak_*_warmup) and then `reps`
+ * timed launches on the context's stream, and returns when they are done; with
+ * profiling on (pt_profile_enable) pt_profile_read gives their durations.
+ * *units_out: the bytes (what 0) or flops (what 1) of one launch. */
 pt_status pt_peak_probe(pt_ctx* ctx, int what, size_t n, int reps, double* units_out);
 
 /* ---- Serving --------------------------------------------------------------
