@@ -243,6 +243,56 @@ def measured_peaks():
     return None
 
 
+def roofline_of(prof: dict, pcnt: dict, workload: str, tag: str):
+    """The roofline object of the dominant kernel: prof maps kernel names to
+    (launches, total ms) from the HIP events, pcnt holds the device counters of
+    the same dispatches. SDF marches are priced in FP32 flops against the VALU
+    peak, the other kernels in algorithmic bytes against HBM; both carry the
+    measured peak beside the published one."""
+    roofline = None
+    mp = measured_peaks()
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1][1])
+        name, (launches, total_ms) = dom
+        terms = BYTES_PER_UNIT.get(name)
+        fl = FLOPS_COUNTER.get(name) if workload == "sdf" else None
+        if fl and launches > 0:
+            flops = pcnt.get(fl, 0)
+            achieved = flops / (total_ms * 1e-3) / 1e12
+            roofline = {
+                "bound": "valu",
+                "achieved": round(achieved, 3),
+                "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
+                "traffic": traffic_from_profiles(name, tag),
+                # beside the published peak: the FMA loop measured on this device
+                "peak_measured": mp and mp["TFLOP/s"],
+                "frac_measured": mp and round(achieved / mp["TFLOP/s"], 5),
+                "kernel": name,
+                "avg_launch_ms": round(total_ms / launches, 4),
+                "algorithmic_flops_per_launch": round(flops / launches),
+            }
+        elif terms and launches > 0:
+            total_bytes = sum(bpu * pcnt.get(unit, 0) for bpu, unit in terms)
+            achieved = total_bytes / (total_ms * 1e-3) / 1e9
+            roofline = {
+                "bound": "hbm",
+                "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic_from_profiles(name, tag),
+                # beside the published peak: the streaming copy measured on this device
+                "peak_measured": mp and mp["GB/s"],
+                "frac_measured": mp and round(achieved / mp["GB/s"], 5),
+                "kernel": name,
+                "avg_launch_ms": round(total_ms / launches, 4),
+                "algorithmic_bytes_per_launch": round(total_bytes / launches),
+            }
+    return roofline
+
+
 def cpu_baseline(args, scene_text: str, W: int, H: int, depth: int, full_spp: int = 0):
     """The FP64 oracle tracer (CPU restatement with its own BVH) on a bounded
     sample of the same workload: the full frame at a number of samples per
@@ -792,47 +842,7 @@ def trace_bench(args):
         prof = ctx.profile_read(reset=True)
         ctx.profile(False)
 
-    roofline = None
-    mp = measured_peaks()
-    if prof:
-        dom = max(prof.items(), key=lambda kv: kv[1][1])
-        name, (launches, total_ms) = dom
-        terms = BYTES_PER_UNIT.get(name)
-        fl = FLOPS_COUNTER.get(name) if args.workload == "sdf" else None
-        if fl and launches > 0:
-            flops = pcnt.get(fl, 0)
-            achieved = flops / (total_ms * 1e-3) / 1e12
-            roofline = {
-                "bound": "valu",
-                "achieved": round(achieved, 3),
-                "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-                "traffic": traffic_from_profiles(name, tag),
-                # beside the published peak: the FMA loop measured on this device
-                "peak_measured": mp and mp["TFLOP/s"],
-                "frac_measured": mp and round(achieved / mp["TFLOP/s"], 5),
-                "kernel": name,
-                "avg_launch_ms": round(total_ms / launches, 4),
-                "algorithmic_flops_per_launch": round(flops / launches),
-            }
-        elif terms and launches > 0:
-            total_bytes = sum(bpu * pcnt.get(unit, 0) for bpu, unit in terms)
-            achieved = total_bytes / (total_ms * 1e-3) / 1e9
-            roofline = {
-                "bound": "hbm",
-                "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS,
-                "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic_from_profiles(name, tag),
-                # beside the published peak: the streaming copy measured on this device
-                "peak_measured": mp and mp["GB/s"],
-                "frac_measured": mp and round(achieved / mp["GB/s"], 5),
-                "kernel": name,
-                "avg_launch_ms": round(total_ms / launches, 4),
-                "algorithmic_bytes_per_launch": round(total_bytes / launches),
-            }
+    roofline = roofline_of(prof, pcnt, args.workload, tag)
     full = spp if args.workload == "cornell" else 0
     if args.no_cpu_baseline:
         cpu = None

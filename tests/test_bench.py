@@ -1,6 +1,7 @@
 """bench.py's contract on the CPU: one GPU only, the workloads' shapes, and the
 roofline tables naming kernels and counters that exist, so that a renamed
 launch cannot silently drop the roofline of the dominant kernel."""
+import json
 import re
 import subprocess
 import sys
@@ -80,3 +81,34 @@ def test_measured_peaks_read_from_the_committed_profile():
 
 def test_peak_probe_flag():
     assert args("--peak-probe").peak_probe and not args().peak_probe
+
+
+def test_roofline_of_an_hbm_kernel():
+    """The headline's dominant kernel priced in algorithmic bytes against the
+    published and the measured HBM peaks, traffic from its committed profile."""
+    name = "k_ext_intersect"
+    (bpu, unit), = bench.BYTES_PER_UNIT[name]
+    prof = {name: (12, 36.0), "k_ext_shade_lambert": (12, 10.0)}
+    pcnt = {unit: 10**9}
+    r = bench.roofline_of(prof, pcnt, "spheres", "spheres4096_1920x1080")
+    achieved = bpu * 1e9 / (36.0e-3) / 1e9
+    assert r["bound"] == "hbm" and r["kernel"] == name and r["unit"] == "GB/s"
+    assert r["achieved"] == round(achieved, 2) and r["frac"] == round(achieved / bench.HBM_PEAK_GBS, 5)
+    mp = bench.measured_peaks()
+    assert r["peak_measured"] == mp["GB/s"] and r["frac_measured"] == round(achieved / mp["GB/s"], 5)
+    assert r["traffic"] == bench.traffic_from_profiles(name, "spheres4096_1920x1080") and r["traffic"] > 0
+    assert r["avg_launch_ms"] == 3.0 and r["algorithmic_bytes_per_launch"] == round(bpu * 1e9 / 12)
+
+
+def test_roofline_of_an_sdf_march():
+    name = "k_ext_march_mandelbulb"
+    counter = bench.FLOPS_COUNTER[name]
+    r = bench.roofline_of({name: (8, 16.0)}, {counter: 2 * 10**11}, "sdf", "sdf_1920x1080")
+    achieved = 2e11 / 16e-3 / 1e12
+    assert r["bound"] == "valu" and r["unit"] == "TFLOP/s" and r["achieved"] == round(achieved, 3)
+    assert r["peak"] == bench.FP32_PEAK_TFLOPS and r["frac_measured"] == round(achieved / bench.measured_peaks()["TFLOP/s"], 5)
+    json.dumps(r)  # the bench line serialises it
+
+
+def test_roofline_of_nothing_profiled():
+    assert bench.roofline_of({}, {}, "spheres", "x") is None
