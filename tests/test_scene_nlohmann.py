@@ -356,3 +356,43 @@ def test_trailing_content_follows_the_strict_parse(tmp_path):
         strict = subprocess.run([str(PROBE), str(f)], capture_output=True, timeout=60).returncode
         stream = subprocess.run([str(PROBE), "--stream", str(f)], capture_output=True, timeout=60).returncode
         assert strict == stream, name
+
+
+def json_leaves(node, ptr=""):
+    if isinstance(node, dict):
+        for k, v in node.items():
+            yield from json_leaves(v, f"{ptr}/{k}")
+    elif isinstance(node, list):
+        for i, v in enumerate(node):
+            yield from json_leaves(v, f"{ptr}/{i}")
+    elif isinstance(node, (int, float)) and not isinstance(node, bool):
+        yield ptr, node
+
+
+# keys the writer saves exactly (extended schema); the reference's own float
+# fields are saved on RoundDecimal's 1e-5 grid
+EXACT_KEYS = {"quad", "gaussians", "cauchy", "sellmeier", "n", "k", "environment", "render", "bailout", "edge1",
+              "edge2"}
+
+
+@pytest.mark.parametrize("name", ["cornell", "spheres", "sdf"])
+def test_extended_fields_read_as_nlohmann_reads_them(name, tmp_path):
+    """The generated scenes the benchmarks render: every float the loader holds
+    for an extended field (written back exactly by the writer) is nlohmann's
+    get<float>() of the same literal; reference fields are within the 1e-5
+    grid they are saved on."""
+    text = generated_scenes()[name]
+    vals = nlohmann_values(text, tmp_path)
+    out = json.loads(pt.json_rewrite(text))
+    exact = rounded = 0
+    for ptr, v in json_leaves(out):
+        if ptr not in vals:
+            continue
+        want = vals[ptr][0]
+        if EXACT_KEYS & set(ptr.split("/")):
+            assert np.float32(v).view(np.uint32) == want.view(np.uint32), ptr
+            exact += 1
+        else:
+            assert abs(float(v) - float(want)) <= 0.5e-5 + abs(float(want)) * 2.0 ** -23, ptr
+            rounded += 1
+    assert exact > 10 and rounded > 10
