@@ -166,6 +166,8 @@ typedef struct pt_counters {
     uint64_t march_flops_blob;
     uint64_t march_flops_terrain;
     uint64_t shadow_march_flops;
+    uint64_t bad_refs;        /* extended mode: hit records that name no primitive of their scene; classified as
+                                 misses and counted here (0 in a correct run) */
 } pt_counters;
 
 typedef struct pt_ctx pt_ctx;

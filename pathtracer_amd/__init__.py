@@ -132,6 +132,7 @@ class Counters(C.Structure):
         ("march_flops_blob", C.c_uint64),
         ("march_flops_terrain", C.c_uint64),
         ("shadow_march_flops", C.c_uint64),
+        ("bad_refs", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

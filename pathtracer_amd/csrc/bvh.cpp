@@ -47,6 +47,16 @@ struct Builder {
         for (size_t i = 0; i < p.size(); ++i) order[i] = (int)i;
     }
     float centroid(int i, int axis) const { return 0.5f * (prims[(size_t)i].lo[axis] + prims[(size_t)i].hi[axis]); }
+    // SAH bin of a centroid, clamped to [0, kBins-1]; a NaN offset (non-finite
+    // boxes, which callers reject before building) lands in bin 0 rather than
+    // in an out-of-range int conversion.
+    template <int kBins>
+    static int bin_of(float c, float lo, float scale) {
+        const float f = (c - lo) * scale;
+        if (!(f > 0.0f)) return 0;
+        if (f >= (float)(kBins - 1)) return kBins - 1;
+        return (int)f;
+    }
 
     int build(int first, int count) {
         int id = (int)nodes.size();
@@ -76,7 +86,7 @@ struct Builder {
             for (int b = 0; b < kBins; ++b) bin_box[b] = empty_box();
             for (int i = first; i < first + count; ++i) {
                 int p = order[(size_t)i];
-                int b = std::min(kBins - 1, (int)((centroid(p, axis) - lo) * scale));
+                int b = bin_of<kBins>(centroid(p, axis), lo, scale);
                 bin_n[b]++;
                 grow(bin_box[b], prims[(size_t)p]);
             }
@@ -116,7 +126,7 @@ struct Builder {
         } else {
             float lo = cbox.lo[best_axis], scale = kBins / (cbox.hi[best_axis] - lo);
             auto it = std::partition(order.begin() + first, order.begin() + first + count, [&](int p) {
-                int b = std::min(kBins - 1, (int)((centroid(p, best_axis) - lo) * scale));
+                int b = bin_of<kBins>(centroid(p, best_axis), lo, scale);
                 return b < best_split;
             });
             mid = (int)(it - order.begin());

@@ -384,11 +384,11 @@ pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_const
     if (s->ctx != ctx) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: scene of another context");
     if (!s->has_model) return set_err(ctx, PT_ERR_STATE, "pt_render_features: needs a scene loaded from JSON");
     const int w = pc->resolution[0], h = pc->resolution[1];
-    if (w <= 0 || h <= 0 || pc->samplesPerFrame <= 0 || pc->pathLength < 0 || pc->pathLength > 1024 ||
-        pc->currentSamples < pc->samplesPerFrame)
+    if (w <= 0 || h <= 0 || (int64_t)w * h > (int64_t)1 << 28 || pc->samplesPerFrame <= 0 || pc->pathLength < 0 ||
+        pc->pathLength > 1024 || pc->currentSamples < pc->samplesPerFrame)
         return set_err(ctx, PT_ERR_INVALID_ARG,
-                       "pt_render_features: need a positive resolution, samplesPerFrame > 0, 0 <= pathLength <= 1024, "
-                       "currentSamples >= samplesPerFrame");
+                       "pt_render_features: need a resolution of 1 to 2^28 pixels, samplesPerFrame > 0, "
+                       "0 <= pathLength <= 1024, currentSamples >= samplesPerFrame");
     const size_t P = (size_t)w * h;
     if (n_floats != P * 12) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_render_features: size mismatch");
     (void)hipSetDevice(ctx->device);
@@ -473,6 +473,7 @@ pt_status pt_counters_get(pt_ctx* ctx, pt_counters* out) {
     out->march_flops_blob = h[ptd::kStatMarchFlops + 2];
     out->march_flops_terrain = h[ptd::kStatMarchFlops + 3];
     out->shadow_march_flops = h[ptd::kStatShadowMarchFlops];
+    out->bad_refs = h[ptd::kStatBadRefs];
     return PT_OK;
 }
 

@@ -656,18 +656,31 @@ __device__ __forceinline__ Surf surface(const ExtSceneDev& S, uint32_t ref, f3 o
     }
     return s;
 }
-__device__ __forceinline__ int hit_class(const ExtSceneDev& S, uint32_t ref) {
-    if (ref == kMiss) return Q_MISS;
-    uint32_t idx = ref & kRefMask;
+// True for a hit record that is neither a miss nor a primitive of the scene.
+__device__ __forceinline__ bool ref_is_bad(const ExtSceneDev& S, uint32_t ref) {
+    if (ref == kMiss) return false;
     const uint32_t kind = ref >> kRefShift;
-    // A reference to no primitive of the scene is treated as a miss, so no
-    // later kernel indexes the scene tables with it.
     const uint32_t limit = kind == REF_SPHERE ? (uint32_t)S.nsph
                          : kind == REF_QUAD   ? (uint32_t)S.nquad
                          : kind == REF_PLANE  ? (uint32_t)S.nplane
                          : kind == REF_SDF    ? (uint32_t)S.nsdf
                                               : 0u;
-    if (idx >= limit) return Q_MISS;
+    return (ref & kRefMask) >= limit;
+}
+// Counts the lanes whose hit record is bad (pt_counters.bad_refs). Call from
+// converged control flow.
+__device__ __forceinline__ void count_bad_refs(unsigned long long* stats, bool bad) {
+    const uint64_t m = __ballot(bad);
+    if (m && __lane_id() == 0) atomicAdd(stat_slot(stats, kStatBadRefs), (unsigned long long)__popcll(m));
+}
+__device__ __forceinline__ int hit_class(const ExtSceneDev& S, uint32_t ref) {
+    if (ref == kMiss) return Q_MISS;
+    const uint32_t idx = ref & kRefMask;
+    const uint32_t kind = ref >> kRefShift;
+    // A reference to no primitive of the scene is classified as a miss, so no
+    // later kernel indexes the scene tables with it; the classify passes count
+    // such records in pt_counters.bad_refs.
+    if (ref_is_bad(S, ref)) return Q_MISS;
     int mat, light;
     switch (kind) {
         case REF_SPHERE: { int2 ml = S.sph_ml[idx]; mat = ml.x; light = ml.y; break; }
@@ -862,6 +875,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const Ext
 // Appends the path to its material queue; every thread of the block calls it.
 __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& Wv, bool active, int p, uint32_t ref) {
     const int cls = active ? hit_class(S, ref) : -1;
+    count_bad_refs(Wv.stats, active && ref_is_bad(S, ref));
     int* const ctr[N_MATQ] = {qlen(Wv, CNT_MAT + 0), qlen(Wv, CNT_MAT + 1), qlen(Wv, CNT_MAT + 2),
                               qlen(Wv, CNT_MAT + 3), qlen(Wv, CNT_MAT + 4)};
     int slot[N_MATQ];
@@ -1193,16 +1207,20 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
     if ((int)threadIdx.x < N_MATQ) s_base[threadIdx.x] = 0;
     __syncthreads();
     int cnt[N_MATQ] = {};
+    int bad = 0;
     for (int r = 0; r < rounds; ++r) {
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         if (idx < n) {
             const int p = Wv.q[cur][idx];
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
-            const int cls = hit_class(S, __float_as_uint(Wv.hit[p].y));
+            const uint32_t ref = __float_as_uint(Wv.hit[p].y);
+            const int cls = hit_class(S, ref);
+            bad += ref_is_bad(S, ref) ? 1 : 0;
 #pragma unroll
             for (int q = 0; q < N_MATQ; ++q) cnt[q] += cls == q ? 1 : 0;
         }
     }
+    if (bad) atomicAdd(stat_slot(Wv.stats, kStatBadRefs), (unsigned long long)bad);
 #pragma unroll
     for (int q = 0; q < N_MATQ; ++q) {
         int v = cnt[q];
@@ -1924,6 +1942,10 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
             for (int v = 0; v < 4; ++v) { b.lo[k] = std::min(b.lo[k], comp(c[v], k)); b.hi[k] = std::max(b.hi[k], comp(c[v], k)); }
             float pad = 1e-6f * std::max(1.0f, std::fabs(b.hi[k]));
             b.lo[k] -= pad; b.hi[k] += pad;
+            if (!std::isfinite(b.lo[k]) || !std::isfinite(b.hi[k])) {
+                ctx->err = "quad/box bounds overflow the float range";
+                return PT_ERR_INVALID_ARG;
+            }
         }
         qbox.push_back(b);
         return PT_OK;
@@ -2011,6 +2033,10 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
             float c = k == 0 ? sph[i].x : (k == 1 ? sph[i].y : sph[i].z);
             sbox[i].lo[k] = c - sph[i].w;
             sbox[i].hi[k] = c + sph[i].w;
+            if (!std::isfinite(sbox[i].lo[k]) || !std::isfinite(sbox[i].hi[k])) {
+                ctx->err = "sphere bounds overflow the float range";
+                return PT_ERR_INVALID_ARG;
+            }
         }
         sref[i] = (uint32_t)i;
     }

@@ -158,10 +158,12 @@ __device__ __forceinline__ void block_append(int* const (&ctr)[NQ], uint32_t wan
 // one 256-byte line per copy; a block adds into copy blockIdx.x % kStatLines and
 // the host sums the copies, so per-wave additions from a whole grid do not
 // serialise on one address.
-constexpr int kStatSlots = 19, kStatLines = 64, kStatStride = 32;
+constexpr int kStatSlots = 20, kStatLines = 64, kStatStride = 32;
 // Slots 14..17: FP32 operations of extension-ray sphere tracing by SDF kind
-// (PT_SDF_MANDELBULB .. PT_SDF_TERRAIN); slot 18: those of shadow-ray marching.
-constexpr int kStatMarchFlops = 14, kStatShadowMarchFlops = 18;
+// (PT_SDF_MANDELBULB .. PT_SDF_TERRAIN); slot 18: those of shadow-ray marching;
+// slot 19: hit records naming no primitive of their scene (always 0 unless a
+// kernel left a hit record unwritten or corrupt).
+constexpr int kStatMarchFlops = 14, kStatShadowMarchFlops = 18, kStatBadRefs = 19;
 __device__ __forceinline__ unsigned long long* stat_slot(unsigned long long* stats, int slot) {
     return stats + (size_t)(blockIdx.x % kStatLines) * kStatStride + slot;
 }

@@ -115,6 +115,9 @@ int64_t estimate_bytes(const Request& r) {
     return (int64_t)r.P * (r.denoise ? 112 : 28) + prims * 320 + (1 << 20);
 }
 
+// Paths per batched wave: the context's wave capacity, else 2^22.
+int64_t server_wave_paths(const pt_ctx* ctx) { return ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22); }
+
 void release_request(pt_server* s, Request& r) {
     if (r.scene) pt_scene_release(r.scene);
     r.scene = nullptr;
@@ -281,6 +284,9 @@ pt_status pt_server_submit(pt_server* s, const pt_request_desc* d, double t_arri
         return fail(s, PT_ERR_INVALID_ARG, "pt_server_submit: bad request");
     if (d->denoise && !s->dn)
         return fail(s, PT_ERR_STATE, "pt_server_submit: denoising asked for but no denoiser attached");
+    // a batched wave traces whole requests: one sample of every pixel must fit it
+    if ((int64_t)d->width * d->height > server_wave_paths(s->ctx))
+        return fail(s, PT_ERR_CAPACITY, "pt_server_submit: request has more pixels than a batched wave holds paths");
     auto r = std::make_unique<Request>();
     r->id = s->next_id++;
     r->json.assign(d->json, d->len);
@@ -334,7 +340,7 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             return remaining_cost(s, *s->reqs[a]) < remaining_cost(s, *s->reqs[b]);
         });
-    const int64_t wave = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 22);
+    const int64_t wave = server_wave_paths(ctx);
     int64_t left = wave;
     std::vector<ptk::ExtBatchItem> items;
     std::vector<Request*> in;
@@ -365,23 +371,39 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
     pt_counters c0{}, c1{};
     (void)pt_counters_get(ctx, &c0);
     const auto t_start = Clock::now();
+    // Once a wave may have added samples to the moment buffers, an error leaves
+    // those requests' sums ahead of their sample counts; retracing the same
+    // sample indices would count them twice, so such requests fail instead.
+    auto fail_wave = [&](int code) -> pt_status {
+        for (Request* r : in) {
+            r->state = PT_REQ_FAILED;
+            r->t_done = now_s(s);
+            auto it = std::find(s->active.begin(), s->active.end(), r->id);
+            if (it != s->active.end()) s->active.erase(it);
+        }
+        return fail(s, code, ctx->err);
+    };
     int st = ptk::ext_dispatch_batch(ctx, items.data(), (int)items.size());
-    if (st != PT_OK) return fail(s, st, ctx->err);
+    if (st != PT_OK) return fail_wave(st);
     const size_t nline = (size_t)kRelLines * kRelStride;
     hipError_t e = hipMemsetAsync(s->d_rel, 0, items.size() * nline * sizeof(double), ctx->stream);
-    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
+    if (e != hipSuccess) return fail_wave(ptk::hip_fail(ctx, e, "pt_server_step"));
     for (size_t i = 0; i < items.size(); ++i) {
         st = ptk::ext_rel_error(ctx, in[i]->P, in[i]->sum, in[i]->sumsq, in[i]->nsamp, s->d_rel + i * nline);
-        if (st != PT_OK) return fail(s, st, ctx->err);
+        if (st != PT_OK) return fail_wave(st);
     }
     std::vector<double> lines(items.size() * nline);
     e = hipMemcpyAsync(lines.data(), s->d_rel, lines.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return fail(s, ptk::hip_fail(ctx, e, "pt_server_step"), ctx->err);
+    if (e != hipSuccess) return fail_wave(ptk::hip_fail(ctx, e, "pt_server_step"));
     const double dt = std::chrono::duration<double>(Clock::now() - t_start).count();
     (void)pt_counters_get(ctx, &c1);
     const double total_paths = (double)(wave - left);
     s->sec_per_path = s->sec_per_path > 0.0 ? 0.8 * s->sec_per_path + 0.2 * dt / total_paths : dt / total_paths;
+    // every traced request takes its samples even when a denoiser run fails;
+    // the failing request ends PT_REQ_FAILED and the first error is returned
+    pt_status first_err = PT_OK;
+    std::string first_msg;
     for (size_t i = 0; i < items.size(); ++i) {
         Request& r = *in[i];
         double v = 0.0, m = 0.0;
@@ -397,12 +419,22 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
         r.gpu_s += dt * share;
         r.segments += (uint64_t)((double)(c1.segments - c0.segments) * share);
         bool done = (r.spp >= s->cfg.min_spp && r.rel <= r.target) || r.spp >= r.spp_max;
+        bool failed = false;
         if (r.denoise && r.spp >= s->cfg.min_spp && !(r.rel <= r.target)) {
             st = denoise_step(s, r, r.spp >= r.spp_max);
-            if (st != PT_OK) return st;
+            if (st != PT_OK) {
+                if (first_err == PT_OK) {
+                    first_err = st;
+                    first_msg = s->err;
+                }
+                r.state = PT_REQ_FAILED;
+                r.t_done = now_s(s);
+                s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
+                failed = true;
+            }
             done = done || r.denoised;
         }
-        if (done) {
+        if (done && !failed) {
             r.state = PT_REQ_DONE;
             r.t_done = now_s(s);
             s->active.erase(std::find(s->active.begin(), s->active.end(), r.id));
@@ -411,6 +443,10 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
             (void)pt_server_status(s, r.id, &stepped[i]);
             *n_stepped = (int32_t)i + 1;
         }
+    }
+    if (first_err != PT_OK) {
+        s->err = first_msg;
+        return first_err;
     }
     return PT_OK;
 }

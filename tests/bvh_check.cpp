@@ -123,6 +123,13 @@ int main(int argc, char** argv) {
         } else if (layout == 2) {
             s.c[0] = u01(rng);
             s.c[1] = s.c[2] = 0.5f;
+        } else if (layout == 4) {
+            // random boxes plus one whose upper corner overflows to +inf
+            for (int k = 0; k < 3; ++k) s.c[k] = u01(rng);
+            if (i == n / 2) {
+                for (int k = 0; k < 3; ++k) s.c[k] = 3e38f;
+                s.r = 1e38f;
+            }
         } else {
             const float base = (i % 2) ? 0.2f : 0.8f;
             for (int k = 0; k < 3; ++k) s.c[k] = base + 1e-4f * u01(rng);
@@ -198,6 +205,10 @@ int main(int argc, char** argv) {
     for (int k = 0; k < t.n_nodes; ++k)
         if (!node_seen[(size_t)k]) return fail("unreachable node", k, 0);
 
+    if (layout == 4) {  // the structure holds; ray tests need finite spheres
+        std::printf("ok nodes=%d depth=%d (non-finite box)\n", t.n_nodes, t.depth);
+        return 0;
+    }
     // rays: closest hit by traversal against all spheres, and the stack bound
     std::uniform_real_distribution<float> um(-1.0f, 1.0f);
     int max_seen = 0;

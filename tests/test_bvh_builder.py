@@ -15,12 +15,12 @@ ROOT = Path(__file__).resolve().parent.parent
 CSRC = ROOT / "pathtracer_amd" / "csrc"
 
 
-def build(tmp: Path, bvh_cpp: Path) -> Path:
+def build(tmp: Path, bvh_cpp: Path, flags=()) -> Path:
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not found")
     exe = tmp / "bvh_check"
-    subprocess.run([gxx, "-O2", "-std=c++17", "-Wall", "-I", str(CSRC), str(ROOT / "tests" / "bvh_check.cpp"),
+    subprocess.run([gxx, "-O2", "-std=c++17", "-Wall", *flags, "-I", str(CSRC), str(ROOT / "tests" / "bvh_check.cpp"),
                     str(bvh_cpp), "-o", str(exe)], check=True, timeout=180)
     return exe
 
@@ -75,3 +75,32 @@ def test_checker_catches_planted_mistakes(tmp_path, find, replace, message):
     exe = build(tmp_path, tmp_path / "bvh.cpp")
     rc, out = run(exe, 4096, 8, 4, 0, 1)
     assert rc == 1 and message in out, out
+
+
+UBSAN = ("-fsanitize=undefined,float-cast-overflow", "-fno-sanitize-recover=all", "-g")
+
+
+@pytest.mark.parametrize("width", [2, 4])
+def test_overflowing_box_is_binned_in_range(tmp_path, width):
+    """A primitive whose box overflows to +inf (a legal float centre 3e38 with
+    radius 1e38) makes the centroid bounds infinite and the SAH bin offset NaN.
+    The builder clamps the bin index instead of converting NaN to int (undefined
+    behaviour, INT_MIN on x86, then an out-of-range bin write). Scene upload
+    rejects such primitives before building; the builder stays defined anyway."""
+    exe = build(tmp_path, CSRC / "bvh.cpp", UBSAN)
+    rc, out = run(exe, 300, width, 1, 4, 5 + width)
+    assert rc == 0 and "non-finite box" in out, out
+
+
+def test_overflowing_box_fails_without_the_clamp(tmp_path):
+    """The sanitized check above fails when the bin index is the plain cast."""
+    src = (CSRC / "bvh.cpp").read_text()
+    find = """        const float f = (c - lo) * scale;
+        if (!(f > 0.0f)) return 0;"""
+    planted = src.replace(find, """        const float f = (c - lo) * scale;
+        if ((int)f < 0) return 0;""")
+    assert planted != src
+    (tmp_path / "bvh.cpp").write_text(planted)
+    exe = build(tmp_path, tmp_path / "bvh.cpp", UBSAN)
+    rc, out = run(exe, 300, 2, 1, 4, 7)
+    assert rc != 0, out

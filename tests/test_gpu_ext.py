@@ -196,6 +196,9 @@ def test_counters_and_no_nans(gpu_ctx):
     c = gpu_ctx.counters()
     assert not np.isnan(img).any() and c["nan_samples"] == 0
     assert c["paths"] == 64 * 36 * 4
+    # no hit record names a primitive outside the scene (such a record would be
+    # classified as a miss and hide in the class sum below)
+    assert c["bad_refs"] == 0
     # every segment ends in exactly one class queue
     assert c["segments"] == c["escaped"] + c["emitter_hits"] + c["lambert_hits"] + c["dielectric_hits"] + c["conductor_hits"]
     assert c["dielectric_hits"] > 0 and c["conductor_hits"] > 0 and c["shadow_segments"] > 0
@@ -287,6 +290,7 @@ def test_persistent_traversal_is_invisible(scene_texts, name):
             imgs.append(gpu_render(c, text, 64, 36, 4, depth))
             cnts.append(c.counters())
     assert np.array_equal(imgs[0], imgs[1])
+    assert cnts[0]["bad_refs"] == 0 and cnts[1]["bad_refs"] == 0
     for k in ("segments", "shadow_segments", "sdf_steps", "lambert_hits", "dielectric_hits", "conductor_hits",
               "bvh_nodes", "shadow_bvh_nodes", "march_flops_mandelbulb", "march_flops_menger", "shadow_march_flops"):
         assert cnts[0][k] == cnts[1][k], k

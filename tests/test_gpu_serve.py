@@ -136,6 +136,30 @@ def test_bad_requests_are_rejected(gpu_ctx):
         assert srv.step() is None
 
 
+def test_request_larger_than_a_wave_is_rejected_at_submit():
+    """A batched wave traces at least one sample of every pixel of a request,
+    so a request with more pixels than the wave holds paths is refused when it
+    is submitted (PT_ERR_CAPACITY), and one that fits is still served."""
+    with pt.Context(0, wave_paths=4096) as ctx, pt.Server(ctx) as srv:
+        with pytest.raises(pt.PTError) as e:
+            srv.submit(no_caustic_cornell(), 65, 64, spp_max=4, path_length=4)
+        assert e.value.status == pt.PT_ERR_CAPACITY
+        rid = srv.submit(no_caustic_cornell(), 64, 64, spp_max=4, path_length=4, target=1e-6)
+        st = run_until_done(srv, [rid])
+        assert st[rid].state == pt.REQ_DONE and st[rid].spp == 4
+
+
+def test_scene_with_overflowing_bounds_is_rejected(gpu_ctx):
+    """A sphere whose box overflows the float range (centre 3e38, radius 1e38)
+    is refused at upload instead of reaching the BVH builder."""
+    d = json.loads(scenes.random_spheres(n=8, scene_seed=2, wavelengths=4))
+    d["sphere"][3]["position"] = [3e38, 3e38, 3e38]
+    d["sphere"][3]["radius"] = 1e38
+    with pytest.raises(pt.PTError) as e:
+        gpu_ctx.scene_from_json(json.dumps(d))
+    assert e.value.status == pt.PT_ERR_INVALID_ARG and "float range" in str(e.value)
+
+
 def attach_shipped_denoiser(ctx, srv, size=128, batch=1):
     from pathtracer_amd import denoise_train as T
     dn = pt.Denoiser(ctx, batch, size, size)
