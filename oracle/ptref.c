@@ -467,13 +467,11 @@ void ptref_solve_quartic(float a, float b, float c, float d, float e, float root
     solve_quartic(a, b, c, d, e, roots, real);
 }
 
-static int isect_cyclide(Ray r, v3 pos, v3 rot, v3 scale, float A, float B, float C, float Dd, float mat,
-                         float light, Hit* h) {
-    m3 M = rotation(rot);
-    v3 ol = div3(vm3(sub3(r.o, pos), M), scale);
-    v3 dl = div3(vm3(r.d, M), scale);
-    v3 o = V3(ol.x, ol.z, ol.y); /* .xzy */
-    v3 d = V3(dl.x, dl.z, dl.y);
+/* Coefficients a[4]..a[0] of the quartic in t whose roots are the hits of the
+ * ray o + t d (object space, .xzy swizzled) with the Dupin cyclide
+ * (|p|^2 + B^2 - D^2)^2 - 4 (A x - C D)^2 - 4 B^2 y^2 = 0
+ * (src/shader.comp:636-647). */
+static void cyclide_coeffs(v3 o, v3 d, float A, float B, float C, float Dd, float a[5]) {
     v3 dd = mul3(d, d), oo = mul3(o, o), od = mul3(o, d);
     v3 d_yzx = V3(d.y, d.z, d.x), d_zxy = V3(d.z, d.x, d.y);
     v3 o_yzx = V3(o.y, o.z, o.x), o_zxy = V3(o.z, o.x, o.y);
@@ -488,9 +486,24 @@ static int isect_cyclide(Ray r, v3 pos, v3 rot, v3 scale, float A, float B, floa
     float a0 = dot3(oo, oo) + 2.0f * dot3(oo, mul3(o_yzx, o_yzx)) + B * B * B * B + Dd * Dd * Dd * Dd -
                2.0f * B * B * Dd * Dd - 4.0f * C * C * Dd * Dd + 8.0f * A * C * Dd * o.x +
                2.0f * (bb - dd2) * dot3(o, o) - 4.0f * (aa * o.x * o.x + bb * o.y * o.y);
+    a[4] = a4; a[3] = a3; a[2] = a2; a[1] = a1; a[0] = a0;
+}
+static void cyclide_local(Ray r, v3 pos, v3 rot, v3 scale, v3* o, v3* d) {
+    m3 M = rotation(rot);
+    v3 ol = div3(vm3(sub3(r.o, pos), M), scale);
+    v3 dl = div3(vm3(r.d, M), scale);
+    *o = V3(ol.x, ol.z, ol.y); /* .xzy */
+    *d = V3(dl.x, dl.z, dl.y);
+}
+static int isect_cyclide(Ray r, v3 pos, v3 rot, v3 scale, float A, float B, float C, float Dd, float mat,
+                         float light, Hit* h) {
+    v3 o, d;
+    cyclide_local(r, pos, rot, scale, &o, &d);
+    float cf[5];
+    cyclide_coeffs(o, d, A, B, C, Dd, cf);
     float roots[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     int real[4];
-    solve_quartic(a4, a3, a2, a1, a0, roots, real);
+    solve_quartic(cf[4], cf[3], cf[2], cf[1], cf[0], roots, real);
     float t = 1e6f;
     for (int i = 0; i < 4; ++i)
         if (real[i] && roots[i] < t && roots[i] > 0.0f) t = roots[i];
@@ -883,6 +896,17 @@ static v3 sample_cone(uint32_t* seed, float cosMax) {
     return norm3(V3(F_COS(phi) * st, F_SIN(phi) * st, ct + 1.0f));
 }
 static float mis2(float p1, float p2) { return p1 * p1 / (p1 * p1 + p2 * p2); }
+/* Solid-angle pdf of a light-sampled direction (src/shader.comp:1310-1312):
+ * light chosen with probability 1/nl, direction from sample_cone with
+ * cosd = its cosine to the cone axis. */
+static float light_pdf(float nl, float cosd, float cosMax) {
+    float lpdf = 1.0f / nl;
+    lpdf *= cosd / (PI_F * (1.0f - cosMax * cosMax));
+    return lpdf;
+}
+/* Diffuse bounce direction (src/shader.comp:1371): n plus a uniform point on
+ * the unit sphere, normalised, which is cosine-distributed about n. */
+static v3 sample_bsdf_dir(v3 n, uint32_t* seed) { return norm3(add3(n, sample_unit_sphere(seed))); }
 
 static v4 v4mul(v4 a, v4 b) { v4 r = {a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w}; return r; }
 static v4 v4muls(v4 a, float s) { v4 r = {a.x * s, a.y * s, a.z * s, a.w * s}; return r; }
@@ -901,8 +925,7 @@ static v4 sample_light_source(Ctx* c, v4 l, v4 T, Ray outRay, v3 n, Mat mat, uin
         float sinMax = fminf(brad * invd, 1.0f);
         float cosMax = sqrtf(1.0f - sinMax * sinMax);
         outRay.d = to_world(sample_cone(seed, cosMax), ldir);
-        float lpdf = 1.0f / U(c, U_NUM, 6);
-        lpdf *= dot3(outRay.d, ldir) / (PI_F * (1.0f - cosMax * cosMax));
+        float lpdf = light_pdf(U(c, U_NUM, 6), dot3(outRay.d, ldir), cosMax);
         *w = mis2(brdfPdf, lpdf);
         float ct = dot3(outRay.d, n);
         float death = 1.25f * fmaxf(*w - 0.2f, 0.0f);
@@ -947,7 +970,7 @@ static v4 trace_ray(Ctx* c, v4 l, v4* T, Ray* ray, uint32_t* seed, float* w, int
         }
         Ray out;
         out.o = fma3(ray->d, V3(h.t, h.t, h.t), ray->o);
-        out.d = norm3(add3(h.n, sample_unit_sphere(seed)));
+        out.d = sample_bsdf_dir(h.n, seed);
         float pdf = dot3(out.d, h.n) / PI_F;
         rad = sample_light_source(c, l, *T, out, h.n, mat, seed, pdf, w);
         float ct = dot3(out.d, h.n);
@@ -1108,4 +1131,94 @@ int ptref_pixel_samples(const float* ubo, const pt_push_constants* pc, const pt_
         out_xyz[3 * k] = s.x; out_xyz[3 * k + 1] = s.y; out_xyz[3 * k + 2] = s.z;
     }
     return PT_OK;
+}
+
+/* ---- probes: single functions of the restatement, for the closed-form and
+ * brute-force checks of tests/test_oracle_pins.py ------------------------------------ */
+
+/* One analytic intersector on one ray, with no earlier hit (t = 1e5).
+ * kind 0 sphere   a = pos[3], radius
+ *      1 plane    a = pos[3]
+ *      2 box      a = pos[3], rotation[3] (degrees), size[3]
+ *      3 lens     a = pos[3], rotation[3], radius, focal length, thickness, converging (0/1)
+ *      4 cyclide  a = pos[3], rotation[3], scale[3], A, B, C, D
+ *      5 bounding-sphere test, a = pos[3], squared radius
+ * out = t, normal[3], isOutside (lens). Returns 1 on a hit. */
+int ptref_probe_isect(int kind, const float* a, const float o[3], const float d[3], float out[5]) {
+    Ray r;
+    r.o = V3(o[0], o[1], o[2]);
+    r.d = V3(d[0], d[1], d[2]);
+    Hit h;
+    h.t = MAXDIST; h.n = V3(0.0f, 0.0f, 0.0f); h.mat = 0.0f; h.light = -1.0f;
+    int outside = 1, hit = 0;
+    v3 pos = V3(a[0], a[1], a[2]);
+    switch (kind) {
+        case 0: hit = isect_sphere(r, pos, a[3], 0.0f, -1.0f, &h); break;
+        case 1: hit = isect_plane(r, pos, 0.0f, -1.0f, &h); break;
+        case 2: hit = isect_box(r, pos, V3(a[3], a[4], a[5]), V3(a[6], a[7], a[8]), 0.0f, -1.0f, &h); break;
+        case 3:
+            hit = isect_lens(r, pos, V3(a[3], a[4], a[5]), a[6], a[7], a[8], a[9] != 0.0f, 0.0f, -1.0f, &h, &outside);
+            break;
+        case 4:
+            hit = isect_cyclide(r, pos, V3(a[3], a[4], a[5]), V3(a[6], a[7], a[8]), a[9], a[10], a[11], a[12], 0.0f,
+                                -1.0f, &h);
+            break;
+        case 5: hit = bounding_sphere(r, pos, a[3]); break;
+        default: return -1;
+    }
+    out[0] = h.t; out[1] = h.n.x; out[2] = h.n.y; out[3] = h.n.z; out[4] = (float)outside;
+    return hit;
+}
+
+/* The cyclide's object-space ray (o, d swizzled .xzy) and quartic
+ * coefficients a[0..4] (constant term first) for kind-4 parameters `a`. */
+void ptref_probe_cyclide(const float* a, const float o[3], const float d[3], float lo[3], float ld[3],
+                         float coef[5]) {
+    Ray r;
+    r.o = V3(o[0], o[1], o[2]);
+    r.d = V3(d[0], d[1], d[2]);
+    v3 ol, dl;
+    cyclide_local(r, V3(a[0], a[1], a[2]), V3(a[3], a[4], a[5]), V3(a[6], a[7], a[8]), &ol, &dl);
+    cyclide_coeffs(ol, dl, a[9], a[10], a[11], a[12], coef);
+    lo[0] = ol.x; lo[1] = ol.y; lo[2] = ol.z;
+    ld[0] = dl.x; ld[1] = dl.y; ld[2] = dl.z;
+}
+
+/* One draw of a direction sampler from the PCG chain state `seed`:
+ * kind 0 uniform on the unit sphere, 1 the diffuse bounce about n,
+ * 2 the light cone of cosine `cosMax` about axis n (sample_cone + to_world),
+ * 3 to_world of the vector `n2` in the basis of n. */
+void ptref_probe_sample(int kind, uint32_t seed, const float n[3], float cosMax, const float n2[3], float out[3]) {
+    v3 N = V3(n[0], n[1], n[2]), r = V3(0.0f, 0.0f, 0.0f);
+    uint32_t s = seed;
+    switch (kind) {
+        case 0: r = sample_unit_sphere(&s); break;
+        case 1: r = sample_bsdf_dir(N, &s); break;
+        case 2: r = to_world(sample_cone(&s, cosMax), N); break;
+        case 3: r = to_world(V3(n2[0], n2[1], n2[2]), N); break;
+        default: break;
+    }
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+float ptref_probe_light_pdf(float nl, float cosd, float cosMax) { return light_pdf(nl, cosd, cosMax); }
+float ptref_probe_mis(float p1, float p2) { return mis2(p1, p2); }
+
+/* Sphere tracing of one ray against a scene's SDF objects (SphereTracing):
+ * out = t of the hit, normal[3], material; returns 1 on a hit, and the number
+ * of distance evaluations in *steps. */
+int ptref_probe_march(const float* ubo, const pt_sdf_desc* sdfs, int nsdf, const float o[3], const float d[3],
+                      float out[5], uint64_t* steps) {
+    Ctx c;
+    memset(&c, 0, sizeof c);
+    c.u = ubo; c.sdf = sdfs; c.nsdf = nsdf;
+    Ray r;
+    r.o = V3(o[0], o[1], o[2]);
+    r.d = V3(d[0], d[1], d[2]);
+    Hit h;
+    h.t = MAXDIST; h.n = V3(0.0f, 0.0f, 0.0f); h.mat = 0.0f; h.light = -1.0f;
+    int hit = sphere_trace(&c, r, &h);
+    out[0] = h.t; out[1] = h.n.x; out[2] = h.n.y; out[3] = h.n.z; out[4] = h.mat;
+    if (steps) *steps = c.sdf_steps;
+    return hit;
 }

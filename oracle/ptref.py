@@ -52,7 +52,36 @@ def _open(name: str) -> C.CDLL:
     lib.ptref_sdf_eval.argtypes = [C.c_char_p, C.POINTER(C.c_float)]
     lib.ptref_sdfmat_eval.restype = C.c_float
     lib.ptref_sdfmat_eval.argtypes = [C.c_char_p, C.POINTER(C.c_float)]
+    _probe_types(lib)
     return lib
+
+
+def _probe_types(lib: C.CDLL) -> None:
+    F = C.POINTER(C.c_float)
+    lib.ptref_probe_isect.restype = C.c_int
+    lib.ptref_probe_isect.argtypes = [C.c_int, F, F, F, F]
+    lib.ptref_probe_cyclide.restype = None
+    lib.ptref_probe_cyclide.argtypes = [F, F, F, F, F, F]
+    lib.ptref_probe_sample.restype = None
+    lib.ptref_probe_sample.argtypes = [C.c_int, C.c_uint32, F, C.c_float, F, F]
+    lib.ptref_probe_light_pdf.restype = C.c_float
+    lib.ptref_probe_light_pdf.argtypes = [C.c_float, C.c_float, C.c_float]
+    lib.ptref_probe_mis.restype = C.c_float
+    lib.ptref_probe_mis.argtypes = [C.c_float, C.c_float]
+    lib.ptref_probe_march.restype = C.c_int
+    lib.ptref_probe_march.argtypes = [C.c_char_p, C.c_char_p, C.c_int, F, F, F, C.POINTER(C.c_uint64)]
+
+
+def load(path: str | os.PathLike) -> C.CDLL:
+    """Another build of ptref.c (a planted-mistake variant in a test), with the
+    same argument types."""
+    lib2 = C.CDLL(str(path))
+    lib2.ptref_dispatch.restype = C.c_int
+    lib2.ptref_dispatch.argtypes = lib.ptref_dispatch.argtypes
+    lib2.ptref_pixel_samples.restype = C.c_int
+    lib2.ptref_pixel_samples.argtypes = lib.ptref_pixel_samples.argtypes
+    _probe_types(lib2)
+    return lib2
 
 
 lib = _open("libptref.so")
@@ -127,3 +156,46 @@ def sdf_eval(desc: bytes, p) -> float:
 
 def sdfmat_eval(desc: bytes, p) -> float:
     return float(lib.ptref_sdfmat_eval(desc, (C.c_float * 3)(*p)))
+
+
+def _f(x, n=None) -> C.Array:
+    a = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    return (C.c_float * (n or len(a)))(*a)
+
+
+ISECT = {"sphere": 0, "plane": 1, "box": 2, "lens": 3, "cyclide": 4, "bound": 5}
+
+
+def probe_isect(kind: str, params, o, d, L: C.CDLL | None = None):
+    """One intersector on one ray: (hit, t, normal[3], isOutside)."""
+    out = (C.c_float * 5)()
+    hit = (L or lib).ptref_probe_isect(ISECT[kind], _f(params, 16), _f(o), _f(d), out)
+    return bool(hit), float(out[0]), np.array(out[1:4], np.float32), int(out[4])
+
+
+def probe_cyclide(params, o, d, L: C.CDLL | None = None):
+    """(object-space origin, direction, quartic coefficients constant-first)."""
+    lo, ld, cf = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 5)()
+    (L or lib).ptref_probe_cyclide(_f(params, 16), _f(o), _f(d), lo, ld, cf)
+    return np.array(lo[:], np.float32), np.array(ld[:], np.float32), np.array(cf[:], np.float32)
+
+
+SAMPLE = {"sphere": 0, "bsdf": 1, "cone": 2, "to_world": 3}
+
+
+def probe_samples(kind: str, seeds, n=(0.0, 0.0, 1.0), cos_max=0.0, v=(0.0, 0.0, 1.0), L: C.CDLL | None = None):
+    L = L or lib
+    out = np.zeros((len(seeds), 3), np.float32)
+    nn, vv, o = _f(n), _f(v), (C.c_float * 3)()
+    for i, s in enumerate(seeds):
+        L.ptref_probe_sample(SAMPLE[kind], int(s) & 0xFFFFFFFF, nn, cos_max, vv, o)
+        out[i] = o[:]
+    return out
+
+
+def probe_march(ubo: bytes, sdfs: bytes, nsdf: int, o, d, L: C.CDLL | None = None):
+    """(hit, t, normal[3], material, distance evaluations)."""
+    out = (C.c_float * 5)()
+    steps = C.c_uint64()
+    hit = (L or lib).ptref_probe_march(ubo, sdfs or None, nsdf, _f(o), _f(d), out, C.byref(steps))
+    return bool(hit), float(out[0]), np.array(out[1:4], np.float32), float(out[4]), int(steps.value)
