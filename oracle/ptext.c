@@ -24,7 +24,12 @@
  *  - next-event estimation at Lambert vertices (uniform light choice, area
  *    sampling of parallelograms, solid-angle cone sampling of spheres), power
  *    heuristic against cosine BSDF sampling; Russian roulette from depth
- *    rr_depth with q = min(max throughput, 0.95).
+ *    rr_depth with q = min(max throughput, 0.95). With nee = 0, BSDF sampling
+ *    alone (emitters hit with weight 1): an independent estimator of the same
+ *    image, whose agreement pins the light pdfs;
+ *  - camera: a pinhole, or the reference's camera: a point on the sensor, a
+ *    point on the aperture disk, then refraction at the two spherical faces of
+ *    its biconvex BK7 lens (src/shader.comp:1409-1465).
  * Parallel over rows with OpenMP.
  */
 #include <complex.h>
@@ -72,6 +77,12 @@ typedef struct {
     uint32_t seed;
     int W, H, max_depth, sdf_steps;
     double cam_pos[3], cam_angle[2], cam_size, ap_dist;
+    /* camera model (0: the reference's sensor + aperture + BK7 lens, 1: pinhole)
+     * and the lens parameters of the scene's camera */
+    int camera;
+    double ap_size, lens_dist, lens_radius, lens_focal, lens_thick;
+    int nee; /* 1: NEE with MIS at diffuse vertices; 0: BSDF sampling only */
+    uint64_t lens_missed; /* camera rays that missed a lens face (written) */
 } Scene;
 
 enum { T_SPHERE = 0, T_QUAD = 1, T_PLANE = 2, T_SDF = 3 };
@@ -218,7 +229,7 @@ static int slab(const box* b, d3 o, d3 inv, double tmax) {
     return t0 <= t1;
 }
 
-/* SDF distance estimators (the shapes of sdfs/*.glsl in binary64). */
+/* SDF distance estimators (the shapes of the reference's sdfs/ GLSL files, in binary64). */
 static double de_mandelbulb(d3 p, int iters, double bail) {
     d3 z = p;
     double r = sqrt(dot(z, z)), dr = 1.0;
@@ -490,7 +501,7 @@ static void trace_path(const Scene* s, const Accel* A, uint32_t key, d3 o, d3 d,
         surface(s, &h, p, &n, &mi, &li);
         if (li >= 0) {
             double w = 1.0;
-            if (!specular) {
+            if (!specular && s->nee) {
                 double pl = light_pdf(s, li, o, d, h.t);
                 w = pdf_prev * pdf_prev / (pdf_prev * pdf_prev + pl * pl);
             }
@@ -506,7 +517,7 @@ static void trace_path(const Scene* s, const Accel* A, uint32_t key, d3 o, d3 d,
             double rho[4];
             for (int j = 0; j < 4; ++j) rho[j] = reflect_spec(m, lam[j]);
             d3 P = add(p, mul(N, scale));
-            if (s->nlight > 0) {
+            if (s->nlight > 0 && s->nee) {
                 int k = (int)(rnd(key, b, 3) * s->nlight);
                 if (k >= s->nlight) k = s->nlight - 1;
                 const double* L = s->light + 4 * k;
@@ -605,6 +616,94 @@ static void trace_path(const Scene* s, const Accel* A, uint32_t key, d3 o, d3 d,
     }
 }
 
+/* v * RotationMatrix(a) for GLSL's RotationMatrix (src/shader.comp:142-152),
+ * M = mX mY mZ: rotate about x, then y, then z (angles in degrees). */
+static d3 rot_row(d3 a, d3 v) {
+    double x = a.x * M_PI / 180.0, y = a.y * M_PI / 180.0, z = a.z * M_PI / 180.0;
+    d3 r1 = D3(v.x, v.y * cos(x) - v.z * sin(x), v.y * sin(x) + v.z * cos(x));
+    d3 r2 = D3(r1.x * cos(y) + r1.z * sin(y), r1.y, -r1.x * sin(y) + r1.z * cos(y));
+    return D3(r2.x * cos(z) - r2.y * sin(z), r2.x * sin(z) + r2.y * cos(z), r2.z);
+}
+/* Inverse of rot_row (the matrix is orthogonal): M v. */
+static d3 rot_col(d3 a, d3 v) {
+    double x = a.x * M_PI / 180.0, y = a.y * M_PI / 180.0, z = a.z * M_PI / 180.0;
+    d3 r2 = D3(v.x * cos(z) + v.y * sin(z), -v.x * sin(z) + v.y * cos(z), v.z);
+    d3 r1 = D3(r2.x * cos(y) - r2.z * sin(y), r2.y, r2.x * sin(y) + r2.z * cos(y));
+    return D3(r1.x, r1.y * cos(x) + r1.z * sin(x), -r1.y * sin(x) + r1.z * cos(x));
+}
+/* Schott N-BK7, Sellmeier (wavelength in nm). */
+static double bk7(double lnm) {
+    double l2 = (lnm * 1e-3) * (lnm * 1e-3);
+    return sqrt(1.0 + 1.03961212 * l2 / (l2 - 6.00069867e-3) + 0.231792344 * l2 / (l2 - 2.00179144e-2) +
+                1.01046945 * l2 / (l2 - 103.560653));
+}
+/* Snell's law for unit d and a unit normal n facing against d, eta = n1 / n2;
+ * the zero vector on total internal reflection (GLSL refract). */
+static d3 refract_d(d3 d, d3 n, double eta) {
+    double c = dot(n, d);
+    double k = 1.0 - eta * eta * (1.0 - c * c);
+    if (k < 0.0) return D3(0, 0, 0);
+    return sub(mul(d, eta), mul(n, eta * c + sqrt(k)));
+}
+/* The reference's camera ray at hero wavelength l0: sensor point, aperture
+ * point, then two refractions at the faces of the biconvex lens (radius of
+ * curvature 2f, sagitta h = 2f - sqrt(4f^2 - r^2), faces at object x <= -t/2
+ * and x >= t/2 with their spheres centred at x = +-(2f - t/2 - h); object
+ * axes RotationMatrix(0, 90 - yaw, pitch) about a point lensDistance ahead).
+ * Entering the glass the index ratio is 1 / n(l) and the wavelength is
+ * rescaled by it before the second face, as the reference does
+ * (src/shader.comp:1409-1444). Returns 0 when a face is missed. */
+static int lens_ray(const Scene* s, uint32_t key, double uvx, double uvy, double l0, d3* po, d3* pd) {
+    d3 ca = D3(s->cam_angle[0], s->cam_angle[1], 0.0);
+    d3 cam = D3(s->cam_pos[0], s->cam_pos[1], s->cam_pos[2]);
+    double sc = -0.5 * s->cam_size;
+    d3 o = add(cam, rot_row(ca, D3(uvx * sc, uvy * sc, 0.0)));
+    double ux = rnd(key, CAM, 2), uy = rnd(key, CAM, 3);
+    double r = 0.5 * s->ap_size * sqrt(ux), ph = 2.0 * M_PI * uy;
+    d3 ap = add(cam, rot_row(ca, D3(r * cos(ph), r * sin(ph), s->ap_dist)));
+    d3 d = unit(sub(ap, o));
+    d3 lp = add(cam, mul(rot_row(ca, D3(0, 0, 1)), s->lens_dist));
+    d3 la = D3(0.0, 90.0 - s->cam_angle[1], s->cam_angle[0]);
+    double R = 2.0 * s->lens_focal;
+    double h = R - sqrt(R * R - s->lens_radius * s->lens_radius);
+    double half_t = 0.5 * s->lens_thick;
+    double cx[2] = {R - half_t - h, -(R - half_t - h)};
+    double l = l0;
+    for (int pass = 0; pass < 2; ++pass) {
+        d3 lo = rot_row(la, sub(o, lp)), ld = rot_row(la, d);
+        double best = INFINITY;
+        d3 bn = D3(0, 0, 0);
+        int entering = 0;
+        for (int k = 0; k < 2; ++k) {
+            d3 oc = sub(lo, D3(cx[k], 0, 0));
+            double b = dot(oc, ld), c = dot(oc, oc) - R * R, disc = b * b - c;
+            if (disc < 0) continue;
+            double sq = sqrt(disc);
+            double ts[2] = {-b - sq, -b + sq};
+            for (int j = 0; j < 2; ++j) {
+                double t = ts[j];
+                if (!(t > 1e-4) || t >= best) continue;
+                double x = lo.x + t * ld.x;
+                if (k == 0 ? x > -half_t : x < half_t) continue;
+                best = t;
+                d3 rad = mul(sub(add(lo, mul(ld, t)), D3(cx[k], 0, 0)), 1.0 / R);
+                entering = dot(ld, rad) < 0;
+                bn = entering ? rad : mul(rad, -1.0);
+            }
+        }
+        if (!(best < INFINITY)) return 0;
+        double eta;
+        if (entering) eta = 1.0 / bk7(l);
+        else eta = bk7(l);
+        l *= eta;
+        o = add(o, mul(d, best));
+        d = refract_d(d, rot_col(la, bn), eta);
+    }
+    *po = o;
+    *pd = d;
+    return 1;
+}
+
 /* GLSL RotationMatrix(angle) (src/shader.comp:142-152) applied as v * M. */
 static d3 cam_dir(const Scene* s, d3 v) {
     double ax = s->cam_angle[0] * M_PI / 180.0, ay = s->cam_angle[1] * M_PI / 180.0;
@@ -620,18 +719,18 @@ static d3 cam_dir(const Scene* s, d3 v) {
  * sample indices first_sample .. first_sample + spp - 1. sum[3 * (W * (y - y0) + x)]
  * receives the XYZ sum over samples, sq the sum of squares; cnt[0..2] add
  * segments, shadow rays and SDF steps. */
-int ptext_render(const Scene* s, int first_sample, int spp, int y0, int y1, double* sum, double* sq, uint64_t* cnt,
+int ptext_render(Scene* s, int first_sample, int spp, int y0, int y1, double* sum, double* sq, uint64_t* cnt,
                  int threads) {
     if (!s || !sum || y0 < 0 || y1 > s->H || y0 >= y1 || spp <= 0) return -1;
     Accel A;
     accel_build(&A, s);
-    uint64_t c0 = 0, c1 = 0, c2 = 0;
+    uint64_t c0 = 0, c1 = 0, c2 = 0, missed = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : c0, c1, c2)
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : c0, c1, c2, missed)
     for (int gy = y0; gy < y1; ++gy) {
         uint64_t lc[3] = {0, 0, 0};
         for (int gx = 0; gx < s->W; ++gx) {
@@ -643,11 +742,18 @@ int ptext_render(const Scene* s, int first_sample, int spp, int y0, int y1, doub
                 double uvx = (2.0 * (gx + ju) - s->W) / s->H;
                 double uvy = (2.0 * ((s->H - gy) - jv) - s->H) / s->H;
                 double l0 = s->lmin + s->lrange * rnd(key, CAM, 4);
-                double h = 0.5 * s->cam_size;
-                d3 d = unit(cam_dir(s, D3(uvx * h, uvy * h, s->ap_dist)));
-                d3 o = D3(s->cam_pos[0], s->cam_pos[1], s->cam_pos[2]);
-                double x[3];
-                trace_path(s, &A, key, o, d, l0, x, lc);
+                d3 o, d;
+                double x[3] = {0, 0, 0};
+                if (s->camera == 1) {
+                    double h = 0.5 * s->cam_size;
+                    d = unit(cam_dir(s, D3(uvx * h, uvy * h, s->ap_dist)));
+                    o = D3(s->cam_pos[0], s->cam_pos[1], s->cam_pos[2]);
+                    trace_path(s, &A, key, o, d, l0, x, lc);
+                } else if (lens_ray(s, key, uvx, uvy, l0, &o, &d)) {
+                    trace_path(s, &A, key, o, d, l0, x, lc);
+                } else {
+                    ++missed;
+                }
                 for (int c = 0; c < 3; ++c) {
                     acc[c] += x[c];
                     acc2[c] += x[c] * x[c];
@@ -662,6 +768,7 @@ int ptext_render(const Scene* s, int first_sample, int spp, int y0, int y1, doub
         c0 += lc[0]; c1 += lc[1]; c2 += lc[2];
     }
     if (cnt) { cnt[0] += c0; cnt[1] += c1; cnt[2] += c2; }
+    s->lens_missed = missed;
     accel_free(&A);
     return 0;
 }
@@ -670,3 +777,31 @@ int ptext_render(const Scene* s, int first_sample, int spp, int y0, int y1, doub
 double ptext_fresnel_dielectric(double ci, double eta_i, double eta_t) { return fresnel_diel(ci, eta_i, eta_t); }
 double ptext_fresnel_conductor(double ci, double n, double k) { return fresnel_cond(ci, n, k); }
 double ptext_planck(double lnm, double T, double lum) { return planck_norm(lnm, T, lum); }
+
+/* Probes of single functions, for tests/test_oracle_ext_pins.py. */
+int ptext_probe_quad(const double q[9], const double o[3], const double d[3], double* t) {
+    return hit_quad(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, t);
+}
+int ptext_probe_sphere(const double q[4], const double o[3], const double d[3], double* t) {
+    return hit_sphere(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, t);
+}
+/* m = one material row (type n dispersive p[9]) */
+double ptext_probe_ior(const double m[12], double lnm) { return ior(m, lnm); }
+double ptext_probe_reflectance(const double m[12], double lnm) { return reflect_spec(m, lnm); }
+/* Sphere tracing of one SDF record q (kind iters bailout mat cx cy cz sx sy sz)
+ * along o + t d; returns t (-1 for no hit) and the estimator evaluations. */
+double ptext_probe_march(const double q[10], const double o[3], const double d[3], int max_steps, uint64_t* steps) {
+    uint64_t n = 0;
+    double t = march(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, max_steps, &n);
+    if (steps) *steps = n;
+    return t;
+}
+double ptext_probe_de(const double q[10], const double p[3]) { return de(q, D3(p[0], p[1], p[2])); }
+/* The camera ray of (key, uv, hero wavelength); returns 0 if the lens is missed. */
+int ptext_probe_camera(const Scene* s, uint32_t key, double uvx, double uvy, double l0, double o[3], double d[3]) {
+    d3 po, pd;
+    int ok = lens_ray(s, key, uvx, uvy, l0, &po, &pd);
+    o[0] = po.x; o[1] = po.y; o[2] = po.z;
+    d[0] = pd.x; d[1] = pd.y; d[2] = pd.z;
+    return ok;
+}

@@ -32,15 +32,40 @@ class Scene(C.Structure):
         ("lrange", C.c_double), ("rr_depth", C.c_int), ("seed", C.c_uint32),
         ("W", C.c_int), ("H", C.c_int), ("max_depth", C.c_int), ("sdf_steps", C.c_int),
         ("cam_pos", C.c_double * 3), ("cam_angle", C.c_double * 2), ("cam_size", C.c_double), ("ap_dist", C.c_double),
+        ("camera", C.c_int), ("ap_size", C.c_double), ("lens_dist", C.c_double), ("lens_radius", C.c_double),
+        ("lens_focal", C.c_double), ("lens_thick", C.c_double), ("nee", C.c_int), ("lens_missed", C.c_uint64),
     ]
 
 
-lib.ptext_render.restype = C.c_int
-lib.ptext_render.argtypes = [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, _D, _D, C.POINTER(C.c_uint64),
-                             C.c_int]
-for fn in ("ptext_fresnel_dielectric", "ptext_fresnel_conductor", "ptext_planck"):
-    getattr(lib, fn).restype = C.c_double
-    getattr(lib, fn).argtypes = [C.c_double] * 3
+def _setup(L: C.CDLL) -> C.CDLL:
+    L.ptext_render.restype = C.c_int
+    L.ptext_render.argtypes = [C.POINTER(Scene), C.c_int, C.c_int, C.c_int, C.c_int, _D, _D,
+                               C.POINTER(C.c_uint64), C.c_int]
+    for fn in ("ptext_fresnel_dielectric", "ptext_fresnel_conductor", "ptext_planck"):
+        getattr(L, fn).restype = C.c_double
+        getattr(L, fn).argtypes = [C.c_double] * 3
+    for fn in ("ptext_probe_quad", "ptext_probe_sphere"):
+        getattr(L, fn).restype = C.c_int
+        getattr(L, fn).argtypes = [_D, _D, _D, _D]
+    for fn in ("ptext_probe_ior", "ptext_probe_reflectance"):
+        getattr(L, fn).restype = C.c_double
+        getattr(L, fn).argtypes = [_D, C.c_double]
+    L.ptext_probe_march.restype = C.c_double
+    L.ptext_probe_march.argtypes = [_D, _D, _D, C.c_int, C.POINTER(C.c_uint64)]
+    L.ptext_probe_de.restype = C.c_double
+    L.ptext_probe_de.argtypes = [_D, _D]
+    L.ptext_probe_camera.restype = C.c_int
+    L.ptext_probe_camera.argtypes = [C.POINTER(Scene), C.c_uint32, C.c_double, C.c_double, C.c_double, _D, _D]
+    return L
+
+
+_setup(lib)
+
+
+def load(path: str | os.PathLike) -> C.CDLL:
+    """Another build of ptext.c (a planted-mistake variant in a test)."""
+    return _setup(C.CDLL(str(path)))
+
 
 SDF_KINDS = {"mandelbulb": 1, "menger": 2, "blob": 3, "terrain": 4}
 
@@ -158,6 +183,11 @@ class Flat:
         s.cam_pos[:] = pos
         s.cam_angle[:] = [-ang[1], ang[0]]  # push-constant swizzle (src/pathtracer.cpp:3821)
         s.cam_size, s.ap_dist = float(cam["size"]), float(cam["apertureDistance"])
+        s.camera = 1 if cam.get("model", "lens") == "pinhole" else 0
+        s.ap_size, s.lens_dist = float(cam["apertureSize"]), float(cam["lensDistance"])
+        s.lens_radius, s.lens_focal = float(cam["lensRadius"]), float(cam["lensFocalLength"])
+        s.lens_thick = float(cam["lensThickness"])
+        s.nee = 1
         self.exposure = float(cam["apertureSize"]) ** 2 * float(cam["ISO"])
         self.scene = s
 
@@ -171,26 +201,48 @@ def cie_table() -> np.ndarray:
 
 
 def render(text: str, W: int, H: int, spp: int, depth: int, first_sample: int = 0, threads: int = 0,
-           rows: tuple[int, int] | None = None, seed: int | None = None) -> dict:
+           rows: tuple[int, int] | None = None, seed: int | None = None, nee: bool = True,
+           L: C.CDLL | None = None) -> dict:
     """Renders rows [y0, y1) (all by default). Returns the per-pixel mean XYZ
-    (exposure applied), the per-pixel sample variance of the mean, counters and seconds."""
+    (exposure applied), the per-pixel sample variance of the mean, counters and
+    seconds. nee=False renders with BSDF sampling alone."""
     if seed is not None:
         sc = json.loads(text)
         sc.setdefault("render", {})["seed"] = seed
         text = json.dumps(sc)
     f = Flat(text, W, H, depth, cie_table())
+    f.scene.nee = 1 if nee else 0
     y0, y1 = rows if rows else (0, H)
     n = W * (y1 - y0) * 3
     sums = np.zeros(n, np.float64)
     sq = np.zeros(n, np.float64)
     cnt = (C.c_uint64 * 3)()
     t = time.perf_counter()
-    st = lib.ptext_render(C.byref(f.scene), first_sample, spp, y0, y1, sums.ctypes.data_as(_D), sq.ctypes.data_as(_D),
-                          cnt, threads or (os.cpu_count() or 1))
+    st = (L or lib).ptext_render(C.byref(f.scene), first_sample, spp, y0, y1, sums.ctypes.data_as(_D),
+                                 sq.ctypes.data_as(_D), cnt, threads or (os.cpu_count() or 1))
     dt = time.perf_counter() - t
     if st != 0:
         raise RuntimeError("ptext_render failed")
     mean = (sums / spp).reshape(y1 - y0, W, 3)
     var = np.maximum(sq / spp - (sums / spp) ** 2, 0.0).reshape(y1 - y0, W, 3) / spp
     return {"mean": mean * f.exposure, "var_of_mean": var * f.exposure ** 2, "segments": int(cnt[0]),
-            "shadow_segments": int(cnt[1]), "sdf_steps": int(cnt[2]), "seconds": dt}
+            "shadow_segments": int(cnt[1]), "sdf_steps": int(cnt[2]), "seconds": dt,
+            "lens_missed": int(f.scene.lens_missed)}
+
+
+def camera_ray(text: str, W: int, H: int, key: int, uvx: float, uvy: float, l0: float, L: C.CDLL | None = None):
+    """The lens camera's ray for one (key, uv, hero wavelength): (ok, origin, direction)."""
+    f = Flat(text, W, H, 1, cie_table())
+    o, d = (C.c_double * 3)(), (C.c_double * 3)()
+    ok = (L or lib).ptext_probe_camera(C.byref(f.scene), key & 0xFFFFFFFF, uvx, uvy, l0, o, d)
+    return bool(ok), np.array(o[:]), np.array(d[:])
+
+
+def material_row(m: dict) -> np.ndarray:
+    """One material as the oracle's 12-double row (the Flat layout)."""
+    f = Flat(json.dumps({"camera": {"position": [[0, 0, 0]], "angle": [[0, 0]], "size": 0.057,
+                                    "apertureDistance": 0.049, "apertureSize": 0.0025, "ISO": 1,
+                                    "lensDistance": 0.05, "lensRadius": 0.01, "lensFocalLength": 0.03,
+                                    "lensThickness": 0.0},
+                         "material": [m]}), 1, 1, 1, cie_table())
+    return np.ctypeslib.as_array(f.scene.mat, shape=(12,)).copy()
