@@ -165,6 +165,11 @@ def parse_args():
                     help="GPU time by kernel family in both systems (HIP events per launch, graphs off)")
     ap.add_argument("--serve-denoise", action="store_true",
                     help="the SRPT server may finish requests with the shipped denoiser's image")
+    ap.add_argument("--serve-no-observe", action="store_true",
+                    help="one-at-a-time p99 from the service times only (no real-time one-at-a-time run)")
+    ap.add_argument("--serve-slice", type=int, default=24, metavar="N",
+                    help="the default (spheres) bench line also carries p50/p99 time-to-RMSE of requests 0..N-1 "
+                         "of the serving stream, one at a time and SRPT+denoiser at 1x and 3x arrivals (0 = off)")
     ap.add_argument("--serve-speedup", type=str, default="1",
                     help="comma-separated factors: the SRPT server's arrivals come this many times faster than the "
                          "one-at-a-time ones (one SRPT run per factor)")
@@ -319,6 +324,9 @@ def cpu_baseline(args, scene_text: str, W: int, H: int, depth: int, full_spp: in
         "value": run["segments"] / run["seconds"],
         "unit": "segments/s",
         "cores": threads,
+        # host CPUs visible to this process; a GPU box's share for one GPU is 16
+        # threads, so the run uses at most 16 of them
+        "nproc": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": f"FP64 CPU oracle with its own BVH, the full {W}x{H} frame x {spp} spp, "
                   f"depth {depth}: {run['segments']} segments in {run['seconds']:.2f} s",
@@ -384,31 +392,28 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def serve_bench(args):
-    """Serving workload: requests 0..N-1 of scenes.serving_request.
+def serving_measure(args, n_requests: int, speedups, denoise: bool, observe_fifo: bool = True, log=None):
+    """Requests 0..n-1 of scenes.serving_request, served two ways.
 
     1. Each request's service time alone: all requests traced one at a time to
-       completion (FIFO server, all queued at once).
+       completion (FIFO server, one active request, all queued at once).
     2. Poisson arrivals (seeded) at the rate that loads that one-at-a-time
        tracer to --serve-load.
-    3. The same arrival sequence, compressed --serve-speedup times, served in
-       real time by the SRPT server (continuous batching; with --serve-denoise
-       requests may finish denoised); per request the time from arrival to its
-       relative-RMSE target (or to its sample budget).
-    4. The one-at-a-time (FIFO, plain) response times for the uncompressed
-       arrivals, from the service times of step 1 (a FIFO queue is exact given
-       service times).
-    Prints one JSON line."""
+    3. With observe_fifo, the one-at-a-time server run in real time on those
+       arrivals: its observed response times (time from arrival to the
+       relative-RMSE target or to the sample budget).
+    4. The same arrival sequence, compressed `speedup` times, served in real
+       time by the SRPT server (continuous batching; with `denoise` requests
+       may finish with the shipped denoiser's image).
+    Returns the measurement as a dict."""
     import numpy as np
     import pathtracer_amd as pt
     from pathtracer_amd import scenes
 
-    reqs = [scenes.serving_request(i) for i in range(args.serve)]
+    log = log or (lambda msg: print(msg, file=sys.stderr, flush=True))
+    reqs = [scenes.serving_request(i) for i in range(n_requests)]
     ctx = pt.Context(0, graph_mode=0 if args.serve_graphs else 1, wave_paths=args.serve_wave)
     t_start = time.perf_counter()
-
-    def submit(srv, r, t):
-        return srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target, t_arrive=t)
 
     def kernel_split(prof):
         """GPU ms by kernel family from pt_profile_read totals (--serve-profile)."""
@@ -431,7 +436,8 @@ def serve_bench(args):
         ctx.profile(True)
         ctx.profile_read(reset=True)
     with pt.Server(ctx, policy=pt.SERVE_FIFO, max_active=1) as srv:
-        ids = {submit(srv, r, 0.0): r["id"] for r in reqs}
+        ids = {srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target,
+                          t_arrive=0.0): r["id"] for r in reqs}
         while len(service) < len(reqs):
             sts = srv.step()
             if sts is None:
@@ -441,8 +447,7 @@ def serve_bench(args):
                     service[ids[st.id]] = st.as_dict()
                     srv.release(st.id)
                     if len(service) % 32 == 0:
-                        print(f"one at a time: {len(service)}/{len(reqs)} done "
-                              f"({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
+                        log(f"one at a time: {len(service)}/{len(reqs)} done ({time.perf_counter() - t_start:.0f} s)")
     s_alone = np.array([service[r["id"]]["gpu_seconds"] for r in reqs])
     alone_kernels = kernel_split(ctx.profile_read(reset=True)) if args.serve_profile else None
     alone_converged = sum(service[r["id"]]["converged"] for r in reqs)
@@ -452,9 +457,8 @@ def serve_bench(args):
     gaps = np.random.default_rng(20261202).exponential(1.0 / rate, len(reqs))
     offsets = np.cumsum(gaps) - gaps[0]
 
-    # 3. SRPT in real time, once per arrival speedup
     dn = None
-    if args.serve_denoise:
+    if denoise:
         from pathtracer_amd import denoise_train
         hmax = max(r["height"] for r in reqs)
         wmax = max(r["width"] for r in reqs)
@@ -462,12 +466,13 @@ def serve_bench(args):
         dn = pt.Denoiser(ctx, 2, (hmax + 3) // 4 * 4, (wmax + 3) // 4 * 4)
         denoise_train.load_weights(dn)
 
-    def srpt_run(speedup):
+    def realtime(policy, speedup, max_active, use_dn):
+        """One server run in real time on the arrivals compressed `speedup` times."""
         done, segments = {}, 0
         if args.serve_profile:
             ctx.profile_read(reset=True)
-        with pt.Server(ctx, policy=pt.SERVE_SRPT, max_active=args.serve_max_active) as srv:
-            if dn is not None:
+        with pt.Server(ctx, policy=policy, max_active=max_active) as srv:
+            if use_dn:
                 srv.set_denoiser(dn)
             base = srv.now()
             arrive = base + offsets / speedup
@@ -477,7 +482,7 @@ def serve_bench(args):
                 while nxt < len(reqs) and arrive[nxt] <= now:
                     r = reqs[nxt]
                     ids[srv.submit(r["text"], r["width"], r["height"], r["spp"], r["depth"], args.serve_target,
-                                   t_arrive=float(arrive[nxt]), denoise=dn is not None)] = r["id"]
+                                   t_arrive=float(arrive[nxt]), denoise=use_dn)] = r["id"]
                     nxt += 1
                 sts = srv.step()
                 if sts is None:
@@ -505,25 +510,60 @@ def serve_bench(args):
                 "gpu_seconds": float(sum(done[r["id"]]["gpu_seconds"] for r in reqs)),
                 "kernel_ms": kernel_split(ctx.profile_read(reset=True)) if args.serve_profile else None}
 
-    runs = []
-    for k in args.serve_speedup.split(","):
-        runs.append(srpt_run(float(k)))
-        print(f"SRPT at {k}x arrivals: p99 response {runs[-1]['p99_response_s']:.2f} s, converged "
-              f"{runs[-1]['converged']} ({time.perf_counter() - t_start:.0f} s)", file=sys.stderr, flush=True)
-    if dn is not None:
-        dn.close()
-
-    # 4. one request at a time, same arrivals
+    # 3. one request at a time in real time, uncompressed arrivals
+    fifo_obs = None
+    if observe_fifo:
+        fifo_obs = realtime(pt.SERVE_FIFO, 1.0, 1, False)
+        log(f"one at a time, observed: p99 response {fifo_obs['p99_response_s']:.2f} s "
+            f"({time.perf_counter() - t_start:.0f} s)")
+    # the same from the service times (a FIFO queue is exact given service times)
     fifo_resp, c = [], 0.0
     for k in range(len(reqs)):
         c = max(offsets[k], c) + s_alone[k]
         fifo_resp.append(c - offsets[k])
     fifo_resp = np.array(fifo_resp)
     fifo_conv = np.array([fifo_resp[i] for i in range(len(reqs)) if service[reqs[i]["id"]]["converged"]])
-    p99_fifo = percentile(fifo_resp, 99)
-    # the largest arrival speedup at which the SRPT server's p99 response is no worse
+    p99_fifo = fifo_obs["p99_response_s"] if fifo_obs else percentile(fifo_resp, 99)
+
+    # 4. SRPT in real time, once per arrival speedup
+    runs = []
+    for k in speedups:
+        runs.append(realtime(pt.SERVE_SRPT, float(k), args.serve_max_active, dn is not None))
+        log(f"SRPT at {k}x arrivals: p99 response {runs[-1]['p99_response_s']:.2f} s, converged "
+            f"{runs[-1]['converged']} ({time.perf_counter() - t_start:.0f} s)")
+    if dn is not None:
+        dn.close()
+    ctx.close()
+    # the largest arrival speedup at which the SRPT server's p99 response is no
+    # worse than the one-at-a-time server's at 1x
     equal_p99 = max([r["arrival_speedup"] for r in runs if r["p99_response_s"] <= p99_fifo], default=None)
-    srpt = runs[-1]
+    one = {"p50_time_to_rmse_s": percentile(fifo_conv, 50), "p99_time_to_rmse_s": percentile(fifo_conv, 99),
+           "converged": int(alone_converged),
+           "p50_response_s": percentile(fifo_resp, 50), "p99_response_s": percentile(fifo_resp, 99),
+           "mean_service_s": float(s_alone.mean()),
+           "mean_spp": float(np.mean([service[r["id"]]["spp"] for r in reqs])),
+           "gpu_seconds": float(s_alone.sum()), "kernel_ms": alone_kernels}
+    if fifo_obs:
+        one["observed"] = fifo_obs
+    return {"requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
+            "arrival_rate_per_s": rate, "srpt_denoise": dn is not None,
+            "wave_paths": args.serve_wave or (1 << 22), "srpt_max_active": args.serve_max_active or 16,
+            "p99_one_at_a_time_s": p99_fifo, "p99_basis": "observed" if fifo_obs else "from service times",
+            "srpt": runs[-1], "srpt_runs": runs, "throughput_ratio_at_equal_p99": equal_p99,
+            "one_at_a_time": one, "bench_seconds": time.perf_counter() - t_start}
+
+
+SERVE_WORKLOAD = ("serving stream requests 0..{last}: 50% Cornell+N random spheres (N log-uniform 1..4096, "
+                  "depth 12) / 50% mandelbulb+menger (parameters x U[0.9,1.1], depth 8); resolution "
+                  "U{{256^2, 512^2, 1024x768, 1920x1080}}; spp budget log-uniform 16..1024; Poisson arrivals")
+
+
+def serve_bench(args):
+    """Serving workload (requests 0..--serve - 1): prints one JSON line whose
+    value is the SRPT server's p50 time to the relative-RMSE target."""
+    m = serving_measure(args, args.serve, args.serve_speedup.split(","), args.serve_denoise,
+                        observe_fifo=not args.serve_no_observe)
+    srpt = m["srpt"]
     line = {
         "metric": METRIC,
         "value": srpt["p50_time_to_rmse_s"],
@@ -532,26 +572,13 @@ def serve_bench(args):
         "higher_is_better": False,
         "data": "synthetic",
         "dtype": "fp32",
-        "config": {"workload": f"serving stream requests 0..{len(reqs) - 1}: 50% Cornell+N random spheres "
-                               "(N log-uniform 1..4096, depth 12) / 50% mandelbulb+menger (parameters x U[0.9,1.1], "
-                               "depth 8); resolution U{256^2, 512^2, 1024x768, 1920x1080}; "
-                               "spp budget log-uniform 16..1024; Poisson arrivals",
-                   "requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
-                   "arrival_rate_per_s": rate, "srpt_denoise": bool(args.serve_denoise),
-                   "wave_paths": args.serve_wave or (1 << 22), "srpt_max_active": args.serve_max_active or 16},
-        "srpt": srpt,
-        "srpt_runs": runs,
-        "throughput_ratio_at_equal_p99": equal_p99,
-        "one_at_a_time": {"p50_time_to_rmse_s": percentile(fifo_conv, 50),
-                          "p99_time_to_rmse_s": percentile(fifo_conv, 99), "converged": int(alone_converged),
-                          "p50_response_s": percentile(fifo_resp, 50), "p99_response_s": percentile(fifo_resp, 99),
-                          "mean_service_s": float(s_alone.mean()),
-                          "mean_spp": float(np.mean([service[r["id"]]["spp"] for r in reqs])),
-                          "gpu_seconds": float(s_alone.sum()), "kernel_ms": alone_kernels},
-        "bench_seconds": time.perf_counter() - t_start,
+        "config": {"workload": SERVE_WORKLOAD.format(last=m["requests"] - 1),
+                   **{k: m[k] for k in ("requests", "target_rel", "load", "arrival_rate_per_s", "srpt_denoise",
+                                        "wave_paths", "srpt_max_active")}},
+        **{k: m[k] for k in ("srpt", "srpt_runs", "throughput_ratio_at_equal_p99", "p99_one_at_a_time_s",
+                             "p99_basis", "one_at_a_time", "bench_seconds")},
     }
     print(json.dumps(line))
-    ctx.close()
 
 
 # The U-Net's layers as built (input channels padded to 16, output to 16):
@@ -887,8 +914,32 @@ def trace_bench(args):
     if prof:
         line["profiled_ms_per_step"] = dt_prof * 1e3 / args.steps
         line["kernels_ms"] = {k: round(v[1], 3) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-    print(json.dumps(line))
     ctx.close()
+    if args.workload == "spheres" and args.serve_slice > 0:
+        line["serving"] = serving_slice(args)
+        config["serving_slice"] = SERVE_WORKLOAD.format(last=args.serve_slice - 1) + \
+            " (the first requests of the 256-request stream)"
+    print(json.dumps(line))
+
+
+def serving_slice(args):
+    """The metric's second half on a bounded slice of the serving stream: p50 /
+    p99 time to relative RMSE <= 1 % (over the requests that reach it) and
+    response times, one request at a time (observed in real time) and with the
+    SRPT server plus denoiser at 1x and 3x the arrival rate."""
+    m = serving_measure(args, args.serve_slice, ["1", "3"], denoise=True, observe_fifo=True)
+    one = m["one_at_a_time"]["observed"]
+    pick = ("p50_time_to_rmse_s", "p99_time_to_rmse_s", "converged", "p50_response_s", "p99_response_s",
+            "denoised", "mean_spp", "gpu_seconds")
+    return {"requests": m["requests"], "target_rel": m["target_rel"], "load": m["load"],
+            "arrival_rate_per_s": round(m["arrival_rate_per_s"], 4),
+            "p50_time_to_rmse_s": m["srpt_runs"][0]["p50_time_to_rmse_s"],
+            "p99_time_to_rmse_s": m["srpt_runs"][0]["p99_time_to_rmse_s"],
+            "one_at_a_time": {k: one[k] for k in pick if k in one},
+            "srpt_denoise_1x": {k: m["srpt_runs"][0][k] for k in pick},
+            "srpt_denoise_3x": {k: m["srpt_runs"][1][k] for k in pick},
+            "throughput_ratio_at_equal_p99": m["throughput_ratio_at_equal_p99"],
+            "seconds": round(m["bench_seconds"], 1)}
 
 
 if __name__ == "__main__":

@@ -1,0 +1,188 @@
+"""Extended-mode paths against the FP64 oracle (oracle/ptext.c): Sellmeier
+glass, boxes (traced as six parallelograms), more than 16 parallelograms (the
+parallelogram BVH), the reference's lens camera, and block-wise convergence on
+caustic-free versions of the 4096-sphere and SDF scenes.
+
+Bars: same-seed per-pixel agreement (both draw the same random numbers, so
+they trace the same paths until FP32 and FP64 rounding part them at a grazing
+hit or a roulette threshold; those pixels are counted), and with independent
+seeds, 4x4 blocks within 4 sigma for >= 99 % of blocks and image means within
+3 sigma."""
+import json
+
+import numpy as np
+import pytest
+
+import pathtracer_amd as pt
+from pathtracer_amd import scenes
+from oracle import ptext
+
+pytestmark = pytest.mark.gpu
+
+BK7 = [1.03961212, 0.231792344, 1.01046945, 6.00069867e-3, 2.00179144e-2, 103.560653]
+
+
+def gpu_render(ctx, text, W, H, spp, depth, spf=None):
+    scene = ctx.scene_from_json(text)
+    spf = spf or spp
+    r = pt.Renderer(ctx, scene, W, H, spf=spf, path_length=depth, mode=pt.MODE_EXT)
+    for _ in range(spp // spf):
+        r.render_frame()
+    return r.acc.read()[..., :3].astype(np.float64)
+
+
+def same_seed(ctx, text, W, H, spp, depth):
+    g = gpu_render(ctx, text, W, H, spp, depth)
+    o = ptext.render(text, W, H, spp, depth)
+    den = np.abs(o["mean"]).sum(-1)
+    rel = np.where(den > 0, np.abs(g - o["mean"]).sum(-1) / np.maximum(den, 1e-30), np.abs(g).sum(-1))
+    return float((rel <= 1e-3).mean()), float((rel <= 1e-2).mean()), o
+
+
+def block(a, b):
+    h, w = a.shape[0] // b * b, a.shape[1] // b * b
+    return a[:h, :w].reshape(h // b, b, w // b, b, -1).mean(axis=(1, 3))
+
+
+def converges(ctx, text, W, H, gspp, ospp, depth, spf=512):
+    gpu = gpu_render(ctx, text, W, H, gspp, depth, spf=min(spf, gspp))
+    ora = ptext.render(text, W, H, ospp, depth, seed=9090)
+    var_o = ora["var_of_mean"]
+    var_g = var_o * (ospp / gspp)
+    vb = block(var_o + var_g, 4) / 16
+    z = (block(gpu, 4) - block(ora["mean"], 4)) / np.sqrt(np.maximum(vb, 1e-30))
+    live = vb > 1e-20
+    share = float((np.abs(z) <= 4.0)[live].mean())
+    n = W * H
+    zimg = (gpu.mean((0, 1)) - ora["mean"].mean((0, 1))) / np.sqrt((var_o + var_g).sum((0, 1)) / n ** 2)
+    converges.zmax = float(np.abs(z)[live].max())
+    return share, zimg
+
+
+def cornell_with(**changes):
+    d = scenes.cornell_dict(4, 3)
+    for k, v in changes.items():
+        d[k] = v(d) if callable(v) else v
+    return d
+
+
+def diffuse_glass(text):
+    """The scene with the Cornell box's glass sphere made diffuse (no caustics)."""
+    d = json.loads(text)
+    d["material"][4] = {"type": "lambert", "gaussians": [[0.6, 600.0, 80.0]]}
+    return json.dumps(d)
+
+
+def sellmeier_cornell():
+    d = scenes.cornell_dict(4, 3)
+    d["material"][4] = {"type": "dielectric", "sellmeier": BK7}
+    return json.dumps(d)
+
+
+def box_cornell():
+    d = scenes.cornell_dict(4, 3)
+    d["sphere"] = d["sphere"][:1]  # keep the Lambert sphere, drop the glass one
+    d["box"] = [{"position": [0.32, 0.2, 0.45], "rotation": [0.0, 25.0, 0.0], "size": [0.25, 0.4, 0.25],
+                 "materialID": 4, "lightID": 0}]
+    return json.dumps(d)
+
+
+def many_quads_cornell(n=40, seed=5):
+    """The Cornell box plus n small random parallelograms: more than 16, so the
+    parallelograms go into their own BVH."""
+    d = scenes.cornell_dict(4, 3)
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        c = rng.uniform(0.1, 0.8, 3)
+        e1, e2 = rng.uniform(-0.12, 0.12, 3), rng.uniform(-0.12, 0.12, 3)
+        d["quad"].append({"position": [float(x) for x in c], "edge1": [float(x) for x in e1],
+                          "edge2": [float(x) for x in e2], "materialID": int(rng.integers(1, 4)), "lightID": 0})
+    return json.dumps(d)
+
+
+def lens_cornell():
+    """The Cornell box through the reference's camera (sensor, aperture, BK7 lens)."""
+    d = scenes.cornell_dict(4, 3)
+    d["camera"] = scenes._camera([0.5, 0.5, -1.1], [0.0, 0.0], model="lens")
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("name", ["sellmeier", "box", "many_quads", "lens_camera"])
+def test_same_seed_paths_match_fp64_oracle(gpu_ctx, name):
+    text = {"sellmeier": sellmeier_cornell, "box": box_cornell, "many_quads": many_quads_cornell,
+            "lens_camera": lens_cornell}[name]()
+    W, H = 48, 48
+    f3, f2, ora = same_seed(gpu_ctx, text, W, H, 16, 8)
+    print(f"\n{name}: {f3 * 100:.2f}% of pixels within 1e-3, {f2 * 100:.2f}% within 1e-2")
+    if name == "lens_camera":
+        assert ora["lens_missed"] == 0
+    # glass: refraction paths part at grazing angles more often than diffuse ones
+    assert f3 >= (0.99 if name == "sellmeier" else 0.995) and f2 >= 0.995
+
+
+@pytest.mark.parametrize("name", ["sellmeier", "box", "many_quads", "lens_camera"])
+def test_extended_paths_converge_to_fp64_oracle(gpu_ctx, name):
+    text = {"sellmeier": sellmeier_cornell, "box": box_cornell, "many_quads": many_quads_cornell,
+            "lens_camera": lens_cornell}[name]()
+    if name != "sellmeier":
+        text = diffuse_glass(text)
+    share, zimg = converges(gpu_ctx, text, 32, 32, 2048, 512, 8)
+    print(f"\n{name}: {share * 100:.1f}% of blocks within 4 sigma, image-mean z = {np.round(zimg, 2)}")
+    assert np.all(np.abs(zimg) < 3.0)
+    if name != "sellmeier":  # glass makes caustics: heavy tails, image mean only
+        assert share >= 0.99
+
+
+def test_many_quads_traversal_is_invisible():
+    """The parallelogram BVH gives the same image under every traversal setting."""
+    text = many_quads_cornell()
+    imgs = []
+    for tr in (1, 2):
+        with pt.Context(0, traversal=tr) as c:
+            imgs.append(gpu_render(c, text, 40, 30, 8, 8))
+            assert c.counters()["bad_refs"] == 0
+    assert np.array_equal(imgs[0], imgs[1])
+
+
+def test_sellmeier_glass_keeps_energy(gpu_ctx):
+    """A smooth dispersive (Sellmeier) sphere absorbs nothing: under a uniform
+    sky it vanishes on average, the hero-wavelength collapse included."""
+    def furnace(mat, sphere=True):
+        return json.dumps({
+            "camera": scenes._camera([0.0, 0.0, -3.0], [0.0, 0.0], iso=1, ap=1.0, apdist=0.1),
+            "sphere": [{"position": [0.0, 0.0, 0.0], "radius": 1.0, "materialID": 1, "lightID": 0}] if sphere else [],
+            "material": [mat], "light": [], "environment": {"radiance": 1.0},
+            "render": {"wavelengths": 4, "russianRouletteDepth": 64, "seed": 3}})
+    mat = {"type": "dielectric", "sellmeier": BK7}
+    sky = gpu_render(gpu_ctx, furnace(mat, False), 32, 32, 256, 64).mean((0, 1))
+    img = gpu_render(gpu_ctx, furnace(mat), 32, 32, 256, 64).mean((0, 1))
+    np.testing.assert_allclose(img / sky, 1.0, rtol=0.02)
+
+
+def caustic_free_spheres():
+    d = scenes.random_spheres_dict(n=4096, scene_seed=1, wavelengths=4, seed=0)
+    d["material"][4] = {"type": "lambert", "gaussians": [[0.6, 600.0, 80.0]]}
+    for m in d["material"][6:]:
+        if m["type"] != "lambert":
+            m.clear()
+            m.update({"type": "lambert", "gaussians": [[0.5, 550.0, 90.0]]})
+    return json.dumps(d)
+
+
+@pytest.mark.parametrize("name", ["spheres", "sdf"])
+def test_blockwise_convergence_on_caustic_free_scenes(gpu_ctx, name):
+    """The headline's 4096-sphere scene with every glass and metal made diffuse,
+    and the SDF scene (all diffuse under a uniform sky): per 4x4 block, not
+    only the image mean."""
+    text, depth, (W, H), gspp, ospp = {
+        "spheres": (caustic_free_spheres(), 12, (48, 32), 4096, 1024),
+        "sdf": (scenes.sdf_scene(), 8, (48, 27), 1024, 256),
+    }[name]
+    share, zimg = converges(gpu_ctx, text, W, H, gspp, ospp, depth)
+    print(f"\n{name}: {share * 100:.1f}% of 4x4 blocks within 4 sigma (largest |z| {converges.zmax:.2f}), "
+          f"image-mean z = {np.round(zimg, 2)}")
+    # Tiny diffuse spheres right under the quad light receive NEE contributions
+    # that grow as 1/distance^2: a heavy tail that the oracle's sample variance
+    # at 1024 spp can understate for a block or two.
+    assert share >= 0.98 and converges.zmax < 6.0
+    assert np.all(np.abs(zimg) < 3.0)

@@ -151,13 +151,22 @@ def test_request_larger_than_a_wave_is_rejected_at_submit():
 
 def test_scene_with_overflowing_bounds_is_rejected(gpu_ctx):
     """A sphere whose box overflows the float range (centre 3e38, radius 1e38)
-    is refused at upload instead of reaching the BVH builder."""
+    is refused when the extended-mode scene is built (its first dispatch, or a
+    request's admission) instead of reaching the BVH builder."""
     d = json.loads(scenes.random_spheres(n=8, scene_seed=2, wavelengths=4))
     d["sphere"][3]["position"] = [3e38, 3e38, 3e38]
     d["sphere"][3]["radius"] = 1e38
+    text = json.dumps(d)
+    scene = gpu_ctx.scene_from_json(text)
+    r = pt.Renderer(gpu_ctx, scene, 16, 16, spf=1, path_length=4, mode=pt.MODE_EXT)
     with pytest.raises(pt.PTError) as e:
-        gpu_ctx.scene_from_json(json.dumps(d))
+        r.render_frame()
     assert e.value.status == pt.PT_ERR_INVALID_ARG and "float range" in str(e.value)
+    with pt.Server(gpu_ctx) as srv:
+        rid = srv.submit(text, 16, 16, spp_max=4, path_length=4)
+        with pytest.raises(pt.PTError):
+            srv.step()
+        assert srv.status(rid).state == pt.REQ_FAILED
 
 
 def attach_shipped_denoiser(ctx, srv, size=128, batch=1):
