@@ -240,6 +240,16 @@ void pt_accum_release(pt_accum* acc);
 pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* scene, pt_accum* acc, const pt_push_constants* pc,
                       pt_mode mode);
 pt_status pt_sync(pt_ctx* ctx);
+/* No counterpart in the reference: pt_dispatch(..., PT_MODE_REF) that allocates
+ * no device memory, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The path state
+ * lives in a static device array of 65,536 paths (11,010,304 B), held by each
+ * device the library runs on until the process ends; a dispatch of more paths
+ * runs as several waves through it, so it has no size limit beyond
+ * pt_dispatch's. Texels equal pt_dispatch's bit for bit. All contexts on one
+ * device share that array: calls that overlap on one device race. Returns
+ * PT_ERR_STATE while kernel profiling is enabled (it records events). */
+pt_status pt_dispatch_noalloc(pt_ctx* ctx, const pt_scene* scene, pt_accum* acc, const pt_push_constants* pc);
 
 /* Denoiser features (extended mode; no counterpart in the reference, which
  * shows the noisy running mean, src/shader.frag:81-99): traces the samples

@@ -219,6 +219,7 @@ SIGNATURES = {
     "pt_accum_write_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "pt_accum_release": (None, [_P]),
     "pt_dispatch": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants), C.c_int]),
+    "pt_dispatch_noalloc": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants)]),
     "pt_render_features": (C.c_int, [_P, _P, C.POINTER(PushConstants), C.POINTER(C.c_float), C.c_size_t]),
     "pt_sync": (C.c_int, [_P]),
     "pt_counters_get": (C.c_int, [_P, C.POINTER(Counters)]),

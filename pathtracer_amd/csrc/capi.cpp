@@ -101,6 +101,11 @@ pt_status pt_create(pt_ctx** out, int device_ordinal, const pt_config* cfg) {
         delete ctx;
         return set_global(PT_ERR_HIP, msg);
     }
+    // The address of this device's static wave for pt_dispatch_noalloc; a
+    // failure is reported by that entry point, not here.
+    ctx->static_wave_status = ptk::ref_static_setup(ctx);
+    ctx->static_wave_err = ctx->static_wave_status == PT_OK ? "" : ctx->err;
+    ctx->err.clear();
     *out = ctx;
     return PT_OK;
 }
@@ -377,6 +382,27 @@ pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_pu
     } catch (const std::exception& e) {
         return set_err(ctx, PT_ERR_HIP, std::string("pt_dispatch: ") + e.what());
     }
+}
+
+pt_status pt_dispatch_noalloc(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+    if (!ctx || !s || !acc || !pc) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch_noalloc: NULL argument");
+    if (s->ctx != ctx || acc->ctx != ctx)
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch_noalloc: objects of another context");
+    if (pc->resolution[0] != acc->w || pc->resolution[1] != acc->h)
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_dispatch_noalloc: push-constant resolution differs from the accumulator");
+    if (pc->samplesPerFrame <= 0 || pc->pathLength < 0 || pc->pathLength > 1024 || pc->currentSamples < pc->samplesPerFrame)
+        return set_err(ctx, PT_ERR_INVALID_ARG,
+                       "pt_dispatch_noalloc: need samplesPerFrame > 0, 0 <= pathLength <= 1024, "
+                       "currentSamples >= samplesPerFrame");
+    if (!s->ref_ok) return set_err(ctx, PT_ERR_CAPACITY, "reference mode cannot hold this scene: " + s->ref_err);
+    if (ctx->static_wave_status != PT_OK)
+        return set_err(ctx, ctx->static_wave_status, "pt_dispatch_noalloc: " + ctx->static_wave_err);
+    // kernel timing records events, which this entry point does not create
+    if (ctx->prof) return set_err(ctx, PT_ERR_STATE, "pt_dispatch_noalloc: kernel profiling is enabled");
+    (void)hipSetDevice(ctx->device);
+    int st = ptk::ref_dispatch_static(ctx, s, acc, pc);
+    if (st == PT_OK) st = ptk::count_dispatch(ctx);
+    return st;
 }
 
 pt_status pt_render_features(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, float* host, size_t n_floats) {

@@ -16,6 +16,7 @@
 // bit for bit wherever the operations are the same.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "tracer.hpp"
 
@@ -952,29 +953,24 @@ static T* carve(char*& p, size_t n) {
     return r;
 }
 
-static size_t ref_wave_bytes(int cap, int pcmax) {
-    size_t f = (size_t)cap, b = 0;
-    auto add = [&](size_t bytes) { b += ((bytes + 255) / 256) * 256; };
-    for (int i = 0; i < 6; ++i) add(f * 4);  // ray
-    add(f * 4); add(f * 4);                  // seed, lh
-    add(f * 16); add(f * 16); add(f * 4);    // T, L, w
-    for (int i = 0; i < 6; ++i) add(f * 4);  // hit
-    add(f * 4); add(f * 4);                  // queues
-    add(f * 4);                              // s_path
-    for (int i = 0; i < 6; ++i) add(f * 4);  // shadow ray
-    add(f * 4); add(f * 16); add(f * 4); add(f * 4);  // s_lobj, s_c, s_t, s_id
-    add((size_t)pcmax * 12);                 // sums
-    add(64);                                 // counts
-    return b;
+constexpr size_t round256(size_t bytes) { return ((bytes + 255) / 256) * 256; }
+// Bytes of a wave of `cap` paths as carve_ref_wave lays it out.
+constexpr size_t ref_wave_bytes(int cap) {
+    const size_t f = (size_t)cap;
+    return 6 * round256(f * 4)                                            // ray
+           + 2 * round256(f * 4)                                          // seed, lh
+           + 2 * round256(f * 16) + round256(f * 4)                       // T, L, w
+           + 6 * round256(f * 4)                                          // hit
+           + 2 * round256(f * 4)                                          // queues
+           + round256(f * 4)                                              // s_path
+           + 6 * round256(f * 4)                                          // shadow ray
+           + round256(f * 4) + round256(f * 16) + 2 * round256(f * 4)     // s_lobj, s_c, s_t, s_id
+           + round256(f * 12)                                             // sums
+           + round256(64);                                                // counts
 }
 
-static int ensure_ref_wave(pt_ctx* ctx, int cap) {
-    if (ctx->ref_wave.cap >= cap && ctx->ref_mem.p) return PT_OK;
-    dev_free(ctx, ctx->ref_mem);
-    int st = dev_alloc(ctx, ctx->ref_mem, ref_wave_bytes(cap, cap));
-    if (st != PT_OK) return st;
-    char* p = static_cast<char*>(ctx->ref_mem.p);
-    RefWave& W = ctx->ref_wave;
+// Lays a wave of `cap` paths out over `base` (ref_wave_bytes(cap) bytes).
+static void carve_ref_wave(char* p, int cap, unsigned long long* stats, RefWave& W) {
     size_t n = (size_t)cap;
     W.ox = carve<float>(p, n); W.oy = carve<float>(p, n); W.oz = carve<float>(p, n);
     W.dx = carve<float>(p, n); W.dy = carve<float>(p, n); W.dz = carve<float>(p, n);
@@ -989,8 +985,43 @@ static int ensure_ref_wave(pt_ctx* ctx, int cap) {
     W.s_lobj = carve<int>(p, n); W.s_c = carve<float4>(p, n); W.s_t = carve<float>(p, n); W.s_id = carve<int>(p, n);
     W.sum = carve<float>(p, 3 * n);
     W.counts = carve<int>(p, 16);
-    W.stats = ctx->d_stats;
+    W.stats = stats;
     W.cap = cap;
+}
+
+static int ensure_ref_wave(pt_ctx* ctx, int cap) {
+    if (ctx->ref_wave.cap >= cap && ctx->ref_mem.p) return PT_OK;
+    dev_free(ctx, ctx->ref_mem);
+    int st = dev_alloc(ctx, ctx->ref_mem, ref_wave_bytes(cap));
+    if (st != PT_OK) return st;
+    carve_ref_wave(static_cast<char*>(ctx->ref_mem.p), cap, ctx->d_stats, ctx->ref_wave);
+    return PT_OK;
+}
+
+// ---- the allocation-free dispatch's wave --------------------------------------------
+// A wave of kStaticWavePaths paths in a static device array: pt_dispatch_noalloc
+// traces through it instead of allocating. Each device holds its own copy for
+// the life of the process. Every launch writes the records it reads (queue
+// counts by k_ref_counts, sums on the first sample chunk), so nothing needs
+// clearing between calls; calls that overlap on one device race on it.
+constexpr int kStaticWavePaths = 1 << 16;
+constexpr size_t kStaticWaveBytes = ref_wave_bytes(kStaticWavePaths);  // 11,010,304 B
+static __device__ __align__(256) char scratch[kStaticWaveBytes];
+constexpr int kMaxStaticDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch[kMaxStaticDevices] = {};
+
+int ref_static_setup(pt_ctx* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxStaticDevices) {
+        ctx->err = "static dispatch wave: device ordinal beyond 63";
+        return PT_ERR_CAPACITY;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[ctx->device]) return PT_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "static dispatch wave");
+    g_scratch[ctx->device] = static_cast<char*>(ptr);
     return PT_OK;
 }
 
@@ -1141,15 +1172,13 @@ static void launch_shadow_march(pt_ctx* ctx, const RefSceneDev& S, RefWave& W, i
     launch(ctx, "k_ref_shadow_march", k_ref_shadow_march<KIND>, dim3(grid), dim3(kBlock), S, W);
 }
 
-int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+// One dispatch through the wave Wv (capacity Wv.cap paths): pixels in chunks of
+// at most cap, each chunk's samples in waves that fill it.
+static int ref_dispatch_wave(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc, RefWave& Wv) {
     const RefSceneDev& S = s->ref_dev;
     const int W = pc->resolution[0], H = pc->resolution[1];
     const int P = W * H;
-    int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 21);
-    cap = std::max(cap, 1024);
-    int st = ensure_ref_wave(ctx, cap);
-    if (st != PT_OK) return st;
-    RefWave& Wv = ctx->ref_wave;
+    const int cap = Wv.cap;
     RefFrame F = make_frame(pc);
     const int pcmax = std::min(P, cap);
     for (int p0 = 0; p0 < P; p0 += pcmax) {
@@ -1207,6 +1236,29 @@ int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_co
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(ctx, e, "reference dispatch launch");
     return PT_OK;
+}
+
+int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+    int cap = ctx->cfg.wave_paths > 0 ? ctx->cfg.wave_paths : (1 << 21);
+    cap = std::max(cap, 1024);
+    int st = ensure_ref_wave(ctx, cap);
+    if (st != PT_OK) return st;
+    return ref_dispatch_wave(ctx, s, acc, pc, ctx->ref_wave);
+}
+
+int ref_dispatch_static(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc) {
+    char* base = nullptr;
+    if (ctx->device >= 0 && ctx->device < kMaxStaticDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        base = g_scratch[ctx->device];
+    }
+    if (!base) {
+        ctx->err = "pt_dispatch_noalloc: the static wave of this device is not set up";
+        return PT_ERR_STATE;
+    }
+    RefWave Wv{};
+    carve_ref_wave(base, kStaticWavePaths, ctx->d_stats, Wv);
+    return ref_dispatch_wave(ctx, s, acc, pc, Wv);
 }
 
 }  // namespace ptk

@@ -116,6 +116,9 @@ struct pt_ctx {
     hipStream_t stream = nullptr;
     pt_config cfg{};
     std::string err;
+    // pt_dispatch_noalloc: whether this device's static wave was found at pt_create
+    int static_wave_status = PT_OK;
+    std::string static_wave_err;
     int64_t allocated = 0;
     unsigned long long* d_stats = nullptr;  // pt_counters fields, 16 slots
     // kernel timing (pt_profile_enable)
@@ -164,6 +167,11 @@ int hip_fail(pt_ctx* ctx, hipError_t e, const char* what);
 int ref_scene_upload(pt_ctx* ctx, pt_scene* s);
 int ref_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc);
 void ref_wave_release(pt_ctx* ctx);
+// The allocation-free variant: the same kernels through a static device wave.
+// ref_static_setup looks the wave's address up on the context's device (once
+// per device; call with that device current).
+int ref_static_setup(pt_ctx* ctx);
+int ref_dispatch_static(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc);
 
 // Extended mode (ext_tracer.hip).
 int ext_scene_build(pt_ctx* ctx, pt_scene* s);
