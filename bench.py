@@ -894,6 +894,11 @@ def trace_bench(args):
     else:
         config["bvh_nodes_per_segment"] = cnt["bvh_nodes"] / max(1, cnt["segments"])
         config["bvh_nodes_per_shadow_ray"] = cnt["shadow_bvh_nodes"] / max(1, cnt["shadow_segments"])
+    if args.workload != "ref":
+        # segments whose ray left the scene (camera rays outside the box's
+        # opening among them): the rate over the segments that hit something
+        config["escaped"] = cnt["escaped"]
+        config["hit_segments_per_s"] = (cnt["segments"] - cnt["escaped"]) / dt
     line = {
         "metric": METRIC,
         "value": seg_rate,
