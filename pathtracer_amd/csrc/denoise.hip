@@ -19,9 +19,10 @@
 // 16 results of lane l are rows (v / 4) * 8 + (l / 32) * 4 + v % 4, column
 // l % 32):
 //   forward / backward-data: rows = pixels, columns = output channels,
-//     K = 9 taps x input channels in chunks of 16 (each lane loads 8
-//     consecutive channels: 16 bytes of NHWC); backward-data runs the forward
-//     kernel on the output gradient with the weights flipped and transposed;
+//     K = 9 taps x input channels in steps of 16 (each lane reads 8
+//     consecutive channels: 16 bytes of NHWC), both operands staged in LDS per
+//     tile; backward-data runs the forward kernel on the output gradient with
+//     the weights flipped and transposed;
 //   backward-weights: rows = output channels, columns = input channels,
 //     K = pixels; each wave stages 16 pixels x 32 channels of both operands in
 //     LDS to read them transposed, and adds its partial sums into the FP32
@@ -65,13 +66,8 @@ union Frag {
 // ---------------------------------------------------------------------------------
 // Conv forward (also backward-data). X1 (C1 channels) and X2 (C2 channels, may
 // be null) form the input [N, H, W, C1 + C2]; Wt is [9][Cout][C1 + C2] BF16;
-// output channels [0, Co1) go to Y1 (Co1 channels) and [Co1, Cout) to Y2.
-// A block of 4 waves computes 4 * 32 * R pixels x 32 * NC output channels: per
-// tap it stages the tap's weights for its channel tile in LDS (rows padded by
-// 8 channels against bank conflicts), then each wave runs R x NC MFMA tiles of
-// 32 pixels x 32 channels over the input channels, 16 at a time; each B
-// fragment read from LDS feeds R MFMAs, each A fragment (one 16-byte load of 8
-// channels of one pixel per lane) feeds NC. `flip` reads tap 8 - t
+// output channels [0, Co1) go to Y1 (Co1 channels) and [Co1, Cout) to Y2;
+// bias and ReLU are applied at the store. `flip` reads tap 8 - t
 // (backward-data).
 struct ConvArgs {
     const u16* X1;
@@ -87,94 +83,102 @@ struct ConvArgs {
     int flip;
 };
 
-// Largest input channel count a k_conv<NC> block stages (its LDS size).
-__host__ __device__ constexpr int conv_max_cin(int nc) { return nc == 4 ? 128 : 192; }
-
-template <int NC, int R>
-__global__ void __launch_bounds__(kThreads) k_conv(ConvArgs a) {
-    constexpr int kLdw = conv_max_cin(NC) + 8;
-    __shared__ __align__(16) u16 s_w[32 * NC * kLdw];
+// ---------------------------------------------------------------------------------
+// Conv forward (also backward-data), tiled: a block computes TH rows x 32
+// columns of one image's pixels x 32 * NC output channels. Per chunk of 32
+// input channels it stages in LDS the (TH + 2) x 34 input tile around its
+// pixels (zero outside the image) and the chunk's weights for all 9 taps;
+// each wave then runs its TH / 4 pixel rows x NC channel tiles
+// of 32 x 32 MFMAs over 9 taps x 2 steps of 16 channels with both operands
+// read from LDS. Every input pixel is read from memory by about (TH + 2)(34) /
+// (32 TH) tiles instead of once per tap. Pixels per 16-byte vector in LDS are
+// padded to 40 channels so that 32 lanes reading 32 neighbouring pixels spread
+// over the banks.
+template <int NC, int TH>
+__global__ void __launch_bounds__(kThreads) k_conv_t(ConvArgs a) {
+    constexpr int CK = 32, CKP = CK + 8;  // channels per chunk; LDS pitch per pixel / weight row
+    constexpr int TW = 32, IW = TW + 2, IH = TH + 2;
+    constexpr int R = TH / 4;             // pixel rows per wave
+    __shared__ __align__(16) u16 s_x[IH * IW * CKP];
+    __shared__ __align__(16) u16 s_w[9 * 32 * NC * CKP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int P = a.N * a.H * a.W;
-    const int pix0 = (blockIdx.x * 4 + wave) * 32 * R;
+    const int tiles_x = (a.W + TW - 1) / TW, tiles_y = (a.H + TH - 1) / TH;
+    int blk = blockIdx.x;
+    const int tx = blk % tiles_x;
+    blk /= tiles_x;
+    const int ty = blk % tiles_y;
+    const int n = blk / tiles_y;
+    const int x0 = tx * TW, y0 = ty * TH;
     const int co0 = blockIdx.y * 32 * NC;
     const int Cin = a.C1 + a.C2;
-    const int ldw = Cin + 8;
-    // this lane's R A rows (pixels pix0 + lane % 32, 32 further, ...)
-    int pn[R], py[R], px[R];
-    bool pin[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int pm = pix0 + r * 32 + (lane & 31);
-        pin[r] = pm < P;
-        pn[r] = pin[r] ? pm / (a.H * a.W) : 0;
-        const int prem = pin[r] ? pm % (a.H * a.W) : 0;
-        py[r] = prem / a.W;
-        px[r] = prem % a.W;
-    }
-    const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-chunk
+    const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-channel step
+    const int col = lane & 31;
     f16v acc[R][NC];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int j = 0; j < NC; ++j) acc[r][j] = f16v{};
-    const int nvec = 32 * NC * (Cin >> 3);  // 16-byte vectors of the tap's weight tile
-    for (int t = 0; t < 9; ++t) {
-        const int tt = a.flip ? 8 - t : t;
-        __syncthreads();  // every wave is done with the previous tap's tile
-        for (int i = threadIdx.x; i < nvec; i += kThreads) {
-            const int row = i / (Cin >> 3), cv = (i % (Cin >> 3)) * 8;
-            const int co = co0 + row;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (co < a.Cout) v = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + cv);
-            *reinterpret_cast<uint4*>(&s_w[row * ldw + cv]) = v;
+    for (int c0 = 0; c0 < Cin; c0 += CK) {
+        __syncthreads();  // every wave is done with the previous chunk's tiles
+        for (int i = threadIdx.x; i < IH * IW * (CK / 8); i += kThreads) {
+            const int v = i % (CK / 8), pix = i / (CK / 8);
+            const int y = y0 + pix / IW - 1, x = x0 + pix % IW - 1;
+            const int c = c0 + v * 8;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
+                const size_t q = ((size_t)n * a.H + y) * a.W + x;
+                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
+                val = *reinterpret_cast<const uint4*>(src);
+            }
+            *reinterpret_cast<uint4*>(&s_x[pix * CKP + v * 8]) = val;
+        }
+        for (int i = threadIdx.x; i < 9 * 32 * NC * (CK / 8); i += kThreads) {
+            const int v = i % (CK / 8), row = (i / (CK / 8)) % (32 * NC), t = i / ((CK / 8) * 32 * NC);
+            const int tt = a.flip ? 8 - t : t;
+            const int co = co0 + row, c = c0 + v * 8;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (co < a.Cout && c < Cin) val = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + c);
+            *reinterpret_cast<uint4*>(&s_w[(t * 32 * NC + row) * CKP + v * 8]) = val;
         }
         __syncthreads();
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
-        size_t q[R];
-        bool in[R];
+#pragma unroll 1
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;  // offsets into the halo-padded tile
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int y = py[r] + dy, x = px[r] + dx;
-            in[r] = pin[r] && y >= 0 && y < a.H && x >= 0 && x < a.W;
-            q[r] = ((size_t)pn[r] * a.H + (in[r] ? y : 0)) * a.W + (in[r] ? x : 0);
-        }
-        for (int c0 = 0; c0 < Cin; c0 += 16) {
-            const int c = c0 + kq;
-            Frag A[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                A[r].u = make_uint4(0, 0, 0, 0);
-                if (in[r]) {
-                    const u16* src = c < a.C1 ? a.X1 + q[r] * a.C1 + c : a.X2 + q[r] * a.C2 + (c - a.C1);
-                    A[r].u = *reinterpret_cast<const uint4*>(src);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < NC; ++j) {
-                Frag B;
-                B.u = *reinterpret_cast<const uint4*>(&s_w[(j * 32 + (lane & 31)) * ldw + c]);
+            for (int k = 0; k < CK; k += 16) {
+                Frag A[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, B.v, acc[r][j], 0, 0, 0);
+                    A[r].u = *reinterpret_cast<const uint4*>(&s_x[((wave * R + r + dy) * IW + col + dx) * CKP + k + kq]);
+#pragma unroll
+                for (int j = 0; j < NC; ++j) {
+                    Frag B;
+                    B.u = *reinterpret_cast<const uint4*>(&s_w[(t * 32 * NC + j * 32 + col) * CKP + k + kq]);
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        acc[r][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[r].v, B.v, acc[r][j], 0, 0, 0);
+                }
             }
         }
     }
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
-        const int co = co0 + j * 32 + (lane & 31);
+        const int co = co0 + j * 32 + col;
         if (co >= a.Cout) continue;
         const float b = a.bias ? a.bias[co] : 0.0f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            const int y = y0 + wave * R + r;
+            if (y >= a.H) continue;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int p = pix0 + r * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                if (p >= P) continue;
+                const int x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                if (x >= a.W) continue;
+                const size_t p = ((size_t)n * a.H + y) * a.W + x;
                 float o = acc[r][j][v] + b;
                 if (a.relu) o = fmaxf(o, 0.0f);
-                if (co < a.Co1) a.Y1[(size_t)p * a.Co1 + co] = f2bf(o);
-                else a.Y2[(size_t)p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(o);
+                if (co < a.Co1) a.Y1[p * a.Co1 + co] = f2bf(o);
+                else a.Y2[p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(o);
             }
         }
     }
@@ -385,7 +389,7 @@ __global__ void k_head(const float* f, const u16* out, const float* target, floa
     }
 }
 // Adam on FP32 masters; then the BF16 forward copy Wf[t][co][ci] and the
-// backward-data copy Wb[t][ci][co] (k_conv with `flip` reads tap 8 - t: the
+// backward-data copy Wb[t][ci][co] (k_conv_t with `flip` reads tap 8 - t: the
 // gradient of tap t's offset o is gathered from offset -o, which is tap 8 - t).
 __global__ void k_adam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
                        float c1, float c2) {
@@ -529,9 +533,6 @@ int dn_fail(pt_denoiser* d, int st, const std::string& msg) {
     return st;
 }
 
-// Output channels per block of k_conv: 32 * conv_nc(Cout).
-int conv_nc(int cout) { return cout >= 128 ? 4 : (cout >= 64 ? 2 : 1); }
-
 void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, int N, int H, int W, bool bwd_data,
           u16* y2 = nullptr, int co1 = 0) {
     ConvArgs a{};
@@ -549,14 +550,12 @@ void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, 
         a.Y1 = y; a.Y2 = y2; a.Co1 = co1 ? co1 : l.cin1 + l.cin2; a.Cout = l.cin1 + l.cin2;
         a.relu = 0; a.flip = 1;
     }
-    const int P = N * H * W;
-    const int nc = conv_nc(a.Cout);
-    // 128 accumulators per lane at NC = 4 with one pixel row; two rows below
-    const int rows = nc == 4 ? 1 : 2;
-    dim3 grid((P + 128 * rows - 1) / (128 * rows), (a.Cout + 32 * nc - 1) / (32 * nc));
-    if (nc == 4) ptk::launch(d->ctx, "k_dn_conv", k_conv<4, 1>, grid, dim3(kThreads), a);
-    else if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv<2, 2>, grid, dim3(kThreads), a);
-    else ptk::launch(d->ctx, "k_dn_conv", k_conv<1, 2>, grid, dim3(kThreads), a);
+    // tiles of 8 x 32 pixels x 64 output channels (32 below 64 channels)
+    constexpr int kTH = 8;
+    const int nc = a.Cout >= 64 ? 2 : 1;
+    dim3 grid((unsigned)(N * ((H + kTH - 1) / kTH) * ((W + 31) / 32)), (a.Cout + 32 * nc - 1) / (32 * nc));
+    if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv_t<2, kTH>, grid, dim3(kThreads), a);
+    else ptk::launch(d->ctx, "k_dn_conv", k_conv_t<1, kTH>, grid, dim3(kThreads), a);
 }
 
 void wgrad(pt_denoiser* d, const Layer& l, const u16* dz, const u16* x1, const u16* x2, int N, int H, int W) {
@@ -742,7 +741,11 @@ int dn_request_estimate(pt_denoiser* d, const DnRequest& q, float scale, int spp
     const double credit = std::min(1.0, spp / 64.0);
     const double err2 = err2_half - credit * 0.25 * s[1][3];
     const double m = s[0][2];
-    *rel = m > 0.0 ? std::sqrt(std::max(err2, 0.0) / m) : INFINITY;
+    // Both terms are differences of sums. On heavy-tailed scenes at few samples
+    // (fireflies in a half's variance) they can come out zero or negative: the
+    // estimate is then inconclusive, never a zero error that would finish the
+    // request denoised.
+    *rel = (m > 0.0 && err2_half > 0.0 && err2 > 0.0) ? std::sqrt(err2 / m) : INFINITY;
     return PT_OK;
 }
 
@@ -781,10 +784,11 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
         l.b_off = off;
         off += (size_t)l.cout;
         d->L.push_back(l);
-        // k_conv stages one tap's weight tile in LDS: the input channels of both
-        // passes (forward, backward-data) must fit it and be multiples of 16
+        // k_conv_t reads 8 channels per 16-byte vector and splits the input at
+        // C1: channel counts of both passes (forward, backward-data) are
+        // multiples of 16
         const int cin = l.cin1 + l.cin2;
-        if (cin % 16 || l.cout % 16 || cin > conv_max_cin(conv_nc(l.cout)) || l.cout > conv_max_cin(conv_nc(cin))) {
+        if (cin % 16 || l.cout % 16 || l.cin1 % 16) {
             delete d;
             return PT_ERR_INVALID_ARG;
         }

@@ -1887,6 +1887,11 @@ struct ExtCtxState {
     ExtWave wave{};
     int cu = 0;  // compute units of the context's device (grid sizing)
     DevAlloc batch_mem;  // batched waves: header + scene, frame, segment and sink tables of up to kMaxBatch requests
+    // pinned host copy of those tables: one asynchronous upload per wave, and
+    // an event marking when the staging copy may be rewritten
+    void* batch_host = nullptr;
+    hipEvent_t batch_done = nullptr;
+    bool batch_pending = false;
     std::map<uint64_t, hipGraphExec_t> graphs;  // captured batched waves, by launch shape
 };
 
@@ -2221,6 +2226,9 @@ void ext_wave_release(pt_ctx* ctx) {
     ExtCtxState* st = static_cast<ExtCtxState*>(ctx->ext_state);
     if (!st) return;
     drop_graphs(st);
+    if (st->batch_done) (void)hipEventSynchronize(st->batch_done);
+    if (st->batch_done) (void)hipEventDestroy(st->batch_done);
+    if (st->batch_host) (void)hipHostFree(st->batch_host);
     dev_free(ctx, st->batch_mem);
     dev_free(ctx, st->mem);
     delete st;
@@ -2555,26 +2563,39 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
         st = dev_alloc(ctx, cs->batch_mem, tb);
         if (st != PT_OK) return st;
     }
+    hipError_t e = hipSuccess;
+    if (!cs->batch_host) {
+        e = hipHostMalloc(&cs->batch_host, tb, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&cs->batch_done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (cs->batch_host) (void)hipHostFree(cs->batch_host);
+            cs->batch_host = nullptr;
+            return hip_fail(ctx, e, "batched wave staging");
+        }
+    }
+    // the previous wave's upload has left the staging copy before it is rewritten
+    if (cs->batch_pending) e = hipEventSynchronize(cs->batch_done);
+    if (e != hipSuccess) return hip_fail(ctx, e, "batched wave staging");
+    cs->batch_pending = false;
     char* base = static_cast<char*>(cs->batch_mem.p);
+    char* hb = static_cast<char*>(cs->batch_host);
     ExtBatchHdr hdr{(int)npaths, (int)npix, n, 0};
+    std::memcpy(hb, &hdr, sizeof hdr);
+    std::memcpy(hb + o_sc, scenes.data(), sizeof(ExtSceneDev) * (size_t)n);
+    std::memcpy(hb + o_fr, frames.data(), sizeof(ExtFrame) * (size_t)n);
+    std::memcpy(hb + o_sg, segs.data(), sizeof(ExtSeg) * (size_t)n);
+    std::memcpy(hb + o_sk, sinks.data(), sizeof(ExtMomentSink) * (size_t)n);
     const ExtBatchHdr* d_hdr = reinterpret_cast<const ExtBatchHdr*>(base);
     const ExtSceneDev* d_scenes = reinterpret_cast<const ExtSceneDev*>(base + o_sc);
     const ExtFrame* d_frames = reinterpret_cast<const ExtFrame*>(base + o_fr);
     const ExtSeg* d_segs = reinterpret_cast<const ExtSeg*>(base + o_sg);
     const ExtMomentSink* d_sinks = reinterpret_cast<const ExtMomentSink*>(base + o_sk);
-    hipError_t e = hipMemcpyAsync(base, &hdr, sizeof hdr, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(base + o_sc, scenes.data(), sizeof(ExtSceneDev) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(base + o_fr, frames.data(), sizeof(ExtFrame) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(base + o_sg, segs.data(), sizeof(ExtSeg) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(base + o_sk, sinks.data(), sizeof(ExtMomentSink) * (size_t)n, hipMemcpyHostToDevice,
-                           ctx->stream);
-    // the copies read pageable host vectors: wait for them before the vectors go
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    // one upload of the used part of the tables (from pinned memory, so it is
+    // asynchronous: no host wait before the wave is enqueued)
+    e = hipMemcpyAsync(base, hb, o_sk + sizeof(ExtMomentSink) * (size_t)n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(cs->batch_done, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "batched wave tables");
+    cs->batch_pending = true;
     plan.B = ExtBatch{d_scenes, d_frames};
     plan.S0 = scenes[0];
     plan.F0 = frames[0];

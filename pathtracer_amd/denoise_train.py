@@ -146,20 +146,59 @@ def train_two_phase(dn: pt.Denoiser, pairs, steps: int, batch: int = 8, crop: in
     return losses
 
 
+# Held-out scenes (never trained on): a random-sphere box and the SDF scene.
+HELD_OUT = {"spheres": 100, "sdf": 101}
+
+
+def evaluate(ctx: pt.Context, params: np.ndarray, spp_list, size: int = 128, spp_ref: int = 8192) -> dict:
+    """Relative RMSE of the noisy input and of the denoised image against a
+    spp_ref-sample render, on the held-out scenes at each input sample count."""
+    out = {}
+    with pt.Denoiser(ctx, 1, size, size) as dn:
+        dn.set_params(params)
+        for kind, i in HELD_OUT.items():
+            text, depth = training_scene(i)
+            sc = ctx.scene_from_json(text)
+            try:
+                ref = render_mean(ctx, sc, size, size, spp_ref, depth, first=10 ** 5, chunk=512)
+                for spp in spp_list:
+                    feat = render_features(ctx, sc, size, size, spp, depth)
+                    out[(kind, spp)] = (rel_rmse(feat[..., :3], ref), rel_rmse(denoise_image(dn, feat), ref))
+            finally:
+                sc.release()
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--eval", nargs="+", metavar="NPY",
+                    help="instead of training: held-out relative RMSE, noisy and denoised, of these parameter files")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--first", type=int, default=0,
                     help="index of the first training scene (scenes 100 and 101 are the gain tests' held-out ones)")
     ap.add_argument("--size", type=int, default=192)
     ap.add_argument("--spp-in", type=str, default="4,16,64,256,1024")
-    ap.add_argument("--spp-ref", type=int, default=4096)
+    ap.add_argument("--spp-ref", type=int, default=4096, help="samples per pixel of a target (at least)")
+    ap.add_argument("--ref-factor", type=int, default=16,
+                    help="a target has at least this many times its input's samples (up to --spp-ref-max), so "
+                         "that the target's own noise stays well below the input's at every input level")
+    ap.add_argument("--spp-ref-max", type=int, default=16384)
     ap.add_argument("--steps", type=int, default=16000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=96)
     ap.add_argument("--chunk", type=int, default=512, help="samples per pixel per dispatch of the targets")
-    ap.add_argument("--out", type=str, required=True)
+    ap.add_argument("--out", type=str, default="")
     a = ap.parse_args(argv)
+    if a.eval:
+        with pt.Context(0) as ctx:
+            for w in a.eval:
+                res = evaluate(ctx, np.load(w, allow_pickle=False), [4, 16, 64, 256, 1024])
+                for (kind, spp), (noisy, den) in sorted(res.items()):
+                    print(f"{w}: {kind} {spp:5d} spp: noisy {noisy:.4f} denoised {den:.4f} ratio {den / noisy:.3f}",
+                          flush=True)
+        return 0
+    if not a.out:
+        ap.error("--out is required for training")
     spp_in = [int(v) for v in a.spp_in.split(",")]
     t0 = time.perf_counter()
     with pt.Context(0) as ctx:
@@ -168,11 +207,12 @@ def main(argv=None) -> int:
             if a.first <= 101 < a.first + a.scenes and i in (100, 101):
                 continue  # held out for the gain tests
             text, depth = training_scene(i)
-            feat, target = make_pair(ctx, text, depth, a.size, a.size, spp_in[i % len(spp_in)], a.spp_ref,
-                                     chunk=a.chunk)
+            s_in = spp_in[i % len(spp_in)]
+            s_ref = min(max(a.spp_ref, a.ref_factor * s_in), max(a.spp_ref, a.spp_ref_max))
+            feat, target = make_pair(ctx, text, depth, a.size, a.size, s_in, s_ref, chunk=a.chunk)
             pairs.append((feat, target))
             if len(pairs) % 50 == 1 or a.scenes <= 100:
-                print(f"pair {i}: spp {spp_in[i % len(spp_in)]} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
+                print(f"pair {i}: spp {s_in} -> {s_ref} noisy rel {rel_rmse(feat[..., :3], target):.3f} "
                       f"({time.perf_counter() - t0:.0f} s)", flush=True)
         with pt.Denoiser(ctx, a.batch, a.crop, a.crop, seed=1) as dn:
             train_two_phase(dn, pairs, a.steps, a.batch, a.crop, log_every=500)
