@@ -130,6 +130,11 @@ struct ExtWave {
     uint4* aux;    // x: RNG key, y: flags (bit 0 companions dropped, bit 1 last bounce specular), z: BSDF pdf bits
     float2* hit;   // (t, primitive reference bits)
     int* q[2];
+    // the extension rays of queue c in queue order (entry e: the ray of path
+    // q[c][e]), written where the shading kernels append, so the intersect
+    // kernels read them coalesced; org / dir keep them by path for shading
+    float4* rq_org[2];
+    float4* rq_dir[2];
     int* mq[N_MATQ];
     float4* s_org; // shadow ray origin, w = tmax
     float4* s_dir; // direction, w = path index bits
@@ -158,6 +163,21 @@ struct ExtBatch {
 template <bool MULTI>
 __device__ __forceinline__ const ExtSceneDev& scene_of(const ExtSceneDev& S0, const ExtBatch& B, uint32_t rid) {
     return MULTI ? B.scenes[rid] : S0;
+}
+
+// The ray of entry e (path p) of extension queue `cur`: from the queue-ordered
+// copy (rq != 0, every bounce after the first), else by path (bounce 0, whose
+// queue is the identity, so both read the same addresses in order).
+__device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int rq, int e, int p, f3& o, f3& d) {
+    if (rq) {
+        const float4 a = Wv.rq_org[cur][e], b = Wv.rq_dir[cur][e];
+        o = mk3(a.x, a.y, a.z);
+        d = mk3(b.x, b.y, b.z);
+    } else {
+        const float4 a = Wv.org[p], b = Wv.dir[p];
+        o = mk3(a.x, a.y, a.z);
+        d = mk3(b.x, b.y, b.z);
+    }
 }
 
 static __constant__ float c_cie[PT_CIE_SIZE] = {
@@ -896,7 +916,7 @@ __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& W
 // Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
 // is also appended to its material queue (scenes without SDFs).
 template <int W, int STACK, typename SE, bool CLASSIFY, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
+__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, cur);
@@ -911,7 +931,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
         uint32_t ref = kMiss;
         if (active) {
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            f3 o, d;
+            ray_of(Wv, cur, rq, idx, p, o, d);
             float t = 1e30f;
             scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, visits);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
@@ -942,7 +963,7 @@ constexpr int kRefill = 16;
 // atomic per wave and class for 64 paths, where appending as lanes finish
 // would take one per finishing lane).
 template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur) {
+__global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur, int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, cur);
@@ -960,8 +981,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
             int e = wave_refill(idle, chunk);
             if (e >= 0) {
                 p = q[e];
-                o = ld3(Wv.org, p);
-                d = ld3(Wv.dir, p);
+                ray_of(Wv, cur, rq, e, p, o, d);
                 t = 1e30f;
                 ref = kMiss;
                 best = -1;
@@ -1011,7 +1031,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
 // fetches; one launch instead of two pays that once per bounce. Scenes without
 // SDF objects only (their shadow rays also march the SDFs).
 template <int W, int STACK, typename SE, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq) {
+__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq,
+                                                     int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
     SE* stack = stack_mem + threadIdx.x;
     const int ne = *qlen(Wv, cur), ns = *qlen(Wv, shq);
@@ -1032,7 +1053,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
             p = Wv.q[cur][idx];
             if (MULTI) rid = Wv.aux[p].w;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            f3 o, d;
+            ray_of(Wv, cur, rq, idx, p, o, d);
             float t = 1e30f;
             scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, ev);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
@@ -1054,14 +1076,15 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
 
 // Sphere tracing of every SDF object of one kind against the current hit.
 template <int KIND, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
+__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     const int n = *qlen(Wv, cur);
     unsigned steps = 0, exhausted = 0;
     unsigned long long flops = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
-        f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+        f3 o, d;
+        ray_of(Wv, cur, rq, idx, p, o, d);
         float2 h = Wv.hit[p];
         float t = h.x;
         uint32_t ref = __float_as_uint(h.y);
@@ -1094,7 +1117,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
 constexpr int kMarchRefill = 8;
 
 template <int KIND, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
+__global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     const int n = *qlen(Wv, cur);
     const int* q = Wv.q[cur];
     unsigned steps = 0, exhausted = 0, it = 0;
@@ -1123,8 +1146,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
                 sdfs = S.sdf;
                 nsdf = S.nsdf;
                 smax = S.sdf_steps_max;
-                o = ld3(Wv.org, p);
-                d = ld3(Wv.dir, p);
+                ray_of(Wv, cur, rq, e, p, o, d);
                 const float2 h = Wv.hit[p];
                 t = h.x;
                 ref = __float_as_uint(h.y);
@@ -1408,6 +1430,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
         bool active = idx < n;
         int p = 0;
         bool cont = false, shadow = false;
+        float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         f3 sP = mk3(0, 0, 0), sD = sP, sC = sP;
         float smax = 0.0f;
         uint32_t rid = 0u;
@@ -1491,8 +1514,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
             T = T * rho;
             if (bounce + 1 < maxDepth && z > 0.0f && roulette(S, key, bounce, T)) {
                 cont = true;
-                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
-                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.org[p] = ro;
+                Wv.dir[p] = rd;
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
                 Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), aux.w);
             }
@@ -1505,7 +1530,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
             Wv.s_dir[slot[0]] = make_float4(sD.x, sD.y, sD.z, __int_as_float(p));
             Wv.s_xyz[slot[0]] = make_float4(sC.x, sC.y, sC.z, __uint_as_float(rid));
         }
-        if (cont) Wv.q[nxt][slot[1]] = p;
+        if (cont) {
+            Wv.q[nxt][slot[1]] = p;
+            Wv.rq_org[nxt][slot[1]] = ro;
+            Wv.rq_dir[nxt][slot[1]] = rd;
+        }
     }
 }
 
@@ -1524,6 +1553,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
         bool active = idx < n;
         int p = 0;
         bool cont = false;
+        float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
             p = Wv.mq[Q_DIELECTRIC][idx];
             uint4 aux = Wv.aux[p];
@@ -1565,8 +1595,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
             }
             if (bounce + 1 < maxDepth && roulette(S, key, bounce, T)) {
                 cont = true;
-                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
-                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.org[p] = ro;
+                Wv.dir[p] = rd;
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
                 Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, aux.w);
             }
@@ -1574,7 +1606,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
         int* const ctr[1] = {qlen(Wv, nxt)};
         int slot[1];
         block_append<kBlk, 1>(ctr, cont ? 1u : 0u, slot);
-        if (cont) Wv.q[nxt][slot[0]] = p;
+        if (cont) {
+            Wv.q[nxt][slot[0]] = p;
+            Wv.rq_org[nxt][slot[0]] = ro;
+            Wv.rq_dir[nxt][slot[0]] = rd;
+        }
     }
 }
 
@@ -1592,6 +1628,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
         bool active = idx < n;
         int p = 0;
         bool cont = false;
+        float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
             p = Wv.mq[Q_CONDUCTOR][idx];
             uint4 aux = Wv.aux[p];
@@ -1609,8 +1646,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
             f3 P = sf.p + N * offset_scale(sf.p);
             if (bounce + 1 < maxDepth && roulette(S, aux.x, bounce, T)) {
                 cont = true;
-                Wv.org[p] = make_float4(P.x, P.y, P.z, 0.0f);
-                Wv.dir[p] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                Wv.org[p] = ro;
+                Wv.dir[p] = rd;
                 Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
                 Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, aux.w);
             }
@@ -1618,7 +1657,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
         int* const ctr[1] = {qlen(Wv, nxt)};
         int slot[1];
         block_append<kBlk, 1>(ctr, cont ? 1u : 0u, slot);
-        if (cont) Wv.q[nxt][slot[0]] = p;
+        if (cont) {
+            Wv.q[nxt][slot[0]] = p;
+            Wv.rq_org[nxt][slot[0]] = ro;
+            Wv.rq_dir[nxt][slot[0]] = rd;
+        }
     }
 }
 
@@ -2200,9 +2243,10 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     size_t n = (size_t)cap;
     // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
     // shadow org/dir/xyz 48, sums 12
-    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12;
-    // + 256-byte rounding of 22 arrays + the counters and cursors, one line each
-    int rc = dev_alloc(ctx, st->mem, n * per_path + 20 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
+    // queue-ordered rays 64
+    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12 + 64;
+    // + 256-byte rounding of 26 arrays + the counters and cursors, one line each
+    int rc = dev_alloc(ctx, st->mem, n * per_path + 28 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
     char* p = static_cast<char*>(st->mem.p);
     ExtWave& W = st->wave;
@@ -2210,6 +2254,10 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     W.T = carve_e<float4>(p, n); W.acc = carve_e<float4>(p, n);
     W.aux = carve_e<uint4>(p, n); W.hit = carve_e<float2>(p, n);
     W.q[0] = carve_e<int>(p, n); W.q[1] = carve_e<int>(p, n);
+    for (int c = 0; c < 2; ++c) {
+        W.rq_org[c] = carve_e<float4>(p, n);
+        W.rq_dir[c] = carve_e<float4>(p, n);
+    }
     for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
     W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
     W.aov0 = carve_e<float4>(p, n); W.aov1 = carve_e<float4>(p, n);
@@ -2257,14 +2305,14 @@ struct ExtPlan {
 constexpr int kMarchBlocksPerCu = 8;
 
 template <int KIND, bool MULTI>
-static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int grid) {
+static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int rq, int grid) {
     if (ctx->cfg.march == 1) {
-        launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
+        launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur, rq);
         return;
     }
     const int bpc = ctx->cfg.march_blocks_per_cu > 0 ? ctx->cfg.march_blocks_per_cu : kMarchBlocksPerCu;
     const int mgrid = std::max(1, std::min(grid, P.cus * bpc));
-    launch(ctx, name, k_ext_march_pt<KIND, MULTI>, dim3(mgrid), dim3(kBlk), P.S0, P.B, Wv, cur);
+    launch(ctx, name, k_ext_march_pt<KIND, MULTI>, dim3(mgrid), dim3(kBlk), P.S0, P.B, Wv, cur, rq);
 }
 
 // One bounce b (extension queue `cur`). Scenes without SDF objects trace the
@@ -2281,27 +2329,30 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
     // faster on the analytic scenes (4096 spheres, Cornell box) and even on SDFs.
     const bool persistent = !MULTI && ctx->cfg.traversal != 2 && !P.quad_bvh;
     const bool fused = !persistent && !P.any_sdf;
+    // bounce 0's queue is the identity over the camera rays, which raygen
+    // writes by path; later queues have their rays in queue order
+    const int rq = b > 0 ? 1 : 0;
     launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1, shadow_cnt(b));
     if (fused) {
         launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur,
-               shadow_cnt(b - 1));
+               shadow_cnt(b - 1), rq);
     } else if (P.any_sdf) {
         if (persistent)
-            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
+            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
         else
             launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
-                   Wv, cur);
+                   Wv, cur, rq);
         for (int kind : P.sdf_kinds) {
             switch (kind) {
-                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, grid); break;
-                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER, MULTI>(ctx, "k_ext_march_menger", P, Wv, cur, grid); break;
-                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB, MULTI>(ctx, "k_ext_march_blob", P, Wv, cur, grid); break;
-                default: launch_march<PT_SDF_TERRAIN, MULTI>(ctx, "k_ext_march_terrain", P, Wv, cur, grid); break;
+                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, rq, grid); break;
+                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER, MULTI>(ctx, "k_ext_march_menger", P, Wv, cur, rq, grid); break;
+                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB, MULTI>(ctx, "k_ext_march_blob", P, Wv, cur, rq, grid); break;
+                default: launch_march<PT_SDF_TERRAIN, MULTI>(ctx, "k_ext_march_terrain", P, Wv, cur, rq, grid); break;
             }
         }
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     } else {
-        launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
+        launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     }
     if (b == 0 && P.aov) launch(ctx, "k_ext_aov", k_ext_aov<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv);
@@ -2328,7 +2379,7 @@ static void launch_tail(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int nb, int 
     const bool persistent = !MULTI && ctx->cfg.traversal != 2 && !P.quad_bvh;
     if (persistent || P.any_sdf || nb <= 0) return;
     launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, nb & 1,
-           shadow_cnt(nb - 1));
+           shadow_cnt(nb - 1), 1);
 }
 
 template <int W, int STACK, typename SE>
