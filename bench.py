@@ -146,7 +146,8 @@ def parse_args():
     ap.add_argument("--bvh-leaf", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--traversal", type=int, default=0, help="0 default (persistent), 1 persistent lanes refilled from the queue, 2 one ray per lane")
-    ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane")
+    ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane, "
+                         "2 persistent with the mandelbulb's estimates split into passes")
     ap.add_argument("--march-blocks", type=int, default=0, help="blocks per CU of the persistent march (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -431,7 +432,10 @@ def serving_measure(args, n_requests: int, speedups, denoise: bool, observe_fifo
                 fam["denoise"] += ms
             else:
                 fam["other"] += ms
-        return {k: round(v, 1) for k, v in fam.items()}
+        out = {k: round(v, 1) for k, v in fam.items()}
+        top = sorted(prof.items(), key=lambda kv: -kv[1][1])[:16]
+        out["by_kernel"] = {name: [n, round(ms, 1)] for name, (n, ms) in top}
+        return out
 
     # 1. service times alone
     service = {}

@@ -94,11 +94,21 @@ struct ConvArgs {
 // (32 TH) tiles instead of once per tap. Pixels per 16-byte vector in LDS are
 // padded to 40 channels so that 32 lanes reading 32 neighbouring pixels spread
 // over the banks.
+//
+// Staging goes through registers: each thread issues all of its 16-byte loads
+// of a chunk back to back (one wait for the lot, not one per load), and the
+// next chunk's loads are issued before the current chunk's MFMAs, so they are
+// in flight while the matrix cores work. The 9 taps are unrolled, which lets
+// the compiler issue the LDS reads of the next MFMAs ahead of the current ones.
+// Register budget: the LDS tiles allow 2 blocks per CU at NC = 2 (73 KB each)
+// and 3 at NC = 1 (50 KB), so the launch bounds ask for that many waves per SIMD.
 template <int NC, int TH>
-__global__ void __launch_bounds__(kThreads) k_conv_t(ConvArgs a) {
+__global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a) {
     constexpr int CK = 32, CKP = CK + 8;  // channels per chunk; LDS pitch per pixel / weight row
     constexpr int TW = 32, IW = TW + 2, IH = TH + 2;
     constexpr int R = TH / 4;             // pixel rows per wave
+    constexpr int NXV = IH * IW * (CK / 8), NWV = 9 * 32 * NC * (CK / 8);  // 16-byte vectors per chunk
+    constexpr int NXT = (NXV + kThreads - 1) / kThreads, NWT = (NWV + kThreads - 1) / kThreads;
     __shared__ __align__(16) u16 s_x[IH * IW * CKP];
     __shared__ __align__(16) u16 s_w[9 * 32 * NC * CKP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -118,34 +128,68 @@ __global__ void __launch_bounds__(kThreads) k_conv_t(ConvArgs a) {
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int j = 0; j < NC; ++j) acc[r][j] = f16v{};
-    for (int c0 = 0; c0 < Cin; c0 += CK) {
-        __syncthreads();  // every wave is done with the previous chunk's tiles
-        for (int i = threadIdx.x; i < IH * IW * (CK / 8); i += kThreads) {
+    float bias[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const int co = co0 + j * 32 + col;
+        bias[j] = a.bias && co < a.Cout ? a.bias[co] : 0.0f;
+    }
+    uint4 xr[NXT], wr[NWT];
+    // this thread's share of chunk c0: input vectors i = threadIdx.x + u * kThreads
+    // (pixel i / 4 of the halo tile, channels 8 (i % 4) ..), weight vectors likewise
+    // (tap, output row, 8 channels)
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int u = 0; u < NXT; ++u) {
+            const int i = threadIdx.x + u * kThreads;
             const int v = i % (CK / 8), pix = i / (CK / 8);
             const int y = y0 + pix / IW - 1, x = x0 + pix % IW - 1;
             const int c = c0 + v * 8;
             uint4 val = make_uint4(0, 0, 0, 0);
-            if (y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
+            if (i < NXV && y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
                 const size_t q = ((size_t)n * a.H + y) * a.W + x;
                 const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
                 val = *reinterpret_cast<const uint4*>(src);
             }
-            *reinterpret_cast<uint4*>(&s_x[pix * CKP + v * 8]) = val;
+            xr[u] = val;
         }
-        for (int i = threadIdx.x; i < 9 * 32 * NC * (CK / 8); i += kThreads) {
+#pragma unroll
+        for (int u = 0; u < NWT; ++u) {
+            const int i = threadIdx.x + u * kThreads;
             const int v = i % (CK / 8), row = (i / (CK / 8)) % (32 * NC), t = i / ((CK / 8) * 32 * NC);
             const int tt = a.flip ? 8 - t : t;
             const int co = co0 + row, c = c0 + v * 8;
             uint4 val = make_uint4(0, 0, 0, 0);
-            if (co < a.Cout && c < Cin) val = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + c);
-            *reinterpret_cast<uint4*>(&s_w[(t * 32 * NC + row) * CKP + v * 8]) = val;
+            if (i < NWV && co < a.Cout && c < Cin)
+                val = *reinterpret_cast<const uint4*>(a.Wt + ((size_t)tt * a.Cout + co) * Cin + c);
+            wr[u] = val;
         }
-        __syncthreads();
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int u = 0; u < NXT; ++u) {
+            const int i = threadIdx.x + u * kThreads;
+            if (i < NXV) *reinterpret_cast<uint4*>(&s_x[(i / (CK / 8)) * CKP + (i % (CK / 8)) * 8]) = xr[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NWT; ++u) {
+            const int i = threadIdx.x + u * kThreads;
+            if (i < NWV) *reinterpret_cast<uint4*>(&s_w[(i / (CK / 8)) * CKP + (i % (CK / 8)) * 8]) = wr[u];
+        }
+    };
+    fetch(0);
 #pragma unroll 1
+    for (int c0 = 0; c0 < Cin; c0 += CK) {
+        __syncthreads();  // every wave is done with the previous chunk's tiles
+        put();
+        __syncthreads();
+        if (c0 + CK < Cin) fetch(c0 + CK);
+#pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int dy = t / 3, dx = t % 3;  // offsets into the halo-padded tile
 #pragma unroll
             for (int k = 0; k < CK; k += 16) {
+                if (c0 + k >= Cin) break;  // a 16-channel input's second step is all zeros
                 Frag A[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r)
@@ -161,26 +205,43 @@ __global__ void __launch_bounds__(kThreads) k_conv_t(ConvArgs a) {
             }
         }
     }
+    // Epilogue through LDS: bias, ReLU and BF16 rounding per accumulator, the
+    // block's TH x 32 pixels x 32 NC channels written to LDS in pixel-major
+    // order, then stored as 16-byte vectors of 8 channels (the tensors' channel
+    // counts and the Y1 / Y2 split are multiples of 8), one bounds check per
+    // vector. The weight tile's LDS holds it (32 NC + 8 channels per pixel).
+    constexpr int OP = 32 * NC + 8;
+    static_assert(TH * TW * OP <= 9 * 32 * NC * CKP, "output tile must fit the weight tile's LDS");
+    u16* s_o = s_w;
+    __syncthreads();  // every wave is done with the last chunk's tiles
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
         const int co = co0 + j * 32 + col;
-        if (co >= a.Cout) continue;
-        const float b = a.bias ? a.bias[co] : 0.0f;
+        const float b = bias[j];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int y = y0 + wave * R + r;
-            if (y >= a.H) continue;
+            const int row = wave * R + r;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int x = x0 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                if (x >= a.W) continue;
-                const size_t p = ((size_t)n * a.H + y) * a.W + x;
+                const int px = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
                 float o = acc[r][j][v] + b;
                 if (a.relu) o = fmaxf(o, 0.0f);
-                if (co < a.Co1) a.Y1[p * a.Co1 + co] = f2bf(o);
-                else a.Y2[p * (a.Cout - a.Co1) + (co - a.Co1)] = f2bf(o);
+                s_o[(row * TW + px) * OP + j * 32 + col] = co < a.Cout ? f2bf(o) : (u16)0;
             }
         }
+    }
+    __syncthreads();
+    constexpr int VPP = 32 * NC / 8;  // 16-byte vectors per pixel
+#pragma unroll
+    for (int u = 0; u < (TH * TW * VPP) / kThreads; ++u) {
+        const int i = threadIdx.x + u * kThreads;
+        const int g = i % VPP, pix = i / VPP;
+        const int y = y0 + pix / TW, x = x0 + pix % TW, c = co0 + g * 8;
+        if (y >= a.H || x >= a.W || c >= a.Cout) continue;
+        const uint4 val = *reinterpret_cast<const uint4*>(&s_o[pix * OP + g * 8]);
+        const size_t p = ((size_t)n * a.H + y) * a.W + x;
+        u16* dst = c < a.Co1 ? a.Y1 + p * a.Co1 + c : a.Y2 + p * (a.Cout - a.Co1) + (c - a.Co1);
+        *reinterpret_cast<uint4*>(dst) = val;
     }
 }
 
@@ -784,9 +845,10 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
         l.b_off = off;
         off += (size_t)l.cout;
         d->L.push_back(l);
-        // k_conv_t reads 8 channels per 16-byte vector and splits the input at
-        // C1: channel counts of both passes (forward, backward-data) are
-        // multiples of 16
+        // k_conv_t reads and writes 8 channels per 16-byte vector, splits the
+        // input at C1 and the output at Co1, and skips the second 16-channel
+        // step of a 16-channel input: channel counts of both passes (forward,
+        // backward-data) are multiples of 16
         const int cin = l.cin1 + l.cin2;
         if (cin % 16 || l.cout % 16 || l.cin1 % 16) {
             delete d;
