@@ -234,6 +234,20 @@ pt_status pt_accum_read_xyz(pt_accum* acc, float* host_rgba, size_t n_floats);
 pt_status pt_accum_write_xyz(pt_accum* acc, const float* host_rgba, size_t n_floats);
 void pt_accum_release(pt_accum* acc);
 
+/* Zero-copy display (replaces the texel buffer the reference's fragment pass
+ * samples in place, src/shader.frag:81-99, bound at src/pathtracer.cpp:2294-2366).
+ * pt_accum_create_exportable makes an accumulator like pt_accum_create whose
+ * device memory can be shared with another API: pt_accum_export_fd returns a new
+ * dma-buf file descriptor of it and the exported size in bytes (W*H*16 rounded
+ * up to the allocation granularity; the texels start at offset 0). The caller
+ * owns the descriptor: Vulkan imports it with VkImportMemoryFdInfoKHR
+ * (VK_EXTERNAL_MEMORY_HANDLE_TYPE_DMA_BUF_BIT_EXT), or close() it. Dispatches
+ * are asynchronous: pt_sync (or a semaphore of the caller's) before the other
+ * API reads. Every other pt_accum_* call works on either kind; an accumulator
+ * from pt_accum_create returns PT_ERR_STATE here. */
+pt_status pt_accum_create_exportable(pt_ctx* ctx, int width, int height, pt_accum** out);
+pt_status pt_accum_export_fd(pt_accum* acc, int* fd, uint64_t* bytes);
+
 /* == vkCmdPushConstants + vkCmdDispatch + vkQueueSubmit: trace
  * pc->samplesPerFrame samples per pixel and fold them into `acc`
  * (src/shader.comp:1509-1533). Asynchronous on the context's stream. */

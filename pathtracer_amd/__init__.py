@@ -218,6 +218,8 @@ SIGNATURES = {
     "pt_accum_read_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "pt_accum_write_xyz": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "pt_accum_release": (None, [_P]),
+    "pt_accum_create_exportable": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(_P)]),
+    "pt_accum_export_fd": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
     "pt_dispatch": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants), C.c_int]),
     "pt_dispatch_noalloc": (C.c_int, [_P, _P, _P, C.POINTER(PushConstants)]),
     "pt_render_features": (C.c_int, [_P, _P, C.POINTER(PushConstants), C.POINTER(C.c_float), C.c_size_t]),
@@ -382,9 +384,12 @@ class Context:
         self.check(lib.pt_scene_from_packed(self.h, C.byref(ubo), arr, len(sdfs), C.byref(h)))
         return Scene(self, h)
 
-    def accum(self, w: int, h: int) -> "Accum":
+    def accum(self, w: int, h: int, exportable: bool = False) -> "Accum":
+        """An accumulator of w x h texels; `exportable` ones can hand their
+        device memory to another API as a dma-buf (Accum.export_fd)."""
         a = C.c_void_p()
-        self.check(lib.pt_accum_create(self.h, w, h, C.byref(a)))
+        create = lib.pt_accum_create_exportable if exportable else lib.pt_accum_create
+        self.check(create(self.h, w, h, C.byref(a)))
         return Accum(self, a, w, h)
 
     def dispatch(self, scene: "Scene", acc: "Accum", pc: PushConstants, mode: int = MODE_REF) -> None:
@@ -517,6 +522,13 @@ class Accum:
     def write(self, texels: np.ndarray) -> None:
         t = np.ascontiguousarray(texels, dtype=np.float32)
         self.ctx.check(lib.pt_accum_write_xyz(self.h, _fptr(t), t.size))
+
+    def export_fd(self) -> tuple:
+        """(file descriptor, bytes): a new dma-buf descriptor of the texels'
+        device memory, owned by the caller (os.close it when done)."""
+        fd, n = C.c_int(-1), C.c_uint64(0)
+        self.ctx.check(lib.pt_accum_export_fd(self.h, C.byref(fd), C.byref(n)))
+        return fd.value, n.value
 
     def resolve_srgb8(self, tonemap: int = 3) -> np.ndarray:
         out = np.empty((self.hgt, self.w, 3), dtype=np.uint8)

@@ -346,15 +346,107 @@ pt_status pt_accum_write_xyz(pt_accum* a, const float* host, size_t n) {
     return PT_OK;
 }
 
+namespace {
+// Unmaps and frees an exportable accumulator's memory (each step only once the
+// previous ones have been made).
+void vmm_free(pt_accum* a, bool mapped, bool reserved, bool created) {
+    if (mapped) (void)hipMemUnmap(a->d_texels, a->vmm_bytes);
+    if (reserved) (void)hipMemAddressFree(a->d_texels, a->vmm_bytes);
+    if (created) (void)hipMemRelease(a->vmm_handle);
+}
+}  // namespace
+
 void pt_accum_release(pt_accum* a) {
     if (!a) return;
     (void)hipSetDevice(a->ctx->device);
     (void)hipStreamSynchronize(a->ctx->stream);
-    DevAlloc m;
-    m.p = a->d_texels;
-    m.bytes = (size_t)a->w * a->h * 16;
-    ptk::dev_free(a->ctx, m);
+    if (a->vmm) {
+        vmm_free(a, true, true, true);
+        a->ctx->allocated -= (int64_t)a->vmm_bytes;
+    } else {
+        DevAlloc m;
+        m.p = a->d_texels;
+        m.bytes = (size_t)a->w * a->h * 16;
+        ptk::dev_free(a->ctx, m);
+    }
     delete a;
+}
+
+pt_status pt_accum_create_exportable(pt_ctx* ctx, int width, int height, pt_accum** out) {
+    if (!ctx || !out) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create_exportable: NULL argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 28)
+        return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create_exportable: resolution out of range");
+    (void)hipSetDevice(ctx->device);
+    hipMemAllocationProp prop{};
+    prop.type = hipMemAllocationTypePinned;
+    prop.requestedHandleType = hipMemHandleTypePosixFileDescriptor;
+    prop.location.type = hipMemLocationTypeDevice;
+    prop.location.id = ctx->device;
+    size_t gran = 0;
+    hipError_t e = hipMemGetAllocationGranularity(&gran, &prop, hipMemAllocationGranularityMinimum);
+    if (e != hipSuccess) return ptk::hip_fail(ctx, e, "pt_accum_create_exportable: hipMemGetAllocationGranularity");
+    if (gran == 0) gran = 1;
+    const size_t bytes = (size_t)width * height * 16;
+    const size_t sz = (bytes + gran - 1) / gran * gran;
+    if (ctx->cfg.hbm_budget > 0 && ctx->allocated + (int64_t)sz > ctx->cfg.hbm_budget)
+        return set_err(ctx, PT_ERR_BUDGET, "HBM budget exceeded: " + std::to_string(ctx->allocated + (int64_t)sz) +
+                                               " > " + std::to_string(ctx->cfg.hbm_budget) + " bytes");
+    pt_accum* a = new (std::nothrow) pt_accum();
+    if (!a) return set_err(ctx, PT_ERR_INVALID_ARG, "pt_accum_create_exportable: out of host memory");
+    a->ctx = ctx;
+    a->w = width;
+    a->h = height;
+    a->vmm = true;
+    a->vmm_bytes = sz;
+    e = hipMemCreate(&a->vmm_handle, sz, &prop, 0);
+    if (e != hipSuccess) {
+        delete a;
+        return ptk::hip_fail(ctx, e, "pt_accum_create_exportable: hipMemCreate");
+    }
+    void* ptr = nullptr;
+    e = hipMemAddressReserve(&ptr, sz, gran, nullptr, 0);
+    if (e != hipSuccess) {
+        vmm_free(a, false, false, true);
+        delete a;
+        return ptk::hip_fail(ctx, e, "pt_accum_create_exportable: hipMemAddressReserve");
+    }
+    a->d_texels = static_cast<float*>(ptr);
+    e = hipMemMap(ptr, sz, 0, a->vmm_handle, 0);
+    if (e != hipSuccess) {
+        vmm_free(a, false, true, true);
+        delete a;
+        return ptk::hip_fail(ctx, e, "pt_accum_create_exportable: hipMemMap");
+    }
+    hipMemAccessDesc access{};
+    access.location = prop.location;
+    access.flags = hipMemAccessFlagsProtReadWrite;
+    e = hipMemSetAccess(ptr, sz, &access, 1);
+    if (e == hipSuccess) e = hipMemsetAsync(ptr, 0, sz, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        vmm_free(a, true, true, true);
+        delete a;
+        return ptk::hip_fail(ctx, e, "pt_accum_create_exportable");
+    }
+    ctx->allocated += (int64_t)sz;
+    *out = a;
+    return PT_OK;
+}
+
+pt_status pt_accum_export_fd(pt_accum* a, int* fd, uint64_t* bytes) {
+    if (!a || !fd || !bytes) return set_global(PT_ERR_INVALID_ARG, "pt_accum_export_fd: NULL argument");
+    *fd = -1;
+    *bytes = 0;
+    if (!a->vmm)
+        return set_err(a->ctx, PT_ERR_STATE, "pt_accum_export_fd: the accumulator was not made by pt_accum_create_exportable");
+    (void)hipSetDevice(a->ctx->device);
+    int f = -1;
+    hipError_t e = hipMemExportToShareableHandle(&f, a->vmm_handle, hipMemHandleTypePosixFileDescriptor, 0);
+    if (e != hipSuccess) return ptk::hip_fail(a->ctx, e, "pt_accum_export_fd: hipMemExportToShareableHandle");
+    *fd = f;
+    *bytes = a->vmm_bytes;
+    return PT_OK;
 }
 
 pt_status pt_dispatch(pt_ctx* ctx, const pt_scene* s, pt_accum* acc, const pt_push_constants* pc, pt_mode mode) {

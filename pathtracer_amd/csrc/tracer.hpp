@@ -153,6 +153,11 @@ struct pt_accum {
     pt_ctx* ctx = nullptr;
     int w = 0, h = 0;
     float* d_texels = nullptr;
+    // pt_accum_create_exportable: the texels are a virtual-memory mapping of a
+    // physical allocation (vmm_handle, vmm_bytes long) that can be exported
+    bool vmm = false;
+    hipMemGenericAllocationHandle_t vmm_handle{};
+    size_t vmm_bytes = 0;
 };
 
 namespace ptk {
