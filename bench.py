@@ -146,8 +146,7 @@ def parse_args():
     ap.add_argument("--bvh-leaf", type=int, default=0)
     ap.add_argument("--wave-paths", type=int, default=0)
     ap.add_argument("--traversal", type=int, default=0, help="0 default (persistent), 1 persistent lanes refilled from the queue, 2 one ray per lane")
-    ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane, "
-                         "2 persistent with the mandelbulb's estimates split into passes")
+    ap.add_argument("--march", type=int, default=0, help="SDF march: 0 persistent lanes refilled from the queue, 1 one ray per lane")
     ap.add_argument("--march-blocks", type=int, default=0, help="blocks per CU of the persistent march (0 = default)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -497,28 +497,38 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
 // SDF distance estimators (extended mode: the shapes of sdfs/*.glsl in plain
 // FP32 arithmetic).
 // `it` counts the iterations executed (the loop leaves early past the bailout).
+// One iteration of the mandelbulb's power-8 map, z <- z^8 + p in spherical form
+// (the angles' multiples by Chebyshev polynomials of their cosines), with the
+// running derivative dr <- 8 r^7 dr + 1; r becomes |z|.
+// Its multiply-adds are fused (written out, as this file is built without
+// contraction): the march is bound by VALU issue.
+__device__ __forceinline__ void mandelbulb_iter(f3& z, float& r, float& dr, f3 p) {
+    float r2 = r * r, r4 = r2 * r2;
+    dr = fmaf(8.0f * (r4 * r2 * r), dr, 1.0f);
+    float invr = hw_rcp(r);
+    float invl = hw_rsq(fmaf(z.x, z.x, z.z * z.z));
+    float r8 = r4 * r4;
+    float cost = z.z * invl, cosp = z.y * invr, sint = z.x * invl, sinp = invr * hw_rcp(invl);
+    float ct2 = cost * cost, cp2 = cosp * cosp;
+    float cost8 = fmaf(fmaf(fmaf(fmaf(128.0f, ct2, -256.0f), ct2, 160.0f), ct2, -32.0f), ct2, 1.0f);
+    float cosp8 = fmaf(fmaf(fmaf(fmaf(128.0f, cp2, -256.0f), cp2, 160.0f), cp2, -32.0f), cp2, 1.0f);
+    float sint8 = fmaf(fmaf(fmaf(128.0f, ct2, -192.0f), ct2, 80.0f), ct2, -8.0f) * sint * cost;
+    float sinp8 = fmaf(fmaf(fmaf(128.0f, cp2, -192.0f), cp2, 80.0f), cp2, -8.0f) * sinp * cosp;
+    z = mk3(fmaf(r8, sint8 * sinp8, p.x), fmaf(r8, cosp8, p.y), fmaf(r8, cost8 * sinp8, p.z));
+    r = hw_sqrt(fdot(z, z));
+}
+// The distance estimate from the map's last iterate.
+__device__ __forceinline__ float mandelbulb_dist(float r, float dr) { return 0.5f * __logf(r) * r * hw_rcp(dr); }
 __device__ __forceinline__ float ext_mandelbulb(f3 p, int iters, float bailout, unsigned& it) {
     f3 z = p;
-    float r = hw_len(z);
+    float r = hw_sqrt(fdot(z, z));
     float dr = 1.0f;
     for (int i = 0; i < iters; ++i) {
         ++it;
-        float r2 = r * r, r4 = r2 * r2;
-        dr = 8.0f * (r4 * r2 * r) * dr + 1.0f;
-        float invr = hw_rcp(r);
-        float invl = hw_rsq(z.x * z.x + z.z * z.z);
-        float r8 = r4 * r4;
-        float cost = z.z * invl, cosp = z.y * invr, sint = z.x * invl, sinp = invr * hw_rcp(invl);
-        float ct2 = cost * cost, cp2 = cosp * cosp;
-        float cost8 = fmaf(fmaf(fmaf(fmaf(128.0f, ct2, -256.0f), ct2, 160.0f), ct2, -32.0f), ct2, 1.0f);
-        float cosp8 = fmaf(fmaf(fmaf(fmaf(128.0f, cp2, -256.0f), cp2, 160.0f), cp2, -32.0f), cp2, 1.0f);
-        float sint8 = fmaf(fmaf(fmaf(128.0f, ct2, -192.0f), ct2, 80.0f), ct2, -8.0f) * sint * cost;
-        float sinp8 = fmaf(fmaf(fmaf(128.0f, cp2, -192.0f), cp2, 80.0f), cp2, -8.0f) * sinp * cosp;
-        z = r8 * mk3(sint8 * sinp8, cosp8, cost8 * sinp8) + p;
-        r = hw_len(z);
+        mandelbulb_iter(z, r, dr, p);
         if (r > bailout) break;
     }
-    return 0.5f * __logf(r) * r * hw_rcp(dr);
+    return mandelbulb_dist(r, dr);
 }
 __device__ __forceinline__ float ext_menger(f3 p, int levels) {
     f3 q = abs3(p) - mk3(1.0f, 1.0f, 1.0f);
@@ -526,10 +536,10 @@ __device__ __forceinline__ float ext_menger(f3 p, int levels) {
     float s = 1.0f;
     for (int m = 0; m < levels; ++m) {
         f3 ps = p * s;
-        f3 a = mk3(ps.x - 2.0f * floorf(ps.x * 0.5f) - 1.0f, ps.y - 2.0f * floorf(ps.y * 0.5f) - 1.0f,
-                   ps.z - 2.0f * floorf(ps.z * 0.5f) - 1.0f);
+        f3 a = mk3(fmaf(-2.0f, floorf(ps.x * 0.5f), ps.x) - 1.0f, fmaf(-2.0f, floorf(ps.y * 0.5f), ps.y) - 1.0f,
+                   fmaf(-2.0f, floorf(ps.z * 0.5f), ps.z) - 1.0f);
         s *= 3.0f;
-        f3 r = abs3(mk3(1.0f - 3.0f * fabsf(a.x), 1.0f - 3.0f * fabsf(a.y), 1.0f - 3.0f * fabsf(a.z)));
+        f3 r = abs3(mk3(fmaf(-3.0f, fabsf(a.x), 1.0f), fmaf(-3.0f, fabsf(a.y), 1.0f), fmaf(-3.0f, fabsf(a.z), 1.0f)));
         float da = fmaxf(r.x, r.y), db = fmaxf(r.y, r.z), dc = fmaxf(r.z, r.x);
         d = fmaxf(d, (fminf(da, fminf(db, dc)) - 1.0f) * hw_rcp(s));
     }
@@ -605,6 +615,11 @@ __device__ __forceinline__ uint32_t sdf_mask(const ExtSceneDev& S, f3 o, f3 d, f
     return m;
 }
 
+// The point at distance t along a ray (fused multiply-adds).
+__device__ __forceinline__ f3 ray_at(f3 o, f3 d, float t) {
+    return mk3(fmaf(d.x, t, o.x), fmaf(d.y, t, o.y), fmaf(d.z, t, o.z));
+}
+
 // Sphere tracing of one SDF object inside its box, up to tmax. Returns the hit
 // distance or a negative value. Adds its steps to `steps` and their FP32
 // operations (de_step_flops, de_iter_flops) to `flops`.
@@ -617,7 +632,7 @@ __device__ float ext_march(const ExtSdf& s, f3 o, f3 d, float tmax, int max_step
     unsigned n = 0, it = 0;
     bool open = true;  // neither a hit nor past the box's exit yet
     while (open && n < (unsigned)max_steps) {
-        float dist = ext_de<KIND>(s, o + d * t, it);
+        float dist = ext_de<KIND>(s, ray_at(o, d, t), it);
         ++n;
         if (dist < 1e-4f) {
             res = t;
@@ -1194,7 +1209,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
         if (!chunk.drained() && __popcll(~busy) >= kMarchRefill) continue;
         if (p >= 0) {
             if (nst < smax) {
-                const float dist = ext_de_at(KIND, sc, siters, sbail, o + d * tm, it);
+                const float dist = ext_de_at(KIND, sc, siters, sbail, ray_at(o, d, tm), it);
                 ++nst;
                 ++steps;
                 flops += sflops;
