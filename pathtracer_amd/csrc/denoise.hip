@@ -62,6 +62,11 @@ union Frag {
     uint4 u;
     bf8 v;
 };
+// A compile-time range of 32-channel output tiles [J0, J0 + N) (k_conv_t's epilogue).
+template <int J0, int N>
+struct SliceRange {
+    static constexpr int j0 = J0, count = N;
+};
 
 // ---------------------------------------------------------------------------------
 // Conv forward (also backward-data). X1 (C1 channels) and X2 (C2 channels, may
@@ -81,28 +86,36 @@ struct ConvArgs {
     int N, H, W;
     int relu;
     int flip;
+    int tiles_per_block;  // consecutive pixel tiles a block computes in turn
 };
 
 // ---------------------------------------------------------------------------------
-// Conv forward (also backward-data), tiled: a block computes TH rows x 32
-// columns of one image's pixels x 32 * NC output channels. Per chunk of 32
+// Conv forward (also backward-data), tiled: a block computes tiles of TH rows x
+// 32 columns of one image's pixels x 32 * NC output channels. Per chunk of 32
 // input channels it stages in LDS the (TH + 2) x 34 input tile around its
-// pixels (zero outside the image) and the chunk's weights for all 9 taps;
-// each wave then runs its TH / 4 pixel rows x NC channel tiles
-// of 32 x 32 MFMAs over 9 taps x 2 steps of 16 channels with both operands
-// read from LDS. Every input pixel is read from memory by about (TH + 2)(34) /
-// (32 TH) tiles instead of once per tap. Pixels per 16-byte vector in LDS are
-// padded to 40 channels so that 32 lanes reading 32 neighbouring pixels spread
-// over the banks.
+// pixels (zero outside the image) and the chunk's weights for all 9 taps; each
+// wave then runs its TH / 4 pixel rows x NC channel tiles of 32 x 32 MFMAs over
+// 9 taps x 2 steps of 16 channels with both operands read from LDS. Every input
+// pixel is read from memory by about (TH + 2)(34) / (32 TH) tiles instead of
+// once per tap. Pixels per 16-byte vector in LDS are padded to 40 channels so
+// that 32 lanes reading 32 neighbouring pixels spread over the banks.
 //
 // Staging goes through registers: each thread issues all of its 16-byte loads
-// of a chunk back to back (one wait for the lot, not one per load), and the
-// next chunk's loads are issued before the current chunk's MFMAs, so they are
-// in flight while the matrix cores work. The 9 taps are unrolled, which lets
-// the compiler issue the LDS reads of the next MFMAs ahead of the current ones.
+// of a chunk back to back (one wait for the lot), and the next chunk's loads
+// are issued before the current chunk's MFMAs, so they are in flight while the
+// matrix cores work. The 9 taps are unrolled, which lets the compiler issue the
+// LDS reads of the next MFMAs ahead of the current ones.
+//  - Several input chunks: a block computes one tile; its output goes through
+//    the weight tile's LDS to 16-byte stores.
+//  - One input chunk (ONE_CHUNK, Cin <= 32: the full-resolution layers whose
+//    load, MFMAs and store would otherwise follow one another): a block
+//    computes `tiles_per_block` consecutive tiles (along x first) with the
+//    weights staged once; the next tile's input loads are in flight under the
+//    current tile's MFMAs and output stores, and the output goes through the
+//    input tile's LDS, 32 channels at a time.
 // Register budget: the LDS tiles allow 2 blocks per CU at NC = 2 (73 KB each)
 // and 3 at NC = 1 (50 KB), so the launch bounds ask for that many waves per SIMD.
-template <int NC, int TH>
+template <int NC, int TH, bool ONE_CHUNK>
 __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a) {
     constexpr int CK = 32, CKP = CK + 8;  // channels per chunk; LDS pitch per pixel / weight row
     constexpr int TW = 32, IW = TW + 2, IH = TH + 2;
@@ -113,12 +126,6 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
     __shared__ __align__(16) u16 s_w[9 * 32 * NC * CKP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tiles_x = (a.W + TW - 1) / TW, tiles_y = (a.H + TH - 1) / TH;
-    int blk = blockIdx.x;
-    const int tx = blk % tiles_x;
-    blk /= tiles_x;
-    const int ty = blk % tiles_y;
-    const int n = blk / tiles_y;
-    const int x0 = tx * TW, y0 = ty * TH;
     const int co0 = blockIdx.y * 32 * NC;
     const int Cin = a.C1 + a.C2;
     const int kq = (lane >> 5) * 8;  // this lane's 8 channels within a 16-channel step
@@ -134,11 +141,18 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
         const int co = co0 + j * 32 + col;
         bias[j] = a.bias && co < a.Cout ? a.bias[co] : 0.0f;
     }
+    // tile -> image n and the tile's first pixel (x0, y0)
+    auto origin = [&](int tile, int& n, int& x0, int& y0) {
+        x0 = (tile % tiles_x) * TW;
+        const int rest = tile / tiles_x;
+        y0 = (rest % tiles_y) * TH;
+        n = rest / tiles_y;
+    };
     uint4 xr[NXT], wr[NWT];
-    // this thread's share of chunk c0: input vectors i = threadIdx.x + u * kThreads
+    // this thread's share of a chunk: input vectors i = threadIdx.x + u * kThreads
     // (pixel i / 4 of the halo tile, channels 8 (i % 4) ..), weight vectors likewise
     // (tap, output row, 8 channels)
-    auto fetch = [&](int c0) {
+    auto fetch_x = [&](int n, int x0, int y0, int c0) {
 #pragma unroll
         for (int u = 0; u < NXT; ++u) {
             const int i = threadIdx.x + u * kThreads;
@@ -153,6 +167,8 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
             }
             xr[u] = val;
         }
+    };
+    auto fetch_w = [&](int c0) {
 #pragma unroll
         for (int u = 0; u < NWT; ++u) {
             const int i = threadIdx.x + u * kThreads;
@@ -165,25 +181,22 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
             wr[u] = val;
         }
     };
-    auto put = [&]() {
+    auto put_x = [&]() {
 #pragma unroll
         for (int u = 0; u < NXT; ++u) {
             const int i = threadIdx.x + u * kThreads;
             if (i < NXV) *reinterpret_cast<uint4*>(&s_x[(i / (CK / 8)) * CKP + (i % (CK / 8)) * 8]) = xr[u];
         }
+    };
+    auto put_w = [&]() {
 #pragma unroll
         for (int u = 0; u < NWT; ++u) {
             const int i = threadIdx.x + u * kThreads;
             if (i < NWV) *reinterpret_cast<uint4*>(&s_w[(i / (CK / 8)) * CKP + (i % (CK / 8)) * 8]) = wr[u];
         }
     };
-    fetch(0);
-#pragma unroll 1
-    for (int c0 = 0; c0 < Cin; c0 += CK) {
-        __syncthreads();  // every wave is done with the previous chunk's tiles
-        put();
-        __syncthreads();
-        if (c0 + CK < Cin) fetch(c0 + CK);
+    // the MFMAs of the chunk at c0 staged in LDS
+    auto mma = [&](int c0) {
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int dy = t / 3, dx = t % 3;  // offsets into the halo-padded tile
@@ -204,44 +217,100 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
                 }
             }
         }
-    }
-    // Epilogue through LDS: bias, ReLU and BF16 rounding per accumulator, the
-    // block's TH x 32 pixels x 32 NC channels written to LDS in pixel-major
-    // order, then stored as 16-byte vectors of 8 channels (the tensors' channel
-    // counts and the Y1 / Y2 split are multiples of 8), one bounds check per
-    // vector. The weight tile's LDS holds it (32 NC + 8 channels per pixel).
-    constexpr int OP = 32 * NC + 8;
-    static_assert(TH * TW * OP <= 9 * 32 * NC * CKP, "output tile must fit the weight tile's LDS");
-    u16* s_o = s_w;
-    __syncthreads();  // every wave is done with the last chunk's tiles
+    };
+    // Output channels [j0, j0 + 32 J) of the tile at (n, x0, y0) staged in s_o
+    // (pitch OP per pixel): bias, ReLU and BF16 rounding per accumulator into LDS
+    // in pixel-major order, then 16-byte vectors of 8 channels to memory (the
+    // tensors' channel counts and the Y1 / Y2 split are multiples of 8), one
+    // bounds check per vector. The caller synchronises before (LDS free) and
+    // after (LDS read) it.
+    auto store_slices = [&](u16* s_o, auto jc, int n, int x0, int y0) {
+        constexpr int J0 = decltype(jc)::j0, J = decltype(jc)::count, OP = 32 * J + 8;
 #pragma unroll
-    for (int j = 0; j < NC; ++j) {
-        const int co = co0 + j * 32 + col;
-        const float b = bias[j];
+        for (int jj = 0; jj < J; ++jj) {
+            const int j = J0 + jj;
+            const int co = co0 + j * 32 + col;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int row = wave * R + r;
+            for (int r = 0; r < R; ++r) {
+                const int row = wave * R + r;
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int px = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                float o = acc[r][j][v] + b;
-                if (a.relu) o = fmaxf(o, 0.0f);
-                s_o[(row * TW + px) * OP + j * 32 + col] = co < a.Cout ? f2bf(o) : (u16)0;
+                for (int v = 0; v < 16; ++v) {
+                    const int px = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                    float o = acc[r][j][v] + bias[j];
+                    if (a.relu) o = fmaxf(o, 0.0f);
+                    s_o[(row * TW + px) * OP + jj * 32 + col] = co < a.Cout ? f2bf(o) : (u16)0;
+                }
             }
         }
-    }
-    __syncthreads();
-    constexpr int VPP = 32 * NC / 8;  // 16-byte vectors per pixel
+        __syncthreads();
+        constexpr int VPP = 32 * J / 8;  // 16-byte vectors per pixel
 #pragma unroll
-    for (int u = 0; u < (TH * TW * VPP) / kThreads; ++u) {
-        const int i = threadIdx.x + u * kThreads;
-        const int g = i % VPP, pix = i / VPP;
-        const int y = y0 + pix / TW, x = x0 + pix % TW, c = co0 + g * 8;
-        if (y >= a.H || x >= a.W || c >= a.Cout) continue;
-        const uint4 val = *reinterpret_cast<const uint4*>(&s_o[pix * OP + g * 8]);
-        const size_t p = ((size_t)n * a.H + y) * a.W + x;
-        u16* dst = c < a.Co1 ? a.Y1 + p * a.Co1 + c : a.Y2 + p * (a.Cout - a.Co1) + (c - a.Co1);
-        *reinterpret_cast<uint4*>(dst) = val;
+        for (int u = 0; u < (TH * TW * VPP) / kThreads; ++u) {
+            const int i = threadIdx.x + u * kThreads;
+            const int g = i % VPP, pix = i / VPP;
+            const int y = y0 + pix / TW, x = x0 + pix % TW, c = co0 + J0 * 32 + g * 8;
+            if (y >= a.H || x >= a.W || c >= a.Cout) continue;
+            const uint4 val = *reinterpret_cast<const uint4*>(&s_o[pix * OP + g * 8]);
+            const size_t p = ((size_t)n * a.H + y) * a.W + x;
+            u16* dst = c < a.Co1 ? a.Y1 + p * a.Co1 + c : a.Y2 + p * (a.Cout - a.Co1) + (c - a.Co1);
+            *reinterpret_cast<uint4*>(dst) = val;
+        }
+    };
+    if constexpr (!ONE_CHUNK) {
+        int n, x0, y0;
+        origin(blockIdx.x, n, x0, y0);
+        fetch_x(n, x0, y0, 0);
+        fetch_w(0);
+#pragma unroll 1
+        for (int c0 = 0; c0 < Cin; c0 += CK) {
+            __syncthreads();  // every wave is done with the previous chunk's tiles
+            put_x();
+            put_w();
+            __syncthreads();
+            if (c0 + CK < Cin) {
+                fetch_x(n, x0, y0, c0 + CK);
+                fetch_w(c0 + CK);
+            }
+            mma(c0);
+        }
+        static_assert(TH * TW * (32 * NC + 8) <= 9 * 32 * NC * CKP, "the output tile must fit the weight tile's LDS");
+        __syncthreads();  // every wave is done with the last chunk's tiles
+        store_slices(s_w, SliceRange<0, NC>{}, n, x0, y0);
+    } else {
+        static_assert(TH * TW * (32 + 8) <= IH * IW * CKP, "an output slice must fit the input tile's LDS");
+        const int ntiles = a.N * tiles_x * tiles_y;
+        const int t_begin = blockIdx.x * a.tiles_per_block;
+        const int t_count = min(a.tiles_per_block, ntiles - t_begin);
+        int n, x0, y0;
+        origin(t_begin, n, x0, y0);
+        fetch_x(n, x0, y0, 0);
+        fetch_w(0);
+#pragma unroll 1
+        for (int k = 0; k < t_count; ++k) {
+            __syncthreads();  // every wave is done with the previous tile's output
+            put_x();
+            if (k == 0) put_w();
+            __syncthreads();
+            int n1 = n, x1 = x0, y1 = y0;
+            if (k + 1 < t_count) {
+                origin(t_begin + k + 1, n1, x1, y1);
+                fetch_x(n1, x1, y1, 0);
+            }
+            mma(0);
+            __syncthreads();  // every wave is done with the input tile
+            store_slices(s_x, SliceRange<0, 1>{}, n, x0, y0);
+            if constexpr (NC == 2) {
+                __syncthreads();
+                store_slices(s_x, SliceRange<1, 1>{}, n, x0, y0);
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) acc[r][j] = f16v{};
+            n = n1;
+            x0 = x1;
+            y0 = y1;
+        }
     }
 }
 
@@ -570,6 +639,7 @@ using namespace ptdn;
 
 struct pt_denoiser {
     pt_ctx* ctx = nullptr;
+    int cus = 256;  // compute units of the context's device
     int maxN = 0, maxH = 0, maxW = 0;
     std::vector<Layer> L;
     size_t nparam = 0;
@@ -611,12 +681,25 @@ void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, 
         a.Y1 = y; a.Y2 = y2; a.Co1 = co1 ? co1 : l.cin1 + l.cin2; a.Cout = l.cin1 + l.cin2;
         a.relu = 0; a.flip = 1;
     }
-    // tiles of 8 x 32 pixels x 64 output channels (32 below 64 channels)
+    // tiles of 8 x 32 pixels x 64 output channels (32 below 64 channels); each
+    // block takes consecutive tiles so that the grid is about two rounds of the
+    // blocks the CUs hold at once (2 per CU at 64 channels, 3 at 32), at most 16
     constexpr int kTH = 8;
     const int nc = a.Cout >= 64 ? 2 : 1;
-    dim3 grid((unsigned)(N * ((H + kTH - 1) / kTH) * ((W + 31) / 32)), (a.Cout + 32 * nc - 1) / (32 * nc));
-    if (nc == 2) ptk::launch(d->ctx, "k_dn_conv", k_conv_t<2, kTH>, grid, dim3(kThreads), a);
-    else ptk::launch(d->ctx, "k_dn_conv", k_conv_t<1, kTH>, grid, dim3(kThreads), a);
+    const int ntiles = N * ((H + kTH - 1) / kTH) * ((W + 31) / 32);
+    const int cotiles = (a.Cout + 32 * nc - 1) / (32 * nc);
+    const int64_t resident = (int64_t)d->cus * (nc == 2 ? 2 : 3);
+    const int64_t want = ((int64_t)ntiles * cotiles + 2 * resident - 1) / (2 * resident);
+    const bool one = a.C1 + a.C2 <= 32;  // one input chunk: the weights stay in LDS for all the block's tiles
+    a.tiles_per_block = one ? (int)std::max<int64_t>(1, std::min<int64_t>(16, want)) : 1;
+    dim3 grid((unsigned)((ntiles + a.tiles_per_block - 1) / a.tiles_per_block), cotiles);
+    if (nc == 2) {
+        if (one) ptk::launch(d->ctx, "k_dn_conv", k_conv_t<2, kTH, true>, grid, dim3(kThreads), a);
+        else ptk::launch(d->ctx, "k_dn_conv", k_conv_t<2, kTH, false>, grid, dim3(kThreads), a);
+    } else {
+        if (one) ptk::launch(d->ctx, "k_dn_conv", k_conv_t<1, kTH, true>, grid, dim3(kThreads), a);
+        else ptk::launch(d->ctx, "k_dn_conv", k_conv_t<1, kTH, false>, grid, dim3(kThreads), a);
+    }
 }
 
 void wgrad(pt_denoiser* d, const Layer& l, const u16* dz, const u16* x1, const u16* x2, int N, int H, int W) {
@@ -830,6 +913,11 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     pt_denoiser* d = new (std::nothrow) pt_denoiser();
     if (!d) return PT_ERR_INVALID_ARG;
     d->ctx = ctx;
+    {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cu > 0)
+            d->cus = cu;
+    }
     d->maxN = max_batch;
     d->maxH = max_height;
     d->maxW = max_width;

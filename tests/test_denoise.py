@@ -87,6 +87,27 @@ def test_forward_matches_the_oracle(gpu_ctx):
     assert err.mean() < 0.01 and np.quantile(err, 0.999) < 0.05
 
 
+@pytest.mark.gpu
+def test_block_tiling_is_invisible(gpu_ctx):
+    """A conv block takes several consecutive pixel tiles when the image is large
+    (about two rounds of the blocks the CUs hold at once) and one tile when it
+    is small. The interior of a 1024x1024 image (3 tiles per block in the
+    full-resolution layers, a last block with fewer) and the interior of a
+    256x256 crop of it, farther from the crop's edge than the network's
+    receptive field (about 30 pixels), are bit-identical."""
+    import pathtracer_amd as pt
+    rng = np.random.default_rng(7)
+    p = random_params(rng).astype(np.float32)
+    feat = random_features(rng, 1, 1024, 1024)
+    y0, x0, m = 384, 256, 64  # crop origin (multiples of 4: the same pooling grid) and margin
+    with pt.Denoiser(gpu_ctx, 1, 1024, 1024, seed=3) as dn:
+        dn.set_params(p)
+        full = dn.denoise(feat)
+        crop = dn.denoise(feat[:, y0:y0 + 256, x0:x0 + 256])
+    assert np.isfinite(full).all()
+    assert np.array_equal(crop[:, m:-m, m:-m], full[:, y0 + m:y0 + 256 - m, x0 + m:x0 + 256 - m])
+
+
 def grad_agreement(g_gpu, g_ref):
     rows = []
     for name, sl in D.layer_slices():
