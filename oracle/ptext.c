@@ -11,13 +11,17 @@
  *
  * Physics (the extended mode's definition):
  *  - spheres, parallelograms (boxes arrive as six), horizontal planes, SDFs
- *    (mandelbulb, menger, blob, terrain) by sphere tracing;
+ *    (mandelbulb, menger, blob, terrain) by sphere tracing, the reference's
+ *    lenses (two spherical caps, src/shader.comp:366-448) and Dupin cyclides
+ *    (the quartic of src/shader.comp:633-679, its real roots isolated by the
+ *    roots of its derivatives and bisection, not by a closed-form solver);
  *  - Lambert with the reference's one-lobe spectrum (src/shader.comp:1030-1038)
  *    or a clamped sum of Gaussians; smooth dielectric (Cauchy A + B / lambda^2,
  *    lambda in um, or Sellmeier) with exact unpolarised Fresnel; smooth
  *    conductor with the complex-index Fresnel reflectance;
  *  - emitters: Planck's law normalised at its Wien peak times luminosity
  *    (src/shader.comp:1040-1055), two-sided; uniform environment radiance;
+ *    emissive lenses and cyclides are found by BSDF sampling only (weight 1);
  *  - hero wavelength uniform in [lmin, lmin + range) with n - 1 companions
  *    rotated by range / n; a dispersive refraction keeps the hero only (its
  *    throughput times n);
@@ -83,9 +87,17 @@ typedef struct {
     double ap_size, lens_dist, lens_radius, lens_focal, lens_thick;
     int nee; /* 1: NEE with MIS at diffuse vertices; 0: BSDF sampling only */
     uint64_t lens_missed; /* camera rays that missed a lens face (written) */
+    int nlens, ncyc;
+    /* lens: pos(3) rotation(3, degrees) radius focalLength thickness isConverging mat light */
+    const double* lens;
+    /* cyclide: pos(3) rotation(3) scale(3) a b c d boundingRadius (squared) mat light */
+    const double* cyc;
+    int nlight_nee; /* lights [0, nlight_nee) are sampled by NEE (spheres, parallelograms) */
 } Scene;
 
-enum { T_SPHERE = 0, T_QUAD = 1, T_PLANE = 2, T_SDF = 3 };
+enum { T_SPHERE = 0, T_QUAD = 1, T_PLANE = 2, T_SDF = 3, T_LENS = 4, T_CYC = 5 };
+#define LENS_REC 12
+#define CYC_REC 16
 enum { M_LOBE = 0, M_GAUSS = 1, M_DIEL = 2, M_COND = 3 };
 
 /* ---- random numbers: pcg_hash (Jarzynski & Olano 2020) keyed per path ---------- */
@@ -218,6 +230,119 @@ static int hit_quad(const double* q, d3 o, d3 d, double tmax, double* t) {
     *t = tt;
     return 1;
 }
+static d3 rot_row(d3 a, d3 v);
+static d3 rot_col(d3 a, d3 v);
+
+/* A lens (src/shader.comp:366-448): in its frame (x on the optical axis, local =
+ * v * RotationMatrix), two spheres of radius R = 2f with centres on the axis at
+ * +-(R - slicePos) and cap height h = R - sqrt(R^2 - r^2); cap 0 is the part of
+ * the first sphere with x - centre <= -(R - h), cap 1 the part of the second
+ * with x - centre >= R - h. slicePos = thickness / 2 + h for a converging lens
+ * (the solid inside both spheres), -thickness / 2 for a diverging one (outside
+ * them). The nearest cap point beyond EPS; *n is the solid's outward normal. */
+static int hit_lens(const double* q, d3 o, d3 d, double tmax, double* t, d3* n) {
+    d3 pos = D3(q[0], q[1], q[2]), ang = D3(q[3], q[4], q[5]);
+    double r = q[6], f = q[7], th = q[8];
+    int conv = q[9] != 0.0;
+    double R = 2.0 * f, h = R - sqrt(R * R - r * r), sp = conv ? 0.5 * th + h : -0.5 * th;
+    d3 lo = rot_row(ang, sub(o, pos)), ld = rot_row(ang, d);
+    double best = tmax;
+    int hit = 0;
+    for (int k = 0; k < 2; ++k) {
+        double cx = k == 0 ? R - sp : sp - R;
+        d3 oc = sub(lo, D3(cx, 0.0, 0.0));
+        double b = dot(oc, ld), c = dot(oc, oc) - R * R, disc = b * b - c;
+        if (disc < 0.0) continue;
+        double sq = sqrt(disc), roots[2] = {-b - sq, -b + sq};
+        for (int i = 0; i < 2; ++i) {
+            double tt = roots[i];
+            if (!(tt > EPS && tt < best)) continue;
+            double xr = oc.x + ld.x * tt;
+            if (k == 0 ? !(xr <= -(R - h)) : !(xr >= R - h)) continue;
+            best = tt;
+            hit = 1;
+            d3 nl = unit(add(oc, mul(ld, tt)));
+            *n = rot_col(ang, conv ? nl : mul(nl, -1.0));
+            break;
+        }
+    }
+    if (hit) *t = best;
+    return hit;
+}
+
+/* p(t) = c[0] + c[1] t + ... + c[n] t^n (Horner). */
+static double poly_at(const double* c, int n, double t) {
+    double v = c[n];
+    for (int k = n - 1; k >= 0; --k) v = v * t + c[k];
+    return v;
+}
+/* The real roots of p in (lo, hi), ascending, at most n: the roots of p' split
+ * (lo, hi) into intervals on which p is monotonic, and an interval whose ends
+ * differ in sign holds one root, found by bisection to the last bit. A double
+ * root (a tangency, where p does not change sign) is not found. */
+static int poly_roots(const double* c, int n, double lo, double hi, double* out) {
+    if (n <= 1) {
+        if (n == 1 && c[1] != 0.0) {
+            double t = -c[0] / c[1];
+            if (t > lo && t < hi) { out[0] = t; return 1; }
+        }
+        return 0;
+    }
+    double dc[4], crit[4], ends[6];
+    for (int k = 0; k < n; ++k) dc[k] = (k + 1) * c[k + 1];
+    int nc = poly_roots(dc, n - 1, lo, hi, crit), ne = 0, m = 0;
+    ends[ne++] = lo;
+    for (int i = 0; i < nc; ++i) ends[ne++] = crit[i];
+    ends[ne++] = hi;
+    for (int i = 0; i + 1 < ne; ++i) {
+        double a = ends[i], b = ends[i + 1], fa = poly_at(c, n, a), fb = poly_at(c, n, b);
+        if ((fa < 0.0) == (fb < 0.0)) continue;
+        for (int it = 0; it < 200; ++it) {
+            double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;
+            double fm = poly_at(c, n, mid);
+            if ((fm < 0.0) == (fa < 0.0)) { a = mid; fa = fm; } else { b = mid; }
+        }
+        out[m++] = 0.5 * (a + b);
+    }
+    return m;
+}
+/* A Dupin cyclide (src/shader.comp:633-679): in its frame, p = (v * RotationMatrix
+ * - ...) / scale, the surface F(p) = (|p|^2 + b^2 - d^2)^2 - 4 (a p.x - c d)^2
+ * - 4 b^2 p.z^2 = 0, searched inside its bounding sphere (radius^2 = brad).
+ * F along the ray is a quartic in t (the same t as the world ray's); its
+ * smallest root beyond EPS is the hit, and *n the world gradient of F
+ * (outward: F > 0 far away). */
+static int hit_cyclide(const double* q, d3 o, d3 d, double tmax, double* t, d3* n) {
+    d3 pos = D3(q[0], q[1], q[2]), ang = D3(q[3], q[4], q[5]), sc = D3(q[6], q[7], q[8]);
+    double A = q[9], B = q[10], Cc = q[11], D = q[12], brad = q[13];
+    d3 oc = sub(o, pos);
+    double bb = dot(oc, d), cc = dot(oc, oc) - brad, disc = bb * bb - cc;
+    if (disc < 0.0) return 0;
+    double sq = sqrt(disc), ta = fmax(-bb - sq, EPS), tb = fmin(-bb + sq, tmax);
+    if (!(ta < tb)) return 0;
+    d3 lp = rot_row(ang, oc), lv = rot_row(ang, d);
+    d3 p = D3(lp.x / sc.x, lp.y / sc.y, lp.z / sc.z), v = D3(lv.x / sc.x, lv.y / sc.y, lv.z / sc.z);
+    /* S(t) = s2 t^2 + s1 t + s0, L1(t) = l1 t + l0, L2(t) = m1 t + m0 */
+    double s2 = dot(v, v), s1 = 2.0 * dot(p, v), s0 = dot(p, p) + B * B - D * D;
+    double l1 = A * v.x, l0 = A * p.x - Cc * D, m1 = v.z, m0 = p.z;
+    double c[5];
+    c[4] = s2 * s2;
+    c[3] = 2.0 * s2 * s1;
+    c[2] = s1 * s1 + 2.0 * s2 * s0 - 4.0 * l1 * l1 - 4.0 * B * B * m1 * m1;
+    c[1] = 2.0 * s1 * s0 - 8.0 * l1 * l0 - 8.0 * B * B * m1 * m0;
+    c[0] = s0 * s0 - 4.0 * l0 * l0 - 4.0 * B * B * m0 * m0;
+    double roots[4];
+    if (poly_roots(c, 4, ta, tb, roots) == 0) return 0;
+    double tt = roots[0];
+    d3 x = add(p, mul(v, tt));
+    double S = dot(x, x) + B * B - D * D;
+    d3 g = D3(4.0 * x.x * S - 8.0 * A * (A * x.x - Cc * D), 4.0 * x.y * S, 4.0 * x.z * S - 8.0 * B * B * x.z);
+    *n = unit(rot_col(ang, D3(g.x / sc.x, g.y / sc.y, g.z / sc.z)));
+    *t = tt;
+    return 1;
+}
+
 static int slab(const box* b, d3 o, d3 inv, double tmax) {
     double t0 = 0.0, t1 = tmax;
     for (int k = 0; k < 3; ++k) {
@@ -311,7 +436,7 @@ static double march(const double* q, d3 o, d3 d, double tmax, int max_steps, uin
     return -1.0;
 }
 
-typedef struct { double t; int type, idx; } hitrec;
+typedef struct { double t; int type, idx; d3 n; /* lens, cyclide: the normal found with the hit */ } hitrec;
 
 static int trace(const Accel* A, d3 o, d3 d, double tmax, int any, hitrec* h, uint64_t* steps) {
     const Scene* s = A->s;
@@ -347,6 +472,28 @@ static int trace(const Accel* A, d3 o, d3 d, double tmax, int any, hitrec* h, ui
             h->t = t;
             h->type = T_PLANE;
             h->idx = i;
+            if (any) return 1;
+        }
+    }
+    for (int i = 0; i < s->nlens; ++i) {
+        double t;
+        d3 nn;
+        if (hit_lens(s->lens + LENS_REC * i, o, d, h->t, &t, &nn)) {
+            h->t = t;
+            h->type = T_LENS;
+            h->idx = i;
+            h->n = nn;
+            if (any) return 1;
+        }
+    }
+    for (int i = 0; i < s->ncyc; ++i) {
+        double t;
+        d3 nn;
+        if (hit_cyclide(s->cyc + CYC_REC * i, o, d, h->t, &t, &nn)) {
+            h->t = t;
+            h->type = T_CYC;
+            h->idx = i;
+            h->n = nn;
             if (any) return 1;
         }
     }
@@ -443,6 +590,20 @@ static void surface(const Scene* s, const hitrec* h, d3 p, d3* n, int* mat, int*
             *mat = s->plane_ml[2 * h->idx];
             *light = s->plane_ml[2 * h->idx + 1];
             break;
+        case T_LENS: {
+            const double* q = s->lens + LENS_REC * h->idx;
+            *n = h->n;
+            *mat = (int)q[10];
+            *light = (int)q[11];
+            break;
+        }
+        case T_CYC: {
+            const double* q = s->cyc + CYC_REC * h->idx;
+            *n = h->n;
+            *mat = (int)q[14];
+            *light = (int)q[15];
+            break;
+        }
         default: {
             const double* q = s->sdf + 10 * h->idx;
             double e = 1e-5;
@@ -471,7 +632,7 @@ static double light_pdf(const Scene* s, int li, d3 x, d3 dir, double dist) {
         double cmax = sqrt(fmax(0.0, 1.0 - q[3] * q[3] / dot(dc, dc)));
         pdf = 1.0 / (2.0 * M_PI * (1.0 - cmax));
     }
-    return pdf / s->nlight;
+    return pdf / s->nlight_nee;
 }
 
 /* One path: returns its XYZ estimate. */
@@ -501,7 +662,7 @@ static void trace_path(const Scene* s, const Accel* A, uint32_t key, d3 o, d3 d,
         surface(s, &h, p, &n, &mi, &li);
         if (li >= 0) {
             double w = 1.0;
-            if (!specular && s->nee) {
+            if (!specular && s->nee && li < s->nlight_nee) {
                 double pl = light_pdf(s, li, o, d, h.t);
                 w = pdf_prev * pdf_prev / (pdf_prev * pdf_prev + pl * pl);
             }
@@ -517,9 +678,9 @@ static void trace_path(const Scene* s, const Accel* A, uint32_t key, d3 o, d3 d,
             double rho[4];
             for (int j = 0; j < 4; ++j) rho[j] = reflect_spec(m, lam[j]);
             d3 P = add(p, mul(N, scale));
-            if (s->nlight > 0 && s->nee) {
-                int k = (int)(rnd(key, b, 3) * s->nlight);
-                if (k >= s->nlight) k = s->nlight - 1;
+            if (s->nlight_nee > 0 && s->nee) {
+                int k = (int)(rnd(key, b, 3) * s->nlight_nee);
+                if (k >= s->nlight_nee) k = s->nlight_nee - 1;
                 const double* L = s->light + 4 * k;
                 double u4 = rnd(key, b, 4), u5 = rnd(key, b, 5);
                 d3 wi;
@@ -786,6 +947,21 @@ int ptext_probe_sphere(const double q[4], const double o[3], const double d[3], 
     return hit_sphere(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, t);
 }
 /* m = one material row (type n dispersive p[9]) */
+int ptext_probe_lens(const double q[12], const double o[3], const double d[3], double* t, double n[3]) {
+    d3 nn = D3(0, 0, 0);
+    int hit = hit_lens(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, t, &nn);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    return hit;
+}
+int ptext_probe_cyclide(const double q[16], const double o[3], const double d[3], double* t, double n[3]) {
+    d3 nn = D3(0, 0, 0);
+    int hit = hit_cyclide(q, D3(o[0], o[1], o[2]), D3(d[0], d[1], d[2]), 1e300, t, &nn);
+    n[0] = nn.x; n[1] = nn.y; n[2] = nn.z;
+    return hit;
+}
+int ptext_probe_poly_roots(const double* c, int n, double lo, double hi, double* out) {
+    return poly_roots(c, n, lo, hi, out);
+}
 double ptext_probe_ior(const double m[12], double lnm) { return ior(m, lnm); }
 double ptext_probe_reflectance(const double m[12], double lnm) { return reflect_spec(m, lnm); }
 /* Sphere tracing of one SDF record q (kind iters bailout mat cx cy cz sx sy sz)

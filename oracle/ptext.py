@@ -34,6 +34,7 @@ class Scene(C.Structure):
         ("cam_pos", C.c_double * 3), ("cam_angle", C.c_double * 2), ("cam_size", C.c_double), ("ap_dist", C.c_double),
         ("camera", C.c_int), ("ap_size", C.c_double), ("lens_dist", C.c_double), ("lens_radius", C.c_double),
         ("lens_focal", C.c_double), ("lens_thick", C.c_double), ("nee", C.c_int), ("lens_missed", C.c_uint64),
+        ("nlens", C.c_int), ("ncyc", C.c_int), ("lens", _D), ("cyc", _D), ("nlight_nee", C.c_int),
     ]
 
 
@@ -54,6 +55,11 @@ def _setup(L: C.CDLL) -> C.CDLL:
     L.ptext_probe_march.argtypes = [_D, _D, _D, C.c_int, C.POINTER(C.c_uint64)]
     L.ptext_probe_de.restype = C.c_double
     L.ptext_probe_de.argtypes = [_D, _D]
+    for fn in ("ptext_probe_lens", "ptext_probe_cyclide"):
+        getattr(L, fn).restype = C.c_int
+        getattr(L, fn).argtypes = [_D, _D, _D, _D, _D]
+    L.ptext_probe_poly_roots.restype = C.c_int
+    L.ptext_probe_poly_roots.argtypes = [_D, C.c_int, C.c_double, C.c_double, _D]
     L.ptext_probe_camera.restype = C.c_int
     L.ptext_probe_camera.argtypes = [C.POINTER(Scene), C.c_uint32, C.c_double, C.c_double, C.c_double, _D, _D]
     return L
@@ -117,6 +123,18 @@ class Flat:
             o2 = o + X + Y + Z
             for Q, e1, e2 in [(o, Z, Y), (o, X, Z), (o, Y, X), (o2, -Y, -Z), (o2, -Z, -X), (o2, -X, -Y)]:
                 add_quad(list(Q), list(e1), list(e2), b["materialID"], b["lightID"])
+        nlight_nee = len(lights)  # spheres and parallelograms: sampled by NEE
+        lens, cyc = [], []
+        for o in sc.get("lens", []):
+            li = light_of(o["lightID"], 2, len(lens))
+            lens.append([*map(float, o["position"]), *map(float, o["rotation"]), float(o["radius"]),
+                         float(o["focalLength"]), float(o["thickness"]), 1.0 if o["isConverging"] else 0.0,
+                         o["materialID"] - 1, li])
+        for o in sc.get("cyclide", []):
+            li = light_of(o["lightID"], 3, len(cyc))
+            cyc.append([*map(float, o["position"]), *map(float, o["rotation"]), *map(float, o["scale"]),
+                        float(o["a"]), float(o["b"]), float(o["c"]), float(o["d"]), float(o["boundingRadius"]),
+                        o["materialID"] - 1, li])
         planes = sc.get("plane", [])
         plane_y = [float(p["position"][1]) for p in planes]
         plane_ml = [[p["materialID"] - 1, -1] for p in planes]
@@ -169,6 +187,8 @@ class Flat:
         s.quad, s.quad_ml = arr(quad, np.float64), arr(quad_ml, np.int32)
         s.plane_y, s.plane_ml = arr(plane_y, np.float64), arr(plane_ml, np.int32)
         s.sdf, s.mat, s.light = arr(sdfs, np.float64), arr(mats, np.float64), arr(lights, np.float64)
+        s.nlens, s.ncyc, s.nlight_nee = len(lens), len(cyc), nlight_nee
+        s.lens, s.cyc = arr(lens, np.float64), arr(cyc, np.float64)
         self.cie = np.ascontiguousarray(cie, dtype=np.float32)
         s.cie = self.cie.ctypes.data_as(C.POINTER(C.c_float))
         env = sc.get("environment")

@@ -365,7 +365,7 @@ def test_sphere_light_pdf_nee_equals_bsdf_sampling():
     ("        pdf = dist * dist / (cl * area);", "        pdf = dist / (cl * area);", False),
     ("        double cl = fabs(dot(nn, dir)) / area;", "        double cl = 1.0;", False),
     ("        pdf = 1.0 / (2.0 * M_PI * (1.0 - cmax));", "        pdf = 1.0 / (M_PI * (1.0 - cmax));", True),
-    ("    return pdf / s->nlight;", "    return pdf;", True),
+    ("    return pdf / s->nlight_nee;", "    return pdf;", True),
 ], ids=["quad_distance", "quad_cosine", "sphere_cone", "light_choice"])
 def test_nee_check_catches_planted_mistakes(tmp_path, find, replace, sphere):
     L = planted(tmp_path, find, replace)
