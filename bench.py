@@ -237,6 +237,20 @@ def traffic_from_profiles(kernel: str, workload: str):
     return None
 
 
+def valu_issue_from_profiles(kernel: str, workload: str):
+    """The kernel's VALU issue share (wave-wide VALU instructions over what the
+    CUs can issue in its launches' time) from the committed rocprofv3 --pmc
+    summary of the same workload, or None."""
+    for f in sorted((ROOT / "profiles").glob(f"pmc_{kernel}_{workload}.txt")):
+        for line in f.read_text().splitlines():
+            if line.startswith("valu_issue_share:"):
+                try:
+                    return float(line.split(":", 1)[1].split()[0])
+                except ValueError:
+                    return None
+    return None
+
+
 def measured_peaks():
     """Device peaks measured by `bench.py --peak-probe` (streaming copy GB/s,
     unpacked FP32 FMA loop TFLOP/s) from the committed summary
@@ -277,6 +291,8 @@ def roofline_of(prof: dict, pcnt: dict, workload: str, tag: str):
                 # beside the published peak: the FMA loop measured on this device
                 "peak_measured": mp and mp["TFLOP/s"],
                 "frac_measured": mp and round(achieved / mp["TFLOP/s"], 5),
+                # what bounds it: VALU issue (rocprofv3 counters of the same workload)
+                "valu_issue_share": valu_issue_from_profiles(name, tag),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_flops_per_launch": round(flops / launches),
@@ -294,6 +310,10 @@ def roofline_of(prof: dict, pcnt: dict, workload: str, tag: str):
                 # beside the published peak: the streaming copy measured on this device
                 "peak_measured": mp and mp["GB/s"],
                 "frac_measured": mp and round(achieved / mp["GB/s"], 5),
+                # the divergent traversal kernels are bound by VALU issue and
+                # dependent fetches, not bandwidth: their issue share from the
+                # rocprofv3 counters of the same workload
+                "valu_issue_share": valu_issue_from_profiles(name, tag),
                 "kernel": name,
                 "avg_launch_ms": round(total_ms / launches, 4),
                 "algorithmic_bytes_per_launch": round(total_bytes / launches),

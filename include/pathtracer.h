@@ -118,7 +118,9 @@ typedef struct pt_sdf_desc {
  * PT_MODE_EXT: the extended physically based mode (quads, multi-Gaussian
  *   reflectance, dispersive dielectrics, conductors, environment light, BVH,
  *   hero wavelengths over [380, 780) nm, counter-based PCG keyed on
- *   (seed, pixel, sample, bounce, dimension)). */
+ *   (seed, pixel, sample, bounce, dimension)); it also takes the reference's
+ *   lenses and cyclides, shaded with their materials (emitting ones are found
+ *   by BSDF sampling only). */
 typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
 
 /* Launch/tuning knobs (0 = library default). */

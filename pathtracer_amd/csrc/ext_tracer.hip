@@ -42,7 +42,7 @@ using namespace ptd;
 
 // ---------------------------------------------------------------------------------
 // Device scene.
-enum : uint32_t { REF_SPHERE = 0u, REF_QUAD = 1u, REF_PLANE = 2u, REF_SDF = 3u };
+enum : uint32_t { REF_SPHERE = 0u, REF_QUAD = 1u, REF_PLANE = 2u, REF_SDF = 3u, REF_LENS = 4u, REF_CYC = 5u };
 constexpr uint32_t kRefShift = 29u, kRefMask = (1u << 29) - 1u, kMiss = 0xFFFFFFFFu;
 enum : int { Q_MISS = 0, Q_EMIT = 1, Q_LAMBERT = 2, Q_DIELECTRIC = 3, Q_CONDUCTOR = 4, N_MATQ = 5 };
 constexpr int kQuadBrute = 16;  // at most this many parallelograms are tested without a BVH
@@ -68,7 +68,7 @@ struct ExtMat {
     float p[9];      // lobe (peak, sigma, invert) | gaussians (w, mean, sigma) x n | dispersion | (n, k)
 };
 struct ExtLight {
-    int type;        // REF_SPHERE or REF_QUAD
+    int type;        // REF_SPHERE or REF_QUAD (sampled by NEE); REF_LENS or REF_CYC (found by BSDF sampling only)
     int prim;
     float T, lum;    // blackbody temperature, luminosity
     f3 p, e1, e2, n; // quad: corner, edges, unit normal; sphere: centre in p
@@ -78,6 +78,27 @@ struct ExtSdf {
     int kind, iters, mat, pad;
     float bailout;
     f3 lo, hi, c;
+};
+// The reference's lens (src/shader.comp:366-448) in its frame (object = v * M,
+// x along the optical axis): two spheres of radius R with centres on the axis
+// at +-(R - sp); cap 0 is the part of the first with x - centre <= -(R - h),
+// cap 1 the part of the second with x - centre >= R - h (h the cap height).
+// The solid is inside both spheres when converging, outside them otherwise.
+struct ExtLens {
+    f3 pos;
+    m3 M;
+    float R, h, sp, br2;  // sphere radius (2f), cap height, slice position, bounding radius^2
+    int conv, mat, light, pad;
+};
+// The reference's Dupin cyclide (src/shader.comp:633-679): in its frame
+// p = (v * M) / scale, the surface (|p|^2 + B^2 - D^2)^2 - 4 (A p.x - C D)^2
+// - 4 B^2 p.z^2 = 0 inside a bounding sphere of radius^2 brad2.
+struct ExtCyc {
+    f3 pos;
+    m3 M;
+    f3 inv_scale;
+    float A, B, C, D, brad2;
+    int mat, light;
 };
 // A BVH over one primitive type with leaf-ordered primitive records.
 struct BvhDev {
@@ -97,8 +118,12 @@ struct ExtSceneDev {
     const ExtSdf* sdf;
     const ExtMat* mat;
     const ExtLight* light;
+    const ExtLens* lens;
+    const ExtCyc* cyc;
     BvhDev sbvh, qbvh;
     int nsph, nquad, nplane, nsdf, nmat, nlight;
+    int nlens, ncyc;
+    int nlight_nee;     // lights [0, nlight_nee) are sampled by next-event estimation
     int quad_bvh;       // parallelograms go through qbvh
     int has_env;
     float env;
@@ -317,6 +342,206 @@ __device__ __forceinline__ bool isect_quad(const float4* q, f3 o, f3 d, float tm
     if (a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f) return false;
     t = tt;
     return true;
+}
+
+// The lens's nearest cap point in (kEps, tmax) (struct ExtLens). Each cap's
+// sphere is solved like isect_sphere; a root counts when it lies on the cap.
+__device__ __forceinline__ bool isect_lens(const ExtLens& L, f3 o, f3 d, float tmax, float& t) {
+    const f3 oc0 = o - L.pos;
+    {  // the bounding sphere
+        const float b = fdot(oc0, d);
+        const f3 f = mk3(fmaf(-b, d.x, oc0.x), fmaf(-b, d.y, oc0.y), fmaf(-b, d.z, oc0.z));
+        if (fdot(f, f) > L.br2) return false;
+    }
+    const f3 lo = vmul(oc0, L.M), ld = vmul(d, L.M);
+    const float R = L.R, rim = R - L.h;
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const f3 oc = mk3(lo.x - (k == 0 ? R - L.sp : L.sp - R), lo.y, lo.z);
+        const float b = fdot(oc, ld);
+        const f3 f = mk3(fmaf(-b, ld.x, oc.x), fmaf(-b, ld.y, oc.y), fmaf(-b, ld.z, oc.z));
+        const float disc = fmaf(R, R, -fdot(f, f));
+        if (disc < 0.0f) continue;
+        const float q = hw_sqrt(disc);
+        const float r0 = -b - q, r1 = -b + q;
+        const float x0 = fmaf(ld.x, r0, oc.x), x1 = fmaf(ld.x, r1, oc.x);
+        const bool c0 = k == 0 ? x0 <= -rim : x0 >= rim, c1 = k == 0 ? x1 <= -rim : x1 >= rim;
+        if (c0 && r0 > kEps && r0 < tmax) {
+            tmax = r0;
+            hit = true;
+        } else if (c1 && r1 > kEps && r1 < tmax) {
+            tmax = r1;
+            hit = true;
+        }
+    }
+    if (hit) t = tmax;
+    return hit;
+}
+// The lens solid's outward normal at a surface point p: from the centre of the
+// cap sphere p lies on (the one whose distance is nearer R), outwards for a
+// converging lens, inwards for a diverging one.
+__device__ __forceinline__ f3 lens_normal(const ExtLens& L, f3 p) {
+    const f3 lp = vmul(p - L.pos, L.M);
+    const f3 a = mk3(lp.x - (L.R - L.sp), lp.y, lp.z), b = mk3(lp.x - (L.sp - L.R), lp.y, lp.z);
+    const float da = fabsf(hw_sqrt(fdot(a, a)) - L.R), db = fabsf(hw_sqrt(fdot(b, b)) - L.R);
+    f3 n = normalize(da <= db ? a : b);
+    if (!L.conv) n = -n;
+    return mvmul(L.M, n);
+}
+
+// Polynomial c[0] + c[1] x + ... + c[N] x^N (Horner, fused multiply-adds).
+template <int N>
+__device__ __forceinline__ float poly_eval(const float* c, float x) {
+    float v = c[N];
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k) v = fmaf(v, x, c[k]);
+    return v;
+}
+// The sign changes of the degree-N polynomial c between consecutive entries of
+// `ends` (ascending; at most NE of them), each found by bisection: `ends` are
+// the interval ends and the real roots of the derivative between them, so the
+// polynomial is monotonic on each interval and a sign change there is its only
+// root. Writes the roots found, ascending, into `roots` (the first only with
+// first_only) and returns their number.
+template <int N, int NE>
+__device__ __forceinline__ int poly_brackets(const float* c, const float* ends, int ne, float* roots, bool first_only) {
+    int m = 0;
+    float fa = poly_eval<N>(c, ends[0]);
+    for (int i = 0; i + 1 < ne && i + 1 < NE; ++i) {
+        float a = ends[i], b = ends[i + 1];
+        const float fb = poly_eval<N>(c, b);
+        if ((fa < 0.0f) != (fb < 0.0f)) {
+            float fl = fa;
+            for (int it = 0; it < 40; ++it) {
+                const float mid = 0.5f * (a + b);
+                if (!(mid > a && mid < b)) break;
+                const float fm = poly_eval<N>(c, mid);
+                if ((fm < 0.0f) == (fl < 0.0f)) {
+                    a = mid;
+                    fl = fm;
+                } else {
+                    b = mid;
+                }
+            }
+            roots[m++] = 0.5f * (a + b);
+            if (first_only) return m;
+        }
+        fa = fb;
+    }
+    return m;
+}
+// Smallest root of the quartic c[0..4] in (0, hi), by isolating the real roots
+// of its derivatives in turn: its second derivative (a quadratic, in closed
+// form), then its first derivative (a cubic) on the intervals between those,
+// then the quartic itself on the intervals between the cubic's roots. A root
+// where the quartic only touches zero (a tangency) is not found.
+__device__ __forceinline__ bool quartic_first_root(const float c[5], float hi, float& root) {
+    const float d1[4] = {c[1], 2.0f * c[2], 3.0f * c[3], 4.0f * c[4]};
+    const float d2[3] = {d1[1], 2.0f * d1[2], 3.0f * d1[3]};
+    float e2[4];
+    int n2 = 0;
+    e2[n2++] = 0.0f;
+    {  // the roots of the quadratic d2 in (0, hi), ascending
+        const float A = d2[2], B = d2[1], C = d2[0];
+        if (A != 0.0f) {
+            const float disc = fmaf(B, B, -4.0f * A * C);
+            if (disc >= 0.0f) {
+                const float q = -0.5f * (B + copysignf(hw_sqrt(disc), B));
+                float r0 = q / A, r1 = q != 0.0f ? C / q : r0;
+                if (r0 > r1) {
+                    const float tmp = r0;
+                    r0 = r1;
+                    r1 = tmp;
+                }
+                if (r0 > 0.0f && r0 < hi) e2[n2++] = r0;
+                if (r1 > 0.0f && r1 < hi && r1 != r0) e2[n2++] = r1;
+            }
+        } else if (B != 0.0f) {
+            const float r0 = -C / B;
+            if (r0 > 0.0f && r0 < hi) e2[n2++] = r0;
+        }
+    }
+    e2[n2++] = hi;
+    float e3[5], r3[3];
+    int n3 = 0;
+    e3[n3++] = 0.0f;
+    const int m3 = poly_brackets<3, 4>(d1, e2, n2, r3, false);
+    for (int i = 0; i < m3; ++i) e3[n3++] = r3[i];
+    e3[n3++] = hi;
+    float r4[1];
+    if (poly_brackets<4, 5>(c, e3, n3, r4, true) == 0) return false;
+    root = r4[0];
+    return true;
+}
+// The cyclide's nearest hit in (kEps, tmax) (struct ExtCyc): the implicit
+// function along the ray is a quartic in s = t - ta, ta the entry into the
+// bounding sphere (expanding about ta keeps the coefficients of the short
+// search interval well scaled in FP32).
+__device__ __forceinline__ bool isect_cyclide(const ExtCyc& C, f3 o, f3 d, float tmax, float& t) {
+    const f3 oc = o - C.pos;
+    const float b = fdot(oc, d);
+    const f3 f = mk3(fmaf(-b, d.x, oc.x), fmaf(-b, d.y, oc.y), fmaf(-b, d.z, oc.z));
+    const float disc = C.brad2 - fdot(f, f);
+    if (disc < 0.0f) return false;
+    const float q = hw_sqrt(disc);
+    const float ta = fmaxf(-b - q, kEps), tb = fminf(-b + q, tmax);
+    if (!(ta < tb)) return false;
+    const f3 lv = vmul(d, C.M), lo = vmul(oc, C.M);
+    const f3 v = mk3(lv.x * C.inv_scale.x, lv.y * C.inv_scale.y, lv.z * C.inv_scale.z);
+    const f3 p = mk3(fmaf(v.x, ta, lo.x * C.inv_scale.x), fmaf(v.y, ta, lo.y * C.inv_scale.y),
+                     fmaf(v.z, ta, lo.z * C.inv_scale.z));
+    // S(s) = s2 s^2 + s1 s + s0, L1(s) = l1 s + l0, L2(s) = m1 s + m0;
+    // F = S^2 - 4 L1^2 - 4 B^2 L2^2
+    const float B2 = C.B * C.B;
+    const float s2 = fdot(v, v), s1 = 2.0f * fdot(p, v), s0 = fdot(p, p) + (B2 - C.D * C.D);
+    const float l1 = C.A * v.x, l0 = fmaf(C.A, p.x, -C.C * C.D), m1 = v.z, m0 = p.z;
+    const float c[5] = {
+        fmaf(s0, s0, -4.0f * fmaf(l0, l0, B2 * m0 * m0)),
+        2.0f * fmaf(s1, s0, -4.0f * fmaf(l1, l0, B2 * m1 * m0)),
+        fmaf(s1, s1, fmaf(2.0f * s2, s0, -4.0f * fmaf(l1, l1, B2 * m1 * m1))),
+        2.0f * s2 * s1,
+        s2 * s2,
+    };
+    float s;
+    if (!quartic_first_root(c, tb - ta, s)) return false;
+    t = ta + s;
+    return t > kEps && t < tmax;
+}
+// Outward normal (the world gradient of the implicit function) at p.
+__device__ __forceinline__ f3 cyclide_normal(const ExtCyc& C, f3 p) {
+    const f3 l = vmul(p - C.pos, C.M);
+    const f3 x = mk3(l.x * C.inv_scale.x, l.y * C.inv_scale.y, l.z * C.inv_scale.z);
+    const float S = fdot(x, x) + (C.B * C.B - C.D * C.D);
+    const f3 g = mk3(4.0f * x.x * S - 8.0f * C.A * fmaf(C.A, x.x, -C.C * C.D), 4.0f * x.y * S,
+                     4.0f * x.z * S - 8.0f * C.B * C.B * x.z);
+    return normalize(mvmul(C.M, mk3(g.x * C.inv_scale.x, g.y * C.inv_scale.y, g.z * C.inv_scale.z)));
+}
+// Lenses and cyclides (a few per scene): the closest hit below t (ANY: the
+// first). The k_ext_extras kernels apply it after the intersect kernels, so
+// the traversal kernels of scenes without them carry none of its code.
+template <bool ANY>
+__device__ __forceinline__ bool trace_extras(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref) {
+    bool hit = false;
+    for (int i = 0; i < S.nlens; ++i) {  // uniform index: scalar loads
+        float tl;
+        if (isect_lens(S.lens[i], o, d, t, tl)) {
+            t = tl;
+            ref = (REF_LENS << kRefShift) | (uint32_t)i;
+            hit = true;
+            if (ANY) return true;
+        }
+    }
+    for (int i = 0; i < S.ncyc; ++i) {
+        float tc;
+        if (isect_cyclide(S.cyc[i], o, d, t, tc)) {
+            t = tc;
+            ref = (REF_CYC << kRefShift) | (uint32_t)i;
+            hit = true;
+            if (ANY) return true;
+        }
+    }
+    return hit;
 }
 
 // ---------------------------------------------------------------------------------
@@ -691,6 +916,20 @@ __device__ __forceinline__ Surf surface(const ExtSceneDev& S, uint32_t ref, f3 o
             s.light = ml.y;
             break;
         }
+        case REF_LENS: {
+            const ExtLens& L = S.lens[idx];
+            s.n = lens_normal(L, s.p);
+            s.mat = L.mat;
+            s.light = L.light;
+            break;
+        }
+        case REF_CYC: {
+            const ExtCyc& C = S.cyc[idx];
+            s.n = cyclide_normal(C, s.p);
+            s.mat = C.mat;
+            s.light = C.light;
+            break;
+        }
         default: {
             const ExtSdf& q = S.sdf[idx];
             s.n = ext_sdf_normal<0>(q, s.p);
@@ -709,6 +948,8 @@ __device__ __forceinline__ bool ref_is_bad(const ExtSceneDev& S, uint32_t ref) {
                          : kind == REF_QUAD   ? (uint32_t)S.nquad
                          : kind == REF_PLANE  ? (uint32_t)S.nplane
                          : kind == REF_SDF    ? (uint32_t)S.nsdf
+                         : kind == REF_LENS   ? (uint32_t)S.nlens
+                         : kind == REF_CYC    ? (uint32_t)S.ncyc
                                               : 0u;
     return (ref & kRefMask) >= limit;
 }
@@ -731,6 +972,8 @@ __device__ __forceinline__ int hit_class(const ExtSceneDev& S, uint32_t ref) {
         case REF_SPHERE: { int2 ml = S.sph_ml[idx]; mat = ml.x; light = ml.y; break; }
         case REF_QUAD: { int2 ml = S.quad_ml[idx]; mat = ml.x; light = ml.y; break; }
         case REF_PLANE: { int2 ml = S.plane_ml[idx]; mat = ml.x; light = ml.y; break; }
+        case REF_LENS: mat = S.lens[idx].mat; light = S.lens[idx].light; break;
+        case REF_CYC: mat = S.cyc[idx].mat; light = S.cyc[idx].light; break;
         default: mat = S.sdf[idx].mat; light = -1; break;
     }
     if (light >= 0) return Q_EMIT;
@@ -1408,7 +1651,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
         const ExtLight& L = S.light[sf.light];
         uint4 aux = Wv.aux[p];
         float w = 1.0f;
-        if (!(aux.y & 2u)) {
+        // emitters that NEE does not sample (lenses, cyclides) count with weight 1
+        if (!(aux.y & 2u) && sf.light < S.nlight_nee) {
             // pdf with which next-event estimation from the previous vertex picks this point
             float pl;
             if (L.type == REF_QUAD) {
@@ -1420,7 +1664,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
                 float cmax = sqrtf(fmaxf(1.0f - sin2, 0.0f));
                 pl = 1.0f / (2.0f * kPiF * fmaxf(1.0f - cmax, 1e-12f));
             }
-            pl *= 1.0f / (float)S.nlight;
+            pl *= 1.0f / (float)S.nlight_nee;
             float pb = __uint_as_float(aux.z);
             w = pb * pb / (pb * pb + pl * pl);
         }
@@ -1468,9 +1712,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
             f4 T = ld4(Wv.T, p);
             uint32_t key = aux.x;
             f3 P = sf.p + N * offset_scale(sf.p);
-            if (S.nlight > 0) {
+            if (S.nlight_nee > 0) {
                 float ul = rnd(key, (uint32_t)bounce, 3);
-                int li = min((int)(ul * (float)S.nlight), S.nlight - 1);
+                int li = min((int)(ul * (float)S.nlight_nee), S.nlight_nee - 1);
                 const ExtLight& L = S.light[li];
                 float u4 = rnd(key, (uint32_t)bounce, 4), u5 = rnd(key, (uint32_t)bounce, 5);
                 f3 wi;
@@ -1502,7 +1746,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
                     dist = th;
                     pl = 1.0f / (2.0f * kPiF * fmaxf(1.0f - cmax, 1e-12f));
                 }
-                pl *= 1.0f / (float)S.nlight;
+                pl *= 1.0f / (float)S.nlight_nee;
                 float cs = dot(N, wi);
                 if (ok && cs > 0.0f) {
                     float pb = cs * (1.0f / kPiF);
@@ -1776,6 +2020,43 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
     wave_add_u64(stat_slot(Wv.stats, kStatShadowMarchFlops), flops);
 }
 
+// Lenses and cyclides against the extension rays of queue `cur`, after the
+// intersect kernel and before the SDF marches and the classification (scenes
+// that have any): a closer hit replaces the record. The intersect kernels'
+// SDF box mask was taken with the farther hit, so it can only name boxes a
+// march then bounds by the nearer one.
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_extras(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
+    const int n = *qlen(Wv, cur);
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        const int p = Wv.q[cur][idx];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+        if (S.nlens + S.ncyc == 0) continue;
+        f3 o, d;
+        ray_of(Wv, cur, rq, idx, p, o, d);
+        const float2 h = Wv.hit[p];
+        float t = h.x;
+        uint32_t ref = __float_as_uint(h.y);
+        if (trace_extras<false>(S, o, d, t, ref)) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+    }
+}
+// Lenses and cyclides against the shadow rays of counter `shq`, before the
+// shadow kernel: a blocked ray's contribution becomes zero.
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlk) k_ext_extras_shadow(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int shq) {
+    const int n = *qlen(Wv, shq);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 sx = Wv.s_xyz[i];
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
+        if (S.nlens + S.ncyc == 0) continue;
+        const float4 so = Wv.s_org[i], sd = Wv.s_dir[i];
+        float t = so.w;
+        uint32_t ref = kMiss;
+        if (trace_extras<true>(S, mk3(so.x, so.y, so.z), mk3(sd.x, sd.y, sd.z), t, ref))
+            Wv.s_xyz[i] = make_float4(0.0f, 0.0f, 0.0f, sx.w);
+    }
+}
+
 // Per pixel, in sample order: add each path's XYZ to the running sum.
 __global__ void __launch_bounds__(kBlk) k_ext_accum(ExtFrame F, ExtWave Wv) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1962,10 +2243,6 @@ static inline f3 v3(const float* a) { return mk3(a[0], a[1], a[2]); }
 
 static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     const pt::SceneModel& m = s->model;
-    if (!m.lenses.empty() || !m.cyclides.empty()) {
-        ctx->err = "extended mode has no lenses or cyclides (reference mode only)";
-        return PT_ERR_STATE;
-    }
     std::vector<float4> sph, quad;
     std::vector<int2> sph_ml, quad_ml, plane_ml;
     std::vector<float> plane_y;
@@ -2042,6 +2319,48 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
             if (st != PT_OK) return st;
         }
     }
+    // Lights found by next-event estimation come first; the lenses' and
+    // cyclides' (hit by BSDF sampling only) after them.
+    const int nlight_nee = (int)lights.size();
+    std::vector<ExtLens> lenses;
+    std::vector<ExtCyc> cycs;
+    for (const pt::RLens& o : m.lenses) {
+        ExtLens L{};
+        L.pos = v3(o.pos);
+        L.M = rotation_deg(v3(o.rotation));
+        const float r = o.radius, f = o.focalLength, th = o.thickness;
+        L.R = 2.0f * f;
+        if (!(r > 0.0f) || !(L.R >= r) || !std::isfinite(L.R) || !std::isfinite(th)) {
+            ctx->err = "lens radius must be in (0, 2 focalLength]";
+            return PT_ERR_INVALID_ARG;
+        }
+        L.h = L.R - std::sqrt(L.R * L.R - r * r);
+        L.conv = o.isConverging ? 1 : 0;
+        L.sp = o.isConverging ? 0.5f * th + L.h : -0.5f * th;
+        // the caps lie within |x| <= |th| / 2 + h of the centre and within r of the axis
+        const float ax = 0.5f * std::fabs(th) + L.h;
+        L.br2 = ax * ax + r * r;
+        L.mat = o.materialID - 1;
+        L.light = light_of(o.lightID, REF_LENS, (int)lenses.size());
+        if (L.light == -2) { ctx->err = "lens lightID out of range"; return PT_ERR_INVALID_ARG; }
+        lenses.push_back(L);
+    }
+    for (const pt::RCyclide& o : m.cyclides) {
+        ExtCyc C{};
+        C.pos = v3(o.pos);
+        C.M = rotation_deg(v3(o.rotation));
+        if (!(o.scale[0] != 0.0f && o.scale[1] != 0.0f && o.scale[2] != 0.0f) || !(o.brad > 0.0f)) {
+            ctx->err = "cyclide scale must be nonzero and its boundingRadius positive";
+            return PT_ERR_INVALID_ARG;
+        }
+        C.inv_scale = mk3(1.0f / o.scale[0], 1.0f / o.scale[1], 1.0f / o.scale[2]);
+        C.A = o.a; C.B = o.b; C.C = o.c; C.D = o.d;
+        C.brad2 = o.brad;  // the reference's boundingRadius is compared with squared distances
+        C.mat = o.materialID - 1;
+        C.light = light_of(o.lightID, REF_CYC, (int)cycs.size());
+        if (C.light == -2) { ctx->err = "cyclide lightID out of range"; return PT_ERR_INVALID_ARG; }
+        cycs.push_back(C);
+    }
     for (const pt::RPlane& pl : m.planes) {
         if (pl.lightID > 0) {
             ctx->err = "extended mode cannot sample an emissive infinite plane";
@@ -2093,6 +2412,8 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     for (auto& ml : quad_ml) if (bad_mat(ml.x)) { ctx->err = "quad/box materialID out of range"; return PT_ERR_INVALID_ARG; }
     for (auto& ml : plane_ml) if (bad_mat(ml.x)) { ctx->err = "plane materialID out of range"; return PT_ERR_INVALID_ARG; }
     for (auto& e : sdfs) if (bad_mat(e.mat)) { ctx->err = "sdf material out of range"; return PT_ERR_INVALID_ARG; }
+    for (auto& e : lenses) if (bad_mat(e.mat)) { ctx->err = "lens materialID out of range"; return PT_ERR_INVALID_ARG; }
+    for (auto& e : cycs) if (bad_mat(e.mat)) { ctx->err = "cyclide materialID out of range"; return PT_ERR_INVALID_ARG; }
 
     const int width = ctx->cfg.bvh_width ? ctx->cfg.bvh_width : 2;
     // Defaults measured on the 4096-sphere workload: binary nodes (64 B) with one
@@ -2166,7 +2487,7 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     auto sz = [&](size_t b) { bytes += ((b + 255) / 256) * 256; };
     sz(sph.size() * 16); sz(sph_ml.size() * 8); sz(quad.size() * 16); sz(quad_ml.size() * 8);
     sz(plane_y.size() * 4); sz(plane_ml.size() * 8); sz(sdfs.size() * sizeof(ExtSdf)); sz(mats.size() * sizeof(ExtMat));
-    sz(lights.size() * sizeof(ExtLight));
+    sz(lights.size() * sizeof(ExtLight)); sz(lenses.size() * sizeof(ExtLens)); sz(cycs.size() * sizeof(ExtCyc));
     sz(sb.words.size() * 4); sz(sleaf.size() * 16); sz(sids.size() * 4);
     sz(qb.words.size() * 4); sz(qleaf.size() * 16); sz(qids.size() * 4);
     int st = dev_alloc(ctx, h->mem, bytes);
@@ -2189,6 +2510,8 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     D.sdf = (const ExtSdf*)put(sdfs.data(), sdfs.size() * sizeof(ExtSdf));
     D.mat = (const ExtMat*)put(mats.data(), mats.size() * sizeof(ExtMat));
     D.light = (const ExtLight*)put(lights.data(), lights.size() * sizeof(ExtLight));
+    D.lens = (const ExtLens*)put(lenses.data(), lenses.size() * sizeof(ExtLens));
+    D.cyc = (const ExtCyc*)put(cycs.data(), cycs.size() * sizeof(ExtCyc));
     D.sbvh.nodes = (const uint32_t*)put(sb.words.data(), sb.words.size() * 4);
     D.sbvh.prims = (const float4*)put(sleaf.data(), sleaf.size() * 16);
     D.sbvh.ids = (const int*)put(sids.data(), sids.size() * 4);
@@ -2205,6 +2528,9 @@ static int upload_ext(pt_ctx* ctx, pt_scene* s, ExtSceneHost* h) {
     D.nsdf = (int)sdfs.size();
     D.nmat = (int)mats.size();
     D.nlight = (int)lights.size();
+    D.nlight_nee = nlight_nee;
+    D.nlens = (int)lenses.size();
+    D.ncyc = (int)cycs.size();
     D.has_env = m.env_enabled ? 1 : 0;
     D.env = m.env_radiance;
     D.n_lambda = m.render.n_lambda;
@@ -2306,6 +2632,7 @@ struct ExtPlan {
     bool multi = false;
     bool any_sdf = false;
     bool quad_bvh = false;  // a scene tests its parallelograms through a BVH
+    bool extras = false;    // a scene has lenses or cyclides (the k_ext_extras kernels)
     std::vector<int> sdf_kinds;
     int width = 2, stack_need = 1;
     bool small_nodes = true;
@@ -2343,10 +2670,15 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
     // the one-ray-per-lane kernels. With 2^25-path waves it measured 9-10 %
     // faster on the analytic scenes (4096 spheres, Cornell box) and even on SDFs.
     const bool persistent = !MULTI && ctx->cfg.traversal != 2 && !P.quad_bvh;
-    const bool fused = !persistent && !P.any_sdf;
+    // lenses and cyclides are tested by their own kernels between the traversal
+    // and the classification, which the fused kernel does in one
+    const bool fused = !persistent && !P.any_sdf && !P.extras;
     // bounce 0's queue is the identity over the camera rays, which raygen
     // writes by path; later queues have their rays in queue order
     const int rq = b > 0 ? 1 : 0;
+    auto extras = [&]() {
+        if (P.extras) launch(ctx, "k_ext_extras", k_ext_extras<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur, rq);
+    };
     launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1, shadow_cnt(b));
     if (fused) {
         launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur,
@@ -2357,6 +2689,7 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
         else
             launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
                    Wv, cur, rq);
+        extras();
         for (int kind : P.sdf_kinds) {
             switch (kind) {
                 case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, rq, grid); break;
@@ -2367,7 +2700,12 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
         }
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     } else {
-        launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
+        if (persistent)
+            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
+        else
+            launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
+                   Wv, cur, rq);
+        extras();
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     }
     if (b == 0 && P.aov) launch(ctx, "k_ext_aov", k_ext_aov<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv);
@@ -2380,6 +2718,9 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
            b);
     // Shadow rays push children unordered: near-first ordering measured slower
     // on the 4096-sphere scene (most shadow rays are occluded early anyway).
+    if (P.extras)
+        launch(ctx, "k_ext_extras_shadow", k_ext_extras_shadow<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv,
+               shadow_cnt(b));
     if (persistent)
         launch(ctx, "k_ext_shadow", k_ext_shadow_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, shadow_cnt(b));
     else if (!fused)
@@ -2392,7 +2733,7 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
 template <int W, int STACK, typename SE, bool MULTI>
 static void launch_tail(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int nb, int grid) {
     const bool persistent = !MULTI && ctx->cfg.traversal != 2 && !P.quad_bvh;
-    if (persistent || P.any_sdf || nb <= 0) return;
+    if (persistent || P.any_sdf || P.extras || nb <= 0) return;
     launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, nb & 1,
            shadow_cnt(nb - 1), 1);
 }
@@ -2507,6 +2848,7 @@ static int ext_run(pt_ctx* ctx, const pt_scene* s, const pt_push_constants* pc, 
     plan.S0 = H.dev;
     plan.any_sdf = H.dev.nsdf > 0;
     plan.quad_bvh = H.dev.quad_bvh != 0;
+    plan.extras = H.dev.nlens + H.dev.ncyc > 0;
     plan.sdf_kinds = H.sdf_kinds;
     plan.width = H.width;
     plan.stack_need = H.stack_need;
@@ -2592,6 +2934,7 @@ int ext_dispatch_batch(pt_ctx* ctx, const ExtBatchItem* items, int n) {
         plan.small_nodes = plan.small_nodes && H.small_nodes;
         plan.any_sdf = plan.any_sdf || H.dev.nsdf > 0;
         plan.quad_bvh = plan.quad_bvh || H.dev.quad_bvh;
+        plan.extras = plan.extras || H.dev.nlens + H.dev.ncyc > 0;
         for (int k : H.sdf_kinds)
             if (std::find(plan.sdf_kinds.begin(), plan.sdf_kinds.end(), k) == plan.sdf_kinds.end())
                 plan.sdf_kinds.push_back(k);
