@@ -30,16 +30,20 @@ def run_until_done(srv, ids, max_steps=10000):
     return {i: srv.status(i) for i in ids}
 
 
-@pytest.mark.parametrize("kind", ["spheres", "sdf"])
+@pytest.mark.parametrize("kind", ["spheres", "sdf", "shapes"])
 def test_result_is_the_same_alone_and_in_the_mix(kind):
     """Continuous batching: a request's paths share waves with other requests
-    (other scenes, SDF and analytic, other depths and resolutions), in quanta
-    of whatever size the wave leaves; its per-pixel sums are added in sample
-    order, so the image is bit-identical to the request traced alone."""
+    (other scenes, SDF and analytic, with and without lenses and cyclides,
+    other depths and resolutions), in quanta of whatever size the wave leaves;
+    its per-pixel sums are added in sample order, so the image is
+    bit-identical to the request traced alone."""
+    from tests.test_gpu_ext_shapes import shapes_cornell
     text, depth = {"spheres": (scenes.random_spheres(n=256, scene_seed=3, wavelengths=4, seed=11), 8),
-                   "sdf": (scenes.sdf_scene(jitter=0.05, scene_seed=4, seed=12), 6)}[kind]
+                   "sdf": (scenes.sdf_scene(jitter=0.05, scene_seed=4, seed=12), 6),
+                   "shapes": (shapes_cornell(emissive=True), 8)}[kind]
     others = [(scenes.cornell(4, seed=1), 24, 24, 6), (scenes.sdf_scene(seed=2), 32, 16, 5),
-              (scenes.random_spheres(n=64, scene_seed=9, wavelengths=1, seed=3), 40, 20, 12)]
+              (scenes.random_spheres(n=64, scene_seed=9, wavelengths=1, seed=3), 40, 20, 12),
+              (shapes_cornell(glass_lens=True), 16, 16, 7)]
     imgs = []
     # graph 0: each wave shape captured once into a hipGraph and replayed every
     # step with new tables; graph 1: kernels launched one by one
