@@ -724,36 +724,44 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
 // `it` counts the iterations executed (the loop leaves early past the bailout).
 // One iteration of the mandelbulb's power-8 map, z <- z^8 + p in spherical form
 // (the angles' multiples by Chebyshev polynomials of their cosines), with the
-// running derivative dr <- 8 r^7 dr + 1; r becomes |z|.
-// Its multiply-adds are fused (written out, as this file is built without
-// contraction): the march is bound by VALU issue.
-__device__ __forceinline__ void mandelbulb_iter(f3& z, float& r, float& dr, f3 p) {
-    float r2 = r * r, r4 = r2 * r2;
+// running derivative dr <- 8 r^7 dr + 1; r2 = |z|^2 before and after. The
+// march is bound by VALU issue, so the iteration is written for it: its
+// multiply-adds are fused (this file is built without contraction), and it
+// takes two reciprocal square roots, 1/r = rsq(r2) (then r = r2 * (1/r)) and
+// 1/l = rsq(x^2 + z^2) with l = sqrt(x^2 + z^2) the polar radius, so that
+// sin(phi) = l / r = (x^2 + z^2) * (1/l) * (1/r): a square root and two
+// reciprocals would be four quarter-rate instructions.
+__device__ __forceinline__ void mandelbulb_iter(f3& z, float& r2, float& dr, f3 p) {
+    const float invr = hw_rsq(r2), r = r2 * invr;
+    const float r4 = r2 * r2;
     dr = fmaf(8.0f * (r4 * r2 * r), dr, 1.0f);
-    float invr = hw_rcp(r);
-    float invl = hw_rsq(fmaf(z.x, z.x, z.z * z.z));
-    float r8 = r4 * r4;
-    float cost = z.z * invl, cosp = z.y * invr, sint = z.x * invl, sinp = invr * hw_rcp(invl);
-    float ct2 = cost * cost, cp2 = cosp * cosp;
-    float cost8 = fmaf(fmaf(fmaf(fmaf(128.0f, ct2, -256.0f), ct2, 160.0f), ct2, -32.0f), ct2, 1.0f);
-    float cosp8 = fmaf(fmaf(fmaf(fmaf(128.0f, cp2, -256.0f), cp2, 160.0f), cp2, -32.0f), cp2, 1.0f);
-    float sint8 = fmaf(fmaf(fmaf(128.0f, ct2, -192.0f), ct2, 80.0f), ct2, -8.0f) * sint * cost;
-    float sinp8 = fmaf(fmaf(fmaf(128.0f, cp2, -192.0f), cp2, 80.0f), cp2, -8.0f) * sinp * cosp;
+    const float l2 = fmaf(z.x, z.x, z.z * z.z);
+    const float invl = hw_rsq(l2);
+    const float r8 = r4 * r4;
+    const float cost = z.z * invl, cosp = z.y * invr, sint = z.x * invl, sinp = invr * (l2 * invl);
+    const float ct2 = cost * cost, cp2 = cosp * cosp;
+    const float cost8 = fmaf(fmaf(fmaf(fmaf(128.0f, ct2, -256.0f), ct2, 160.0f), ct2, -32.0f), ct2, 1.0f);
+    const float cosp8 = fmaf(fmaf(fmaf(fmaf(128.0f, cp2, -256.0f), cp2, 160.0f), cp2, -32.0f), cp2, 1.0f);
+    const float sint8 = fmaf(fmaf(fmaf(128.0f, ct2, -192.0f), ct2, 80.0f), ct2, -8.0f) * sint * cost;
+    const float sinp8 = fmaf(fmaf(fmaf(128.0f, cp2, -192.0f), cp2, 80.0f), cp2, -8.0f) * sinp * cosp;
     z = mk3(fmaf(r8, sint8 * sinp8, p.x), fmaf(r8, cosp8, p.y), fmaf(r8, cost8 * sinp8, p.z));
-    r = hw_sqrt(fdot(z, z));
+    r2 = fdot(z, z);
 }
-// The distance estimate from the map's last iterate.
-__device__ __forceinline__ float mandelbulb_dist(float r, float dr) { return 0.5f * __logf(r) * r * hw_rcp(dr); }
+// The distance estimate 0.5 log(r) r / dr from the map's last iterate (r^2 = r2).
+__device__ __forceinline__ float mandelbulb_dist(float r2, float dr) {
+    return 0.25f * __logf(r2) * hw_sqrt(r2) * hw_rcp(dr);
+}
 __device__ __forceinline__ float ext_mandelbulb(f3 p, int iters, float bailout, unsigned& it) {
     f3 z = p;
-    float r = hw_sqrt(fdot(z, z));
+    float r2 = fdot(z, z);
     float dr = 1.0f;
+    const float bail2 = bailout * bailout;
     for (int i = 0; i < iters; ++i) {
         ++it;
-        mandelbulb_iter(z, r, dr, p);
-        if (r > bailout) break;
+        mandelbulb_iter(z, r2, dr, p);
+        if (r2 > bail2) break;
     }
-    return mandelbulb_dist(r, dr);
+    return mandelbulb_dist(r2, dr);
 }
 __device__ __forceinline__ float ext_menger(f3 p, int levels) {
     f3 q = abs3(p) - mk3(1.0f, 1.0f, 1.0f);
