@@ -329,13 +329,13 @@ __device__ __forceinline__ bool isect_sphere(float4 s, f3 o, f3 d, float tmax, f
     return false;
 }
 __device__ __forceinline__ bool isect_quad(const float4* q, f3 o, f3 d, float tmax, float& t) {
-    float4 q0 = q[0], q1 = q[1];
+    const float4 q0 = q[0], q1 = q[1];
     f3 w = mk3(q1.x, q1.y, q1.z);
     float den = fdot(w, d);
     if (fabsf(den) < 1e-20f) return false;
     float tt = (q0.w - fdot(w, o)) * hw_rcp(den);
     if (!(tt > kEps && tt < tmax)) return false;
-    float4 q2 = q[2], q3 = q[3];
+    const float4 q2 = q[2], q3 = q[3];
     f3 hp = mk3(fmaf(d.x, tt, o.x) - q0.x, fmaf(d.y, tt, o.y) - q0.y, fmaf(d.z, tt, o.z) - q0.z);
     float a = fdot(mk3(q2.x, q2.y, q2.z), hp);
     float b = fdot(mk3(q3.x, q3.y, q3.z), hp);
@@ -523,7 +523,7 @@ __device__ __forceinline__ f3 cyclide_normal(const ExtCyc& C, f3 p) {
 template <bool ANY>
 __device__ __forceinline__ bool trace_extras(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref) {
     bool hit = false;
-    for (int i = 0; i < S.nlens; ++i) {  // uniform index: scalar loads
+    for (int i = 0; i < S.nlens; ++i) {  // uniform index
         float tl;
         if (isect_lens(S.lens[i], o, d, t, tl)) {
             t = tl;
@@ -688,7 +688,7 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
             if (ANY) return true;
         }
     } else {
-        for (int i = 0; i < S.nquad; ++i) {  // uniform index: scalar loads
+        for (int i = 0; i < S.nquad; ++i) {  // uniform index
             float tq;
             if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
                 t = tq;
@@ -707,7 +707,7 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
         }
     }
     for (int i = 0; i < S.nplane; ++i) {
-        float tp = (S.plane_y[i] - o.y) / d.y;
+        float tp = (ld_ro(S.plane_y + i) - o.y) / d.y;
         if (tp > kEps && tp < t) {
             t = tp;
             ref = (REF_PLANE << kRefShift) | (uint32_t)i;
@@ -1223,6 +1223,18 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
 // segments, so the blocks of one XCD (blockIdx.x mod 8) share a few segments.
 constexpr int kRefill = 16;
 
+// The scene's parallelograms staged in LDS by the block (at most kQuadBrute:
+// the host launches the persistent kernels only for scenes without a
+// parallelogram BVH). Each refill tests them all; from LDS their loads are
+// broadcasts, where from global memory each parallelogram was a chain of
+// dependent vector loads on the refill's critical path.
+__device__ __forceinline__ int stage_quads(const ExtSceneDev& S, float4* s_quad) {
+    const int nq = S.nquad < kQuadBrute ? S.nquad : kQuadBrute;
+    for (int i = (int)threadIdx.x; i < 4 * nq; i += kBlk) s_quad[i] = S.quad[i];
+    __syncthreads();
+    return nq;
+}
+
 // Persistent closest-hit kernel (parallelograms tested directly, spheres through
 // their BVH, then planes); see the persistent traversal notes above. Writes the
 // hits only: k_ext_classify appends the paths to the material queues (one
@@ -1231,7 +1243,9 @@ constexpr int kRefill = 16;
 template <int W, int STACK, typename SE>
 __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur, int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
+    __shared__ float4 s_quad[4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
+    const int nq = stage_quads(S, s_quad);
     const int n = *qlen(Wv, cur);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
     const int* q = Wv.q[cur];
@@ -1251,9 +1265,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
                 t = 1e30f;
                 ref = kMiss;
                 best = -1;
-                for (int i = 0; i < S.nquad; ++i) {  // uniform index: scalar loads
+                for (int i = 0; i < nq; ++i) {
                     float tq;
-                    if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
+                    if (isect_quad(s_quad + 4 * i, o, d, t, tq)) {
                         t = tq;
                         ref = (REF_QUAD << kRefShift) | (uint32_t)i;
                     }
@@ -1274,7 +1288,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
         if (p >= 0 && node < 0) {
             if (best >= 0) ref = (REF_SPHERE << kRefShift) | (uint32_t)S.sbvh.ids[best];
             for (int i = 0; i < S.nplane; ++i) {
-                float tp = (S.plane_y[i] - o.y) / d.y;
+                float tp = (ld_ro(S.plane_y + i) - o.y) / d.y;
                 if (tp > kEps && tp < t) {
                     t = tp;
                     ref = (REF_PLANE << kRefShift) | (uint32_t)i;
@@ -1965,7 +1979,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B,
 template <int W, int STACK, typename SE>
 __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave Wv, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
+    __shared__ float4 s_quad[4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
+    const int nq = stage_quads(S, s_quad);
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0, exhausted = 0;
@@ -1988,9 +2004,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
                 t = so.w;
                 path = __float_as_int(sd.w);
                 blocked = false;
-                for (int k = 0; k < S.nquad; ++k) {  // uniform index: scalar loads
+                for (int k = 0; k < nq; ++k) {
                     float tq;
-                    if (isect_quad(S.quad + 4 * k, o, d, t, tq)) {
+                    if (isect_quad(s_quad + 4 * k, o, d, t, tq)) {
                         blocked = true;
                         break;
                     }
@@ -2013,7 +2029,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
         }
         if (i >= 0 && node < 0) {
             for (int k = 0; !blocked && k < S.nplane; ++k) {
-                float tp = (S.plane_y[k] - o.y) / d.y;
+                float tp = (ld_ro(S.plane_y + k) - o.y) / d.y;
                 blocked = tp > kEps && tp < t;
             }
             for (int s = 0; !blocked && s < S.nsdf; ++s)

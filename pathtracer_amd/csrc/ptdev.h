@@ -98,6 +98,17 @@ PT_HD float rand_chain(uint32_t& seed) {
     return (float)seed * 2.3283064365386963e-10f;  // exact: division by 2^32
 }
 
+// A load through the constant address space: it marks the memory read-only
+// for the kernel's duration (the scene tables are), which a generic pointer
+// into memory the kernel also writes cannot show the compiler, so a uniform
+// address becomes a scalar load through the scalar cache instead of a vector
+// load with its full memory latency. Divergent addresses still give vector
+// loads.
+template <typename T>
+__device__ __forceinline__ T ld_ro(const T* p) {
+    return *(const __attribute__((address_space(4))) T*)p;
+}
+
 // Appends `pred` lanes of the calling wave to a queue whose length lives at
 // *counter: one atomic per wave (ballot + popcount), slot order = lane order.
 // Returns the slot for a predicated lane, -1 otherwise. Every active lane of
