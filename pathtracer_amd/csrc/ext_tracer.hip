@@ -1221,7 +1221,11 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
 // segment (chunk = n / waves in the grid, clamped to [1, kChunkMax]) and hands
 // them to its lanes from registers. Consecutive waves have different home
 // segments, so the blocks of one XCD (blockIdx.x mod 8) share a few segments.
-constexpr int kRefill = 16;
+// Half the wave: on the 4096-sphere scene 32 measured 4.36 G segments/s, 16
+// and 48 4.30 G, 8 4.15 G, 4 3.96 G (a refill's ray loads and parallelogram
+// tests run at the width of the lanes it fills); the Cornell box and the SDF
+// scene measured the same for 16, 32 and 48.
+constexpr int kRefill = 32;
 
 // The scene's parallelograms staged in LDS by the block (at most kQuadBrute:
 // the host launches the persistent kernels only for scenes without a
