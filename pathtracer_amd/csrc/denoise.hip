@@ -74,6 +74,13 @@ struct SliceRange {
 // output channels [0, Co1) go to Y1 (Co1 channels) and [Co1, Cout) to Y2;
 // bias and ReLU are applied at the store. `flip` reads tap 8 - t
 // (backward-data).
+// Pixel index of (n, y, x) of an H x W tensor (q), or with `up`, of the
+// (H / 2) x (W / 2) tensor that a nearest-neighbour 2x upsample turns into it:
+// the skip layers read their upsampled input this way instead of a stored copy.
+__device__ __forceinline__ size_t up_index(int up, size_t q, int n, int y, int x, int H, int W) {
+    return up ? ((size_t)n * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1) : q;
+}
+
 struct ConvArgs {
     const u16* X1;
     const u16* X2;
@@ -87,6 +94,7 @@ struct ConvArgs {
     int relu;
     int flip;
     int tiles_per_block;  // consecutive pixel tiles a block computes in turn
+    int up1;              // X1 is at half resolution, upsampled 2x (nearest) as it is read
 };
 
 // ---------------------------------------------------------------------------------
@@ -162,7 +170,8 @@ __global__ void __launch_bounds__(kThreads, NC == 2 ? 2 : 3) k_conv_t(ConvArgs a
             uint4 val = make_uint4(0, 0, 0, 0);
             if (i < NXV && y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
                 const size_t q = ((size_t)n * a.H + y) * a.W + x;
-                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
+                const u16* src = c < a.C1 ? a.X1 + up_index(a.up1, q, n, y, x, a.H, a.W) * a.C1 + c
+                                          : a.X2 + q * a.C2 + (c - a.C1);
                 val = *reinterpret_cast<const uint4*>(src);
             }
             xr[u] = val;
@@ -328,6 +337,7 @@ struct WgradArgs {
     float* dB;   // [Cout]
     int N, H, W;
     int chunk;   // pixels per block (multiple of 64)
+    int up1;     // X1 at half resolution, as in ConvArgs
 };
 
 __global__ void __launch_bounds__(kThreads) k_conv_wgrad(WgradArgs a) {
@@ -357,7 +367,8 @@ __global__ void __launch_bounds__(kThreads) k_conv_wgrad(WgradArgs a) {
             const int c = ci0 + lc;
             if (y >= 0 && y < a.H && x >= 0 && x < a.W && c < Cin) {
                 const size_t q = ((size_t)n * a.H + y) * a.W + x;
-                const u16* src = c < a.C1 ? a.X1 + q * a.C1 + c : a.X2 + q * a.C2 + (c - a.C1);
+                const u16* src = c < a.C1 ? a.X1 + up_index(a.up1, q, n, y, x, a.H, a.W) * a.C1 + c
+                                          : a.X2 + q * a.C2 + (c - a.C1);
                 xv.u = *reinterpret_cast<const uint4*>(src);
             }
         }
@@ -437,20 +448,8 @@ __global__ void k_pool2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) 
         dxv[i] = make_uint4(scale2(v.x, 0.25f), scale2(v.y, 0.25f), scale2(v.z, 0.25f), scale2(v.w, 0.25f));
     }
 }
-// nearest 2x upsample: y [N, 2H, 2W, C] from x [N, H, W, C]
-__global__ void k_up2(const u16* x, u16* y, int N, int H, int W, int C) {
-    const uint32_t C8 = C >> 3, Ho = 2 * H, Wo = 2 * W;
-    const uint32_t n = (uint32_t)N * Ho * Wo * C8;
-    const uint4* xv = reinterpret_cast<const uint4*>(x);
-    uint4* yv = reinterpret_cast<uint4*>(y);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t g = i % C8, r = i / C8;
-        const uint32_t xo = r % Wo, r2 = r / Wo;
-        const uint32_t yo = r2 % Ho, b = r2 / Ho;
-        yv[i] = xv[((b * (uint32_t)H + yo / 2) * (uint32_t)W + xo / 2) * C8 + g];
-    }
-}
-// gradient of the upsample: dX = sum of the 4 dY it fed
+// gradient of the nearest 2x upsample the skip layers apply to their first
+// input as they read it (up_index): dX = sum of the 4 dY it fed
 __global__ void k_up2_bwd(const u16* dy, u16* dx, int N, int H, int W, int C) {
     const uint32_t C8 = C >> 3;
     const uint32_t n = (uint32_t)N * H * W * C8;
@@ -629,6 +628,7 @@ __global__ void k_dn_scale(const float* den, float scale, float* out, int n) {
 struct Layer {
     int cin1, cin2, cout;  // input split (concat), output channels
     bool relu;
+    bool up1;              // the first input is the half-resolution decoder tensor, upsampled as it is read
     size_t w_off, b_off;   // into the FP32 parameter vector
     u16 *wf = nullptr, *wb = nullptr;
 };
@@ -650,8 +650,7 @@ struct pt_denoiser {
     u16* wbf = nullptr;  // BF16 copies, 2 per weight
     // activations (BF16) and gradients
     u16 *x0 = nullptr, *e0a = nullptr, *e0 = nullptr, *p1 = nullptr, *e1a = nullptr, *e1 = nullptr, *p2 = nullptr,
-        *ba = nullptr, *bb = nullptr, *u1 = nullptr, *d1a = nullptr, *d1 = nullptr, *u0 = nullptr, *d0a = nullptr,
-        *d0 = nullptr, *out = nullptr;
+        *ba = nullptr, *bb = nullptr, *d1a = nullptr, *d1 = nullptr, *d0a = nullptr, *d0 = nullptr, *out = nullptr;
     u16 *ga = nullptr, *gb = nullptr, *gc = nullptr, *gs1 = nullptr, *gs0 = nullptr;  // gradient scratch
     float *feat = nullptr, *tgt = nullptr, *den = nullptr, *loss = nullptr;
     std::string err;
@@ -672,6 +671,7 @@ void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, 
     a.W = W;
     if (!bwd_data) {
         a.X1 = x1; a.X2 = x2; a.C1 = l.cin1; a.C2 = l.cin2;
+        a.up1 = l.up1 ? 1 : 0;
         a.Wt = l.wf; a.bias = d->w + l.b_off;
         a.Y1 = y; a.Y2 = nullptr; a.Co1 = l.cout; a.Cout = l.cout;
         a.relu = l.relu; a.flip = 0;
@@ -705,6 +705,7 @@ void conv(pt_denoiser* d, const Layer& l, const u16* x1, const u16* x2, u16* y, 
 void wgrad(pt_denoiser* d, const Layer& l, const u16* dz, const u16* x1, const u16* x2, int N, int H, int W) {
     WgradArgs a{};
     a.dZ = dz; a.Cout = l.cout; a.X1 = x1; a.X2 = x2; a.C1 = l.cin1; a.C2 = l.cin2;
+    a.up1 = l.up1 ? 1 : 0;
     a.dW = d->g + l.w_off; a.dB = d->g + l.b_off;
     a.N = N; a.H = H; a.W = W;
     const int P = N * H * W;
@@ -731,11 +732,9 @@ void forward(pt_denoiser* d, int N, int H, int W) {
     ptk::launch(ctx, "k_dn_pool", k_pool2, ew_grid((size_t)P2 * 64 / 8), dim3(256), (const u16*)d->e1, d->p2, N, H1, W1, 64);
     conv(d, d->L[4], d->p2, nullptr, d->ba, N, H2, W2, false);
     conv(d, d->L[5], d->ba, nullptr, d->bb, N, H2, W2, false);
-    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P1 * 128 / 8), dim3(256), (const u16*)d->bb, d->u1, N, H2, W2, 128);
-    conv(d, d->L[6], d->u1, d->e1, d->d1a, N, H1, W1, false);
+    conv(d, d->L[6], d->bb, d->e1, d->d1a, N, H1, W1, false);  // bb upsampled as it is read
     conv(d, d->L[7], d->d1a, nullptr, d->d1, N, H1, W1, false);
-    ptk::launch(ctx, "k_dn_up", k_up2, ew_grid((size_t)P * 64 / 8), dim3(256), (const u16*)d->d1, d->u0, N, H1, W1, 64);
-    conv(d, d->L[8], d->u0, d->e0, d->d0a, N, H, W, false);
+    conv(d, d->L[8], d->d1, d->e0, d->d0a, N, H, W, false);    // d1 upsampled as it is read
     conv(d, d->L[9], d->d0a, nullptr, d->d0, N, H, W, false);
     conv(d, d->L[10], d->d0, nullptr, d->out, N, H, W, false);
 }
@@ -755,16 +754,16 @@ void backward(pt_denoiser* d, int N, int H, int W) {
     wgrad(d, d->L[9], d->gb, d->d0a, nullptr, N, H, W);
     conv(d, d->L[9], d->gb, nullptr, d->ga, N, H, W, true);           // dL/d d0a (32)
     relu_bwd(d->ga, nullptr, d->d0a, d->gb, (size_t)P * 32);           // dZ8
-    wgrad(d, d->L[8], d->gb, d->u0, d->e0, N, H, W);
-    conv(d, d->L[8], d->gb, nullptr, d->gc, N, H, W, true, d->gs0, 64);  // dL/d u0 (64) | dL/d e0 via skip (32)
+    wgrad(d, d->L[8], d->gb, d->d1, d->e0, N, H, W);
+    conv(d, d->L[8], d->gb, nullptr, d->gc, N, H, W, true, d->gs0, 64);  // dL/d up(d1) (64) | dL/d e0 via skip (32)
     ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P1 * 64 / 8), dim3(256), (const u16*)d->gc, d->ga, N, H1, W1,
                 64);                                                   // dL/d d1 (64)
     relu_bwd(d->ga, nullptr, d->d1, d->gb, (size_t)P1 * 64);           // dZ7
     wgrad(d, d->L[7], d->gb, d->d1a, nullptr, N, H1, W1);
     conv(d, d->L[7], d->gb, nullptr, d->ga, N, H1, W1, true);         // dL/d d1a (64)
     relu_bwd(d->ga, nullptr, d->d1a, d->gb, (size_t)P1 * 64);          // dZ6
-    wgrad(d, d->L[6], d->gb, d->u1, d->e1, N, H1, W1);
-    conv(d, d->L[6], d->gb, nullptr, d->gc, N, H1, W1, true, d->gs1, 128);  // dL/d u1 (128) | dL/d e1 via skip (64)
+    wgrad(d, d->L[6], d->gb, d->bb, d->e1, N, H1, W1);
+    conv(d, d->L[6], d->gb, nullptr, d->gc, N, H1, W1, true, d->gs1, 128);  // dL/d up(bb) (128) | dL/d e1 via skip (64)
     ptk::launch(ctx, "k_dn_up_bwd", k_up2_bwd, ew_grid((size_t)P2 * 128 / 8), dim3(256), (const u16*)d->gc, d->ga, N, H2,
                 W2, 128);                                              // dL/d bb (128)
     relu_bwd(d->ga, nullptr, d->bb, d->gb, (size_t)P2 * 128);          // dZ5
@@ -928,6 +927,7 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     for (auto& sp : spec) {
         Layer l;
         l.cin1 = sp[0]; l.cin2 = sp[1]; l.cout = sp[2]; l.relu = sp[3] != 0;
+        l.up1 = l.cin2 > 0;  // the decoder layers that concatenate a skip connection
         l.w_off = off;
         off += (size_t)9 * l.cout * (l.cin1 + l.cin2);
         l.b_off = off;
@@ -947,7 +947,7 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     const size_t P = (size_t)max_batch * max_height * max_width;
     // activations: BF16 element counts
     const size_t act[] = {P * kIn, P * 32, P * 32, P / 4 * 32, P / 4 * 64, P / 4 * 64, P / 16 * 64, P / 16 * 128,
-                          P / 16 * 128, P / 4 * 128, P / 4 * 64, P / 4 * 64, P * 64, P * 32, P * 32, P * kOutC,
+                          P / 16 * 128, P / 4 * 64, P / 4 * 64, P * 32, P * 32, P * kOutC,
                           P * 64, P * 64, P * 64, P / 4 * 64, P * 32};
     size_t bytes = 4 * off * 4 + 2 * off * 2 + (P * kFeat + P * 3 + P * 3 + 64) * 4 + 64 * 256;
     for (size_t a : act) bytes += (a * 2 + 255) / 256 * 256;
@@ -967,8 +967,9 @@ pt_status pt_dn_create(pt_ctx* ctx, int max_batch, int max_height, int max_width
     d->m = reinterpret_cast<float*>(carve(off * 4));
     d->v = reinterpret_cast<float*>(carve(off * 4));
     d->wbf = reinterpret_cast<u16*>(carve(off * 4));
-    u16** acts[] = {&d->x0, &d->e0a, &d->e0, &d->p1, &d->e1a, &d->e1, &d->p2, &d->ba, &d->bb, &d->u1, &d->d1a, &d->d1,
-                    &d->u0, &d->d0a, &d->d0, &d->out, &d->ga, &d->gb, &d->gc, &d->gs1, &d->gs0};
+    u16** acts[] = {&d->x0, &d->e0a, &d->e0, &d->p1, &d->e1a, &d->e1, &d->p2, &d->ba, &d->bb, &d->d1a, &d->d1,
+                    &d->d0a, &d->d0, &d->out, &d->ga, &d->gb, &d->gc, &d->gs1, &d->gs0};
+    static_assert(sizeof(act) / sizeof(act[0]) == sizeof(acts) / sizeof(acts[0]), "one size per activation");
     for (size_t i = 0; i < sizeof(act) / sizeof(act[0]); ++i) *acts[i] = reinterpret_cast<u16*>(carve(act[i] * 2));
     d->feat = reinterpret_cast<float*>(carve(P * kFeat * 4));
     d->tgt = reinterpret_cast<float*>(carve(P * 3 * 4));
