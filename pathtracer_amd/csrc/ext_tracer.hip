@@ -344,6 +344,33 @@ __device__ __forceinline__ bool isect_quad(const float4* q, f3 o, f3 d, float tm
     return true;
 }
 
+// Stages the flat parallelogram list (quads, nq of them; nq = 0 for a list
+// with its own BVH) that the calling wave's active lanes test: when all of
+// them trace the same scene (always with one scene; in a batched wave, when
+// its lanes hold one request's paths), the wave loads the list into its LDS
+// slice, one 16-byte vector per lane (64 lanes: up to kQuadBrute
+// parallelograms), and returns true (the same on every lane); otherwise each
+// lane reads its own scene's list from global memory. Every lane of the wave
+// calls it (the caller's loop is uniform).
+__device__ __forceinline__ bool wave_quads(const float4* quads, int nq, bool active, float4* slice) {
+    static_assert(4 * kQuadBrute <= 64, "one vector per lane");
+    const uint64_t act = __ballot(active);
+    if (act == 0ull) return false;
+    const int first = __ffsll((unsigned long long)act) - 1;
+    const uint64_t qa = (uint64_t)(uintptr_t)quads;
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)qa, first), hi = (uint32_t)__shfl((int)(uint32_t)(qa >> 32), first);
+    const float4* q0 = reinterpret_cast<const float4*>((uintptr_t)(((uint64_t)hi << 32) | lo));
+    const int n0 = __shfl(nq, first);
+    if (__ballot(active && (quads != q0 || nq != n0)) != 0ull || n0 > kQuadBrute) return false;
+    const int lane = (int)__lane_id();
+    if (lane < 4 * n0) slice[lane] = q0[lane];
+    // the lanes' LDS writes before any lane's reads (one wave: LDS operations in order)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return true;
+}
+
 // The lens's nearest cap point in (kEps, tmax) (struct ExtLens). Each cap's
 // sphere is solved like isect_sphere; a root counts when it lies on the cap.
 __device__ __forceinline__ bool isect_lens(const ExtLens& L, f3 o, f3 d, float tmax, float& t) {
@@ -677,8 +704,10 @@ __device__ __forceinline__ int bvh_trace(const BvhDev& B, f3 o, f3 d, float& tbe
 
 // Closest (or any, with ANY) hit over parallelograms, spheres and planes.
 template <int W, int STACK, typename SE, bool ANY>
-__device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, float& t, uint32_t& ref, SE* stack,
-                                            unsigned& visits) {
+// `staged`: the scene's flat parallelogram list is in the wave's LDS slice
+// (wave_quads), else it is read from global memory.
+__device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, bool staged, const float4* slice, f3 o, f3 d,
+                                            float& t, uint32_t& ref, SE* stack, unsigned& visits) {
     bool hit = false;
     if (S.quad_bvh) {
         int k = bvh_trace<W, STACK, SE, ANY, true>(S.qbvh, o, d, t, stack, visits);
@@ -688,13 +717,27 @@ __device__ __forceinline__ bool scene_trace(const ExtSceneDev& S, f3 o, f3 d, fl
             if (ANY) return true;
         }
     } else {
-        for (int i = 0; i < S.nquad; ++i) {  // uniform index
-            float tq;
-            if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
-                t = tq;
-                ref = (REF_QUAD << kRefShift) | (uint32_t)i;
-                hit = true;
-                if (ANY) return true;
+        // one loop per source (a branch uniform over the wave): a pointer that
+        // may be either would make every load a generic one
+        if (staged) {
+            for (int i = 0; i < S.nquad; ++i) {
+                float tq;
+                if (isect_quad(slice + 4 * i, o, d, t, tq)) {
+                    t = tq;
+                    ref = (REF_QUAD << kRefShift) | (uint32_t)i;
+                    hit = true;
+                    if (ANY) return true;
+                }
+            }
+        } else {
+            for (int i = 0; i < S.nquad; ++i) {
+                float tq;
+                if (isect_quad(S.quad + 4 * i, o, d, t, tq)) {
+                    t = tq;
+                    ref = (REF_QUAD << kRefShift) | (uint32_t)i;
+                    hit = true;
+                    if (ANY) return true;
+                }
             }
         }
     }
@@ -1184,6 +1227,7 @@ __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& W
 template <int W, int STACK, typename SE, bool CLASSIFY, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
+    __shared__ float4 s_wq[kBlk / 64][4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, cur);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
@@ -1195,12 +1239,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
         bool active = idx < n;
         int p = active ? Wv.q[cur][idx] : 0;
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
+        const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, active, s_wq[threadIdx.x >> 6]);
         uint32_t ref = kMiss;
         if (active) {
             f3 o, d;
             ray_of(Wv, cur, rq, idx, p, o, d);
             float t = 1e30f;
-            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, visits);
+            scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, visits);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
             if (S.nsdf > 0) Wv.smask[p] = sdf_mask(S, o, d, t);
         }
@@ -1318,6 +1363,7 @@ template <int W, int STACK, typename SE, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq,
                                                      int rq) {
     __shared__ SE stack_mem[STACK * kBlk];
+    __shared__ float4 s_wq[kBlk / 64][4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
     const int ne = *qlen(Wv, cur), ns = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1330,26 +1376,33 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
     unsigned ev = 0, sv = 0;  // nodes visited by extension and shadow rays
     for (int r = 0; r < rounds; ++r) {
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-        const bool ext = idx < ne;
+        const bool ext = idx < ne, shadow = !ext && idx < n;
         int p = 0;
-        uint32_t ref = kMiss, rid = 0u;
+        uint32_t ref = kMiss, rid = 0u, srid = 0u;
+        float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sd = so, sx = so;
         if (ext) {
             p = Wv.q[cur][idx];
             if (MULTI) rid = Wv.aux[p].w;
-            const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
+        } else if (shadow) {
+            const int i = idx - ne;
+            so = Wv.s_org[i];
+            sd = Wv.s_dir[i];
+            sx = Wv.s_xyz[i];
+            if (MULTI) srid = __float_as_uint(sx.w);
+        }
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, ext ? rid : srid);
+        const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, ext || shadow, s_wq[threadIdx.x >> 6]);
+        if (ext) {
             f3 o, d;
             ray_of(Wv, cur, rq, idx, p, o, d);
             float t = 1e30f;
-            scene_trace<W, STACK, SE, false>(S, o, d, t, ref, stack, ev);
+            scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, ev);
             Wv.hit[p] = make_float2(t, __uint_as_float(ref));
-        } else if (idx < n) {
-            const int i = idx - ne;
-            float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
-            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
+        } else if (shadow) {
             f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
             float t = so.w;
             uint32_t sref = kMiss;
-            if (!scene_trace<W, STACK, SE, true>(S, o, d, t, sref, stack, sv))
+            if (!scene_trace<W, STACK, SE, true>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, sref, stack, sv))
                 add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
         }
         classify_append(scene_of<MULTI>(S0, B, rid), Wv, ext, p, ref);
@@ -1955,18 +2008,30 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
 template <int W, int STACK, typename SE, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_shadow(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
+    __shared__ float4 s_wq[kBlk / 64][4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
     const int n = *qlen(Wv, shq);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[1], (unsigned long long)n);
     unsigned steps = 0, visits = 0, exhausted = 0;
     unsigned long long flops = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 so = Wv.s_org[i], sd = Wv.s_dir[i], sx = Wv.s_xyz[i];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? __float_as_uint(sx.w) : 0u);
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {  // uniform rounds: every lane reaches wave_quads
+        const int i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        const bool active = i < n;
+        float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sd = so, sx = so;
+        if (active) {
+            so = Wv.s_org[i];
+            sd = Wv.s_dir[i];
+            sx = Wv.s_xyz[i];
+        }
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? __float_as_uint(sx.w) : 0u);
+        const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, active, s_wq[threadIdx.x >> 6]);
+        if (!active) continue;
         f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
         float t = so.w;
         uint32_t ref = kMiss;
-        bool blocked = scene_trace<W, STACK, SE, true>(S, o, d, t, ref, stack, visits);
+        bool blocked = scene_trace<W, STACK, SE, true>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, visits);
         for (int s = 0; !blocked && s < S.nsdf; ++s)
             blocked = ext_march<0>(S.sdf[s], o, d, t, S.sdf_steps_max, steps, exhausted, flops) > 0.0f;
         if (!blocked) add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
