@@ -146,13 +146,22 @@ struct ExtFrame {
     int p0, pc, k0, m;
 };
 
-// SoA state of one extended wave, in 16-byte records.
+// A path's state that the shading kernels read and write, in one 64-byte
+// record (one memory sector): they reach paths through the material queues,
+// whose path indices grow sparse as paths end, so every field in an array of
+// its own cost a sector of its own per path.
+struct PathRec {
+    float4 org;  // ray origin; w: hero wavelength
+    float4 dir;  // ray direction; w unused
+    float4 T;    // throughput per wavelength
+    uint4 aux;   // x: RNG key, y: flags (bit 0 companions dropped, bit 1 last bounce specular), z: BSDF pdf bits,
+                 // w: request slot
+};
+
+// State of one extended wave.
 struct ExtWave {
-    float4* org;   // ray origin, w unused
-    float4* dir;   // ray direction
-    float4* T;     // throughput per wavelength
-    float4* acc;   // XYZ gathered so far, w = hero wavelength
-    uint4* aux;    // x: RNG key, y: flags (bit 0 companions dropped, bit 1 last bounce specular), z: BSDF pdf bits
+    PathRec* pr;   // by path index
+    float4* acc;   // XYZ gathered so far (w unused)
     float2* hit;   // (t, primitive reference bits)
     int* q[2];
     // the extension rays of queue c in queue order (entry e: the ray of path
@@ -199,7 +208,7 @@ __device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int rq, int e
         o = mk3(a.x, a.y, a.z);
         d = mk3(b.x, b.y, b.z);
     } else {
-        const float4 a = Wv.org[p], b = Wv.dir[p];
+        const float4 a = Wv.pr[p].org, b = Wv.pr[p].dir;
         o = mk3(a.x, a.y, a.z);
         d = mk3(b.x, b.y, b.z);
     }
@@ -1050,6 +1059,8 @@ __device__ __forceinline__ f3 to_xyz(const ExtSceneDev& S, const float l[4], f4 
     if (S.n_lambda > 1) r = r + cmf(l[1]) * c.y + cmf(l[2]) * c.z + cmf(l[3]) * c.w;
     return r * w;
 }
+__device__ __forceinline__ f3 xyz(float4 v) { return mk3(v.x, v.y, v.z); }
+__device__ __forceinline__ f4 f4of(float4 v) { return f4{v.x, v.y, v.z, v.w}; }
 __device__ __forceinline__ f4 ld4(const float4* a, int p) {
     float4 v = a[p];
     return f4{v.x, v.y, v.z, v.w};
@@ -1165,11 +1176,11 @@ __device__ __forceinline__ void gen_ray(const ExtSceneDev& S, const ExtFrame& F,
             d = refract(d, nb, n12);
         }
     }
-    Wv.org[i] = make_float4(o.x, o.y, o.z, 0.0f);
-    Wv.dir[i] = make_float4(d.x, d.y, d.z, 0.0f);
-    Wv.T[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, l0);
-    Wv.aux[i] = make_uint4(key, 2u, 0u, rid);  // the camera ray counts as specular for emitter MIS
+    Wv.pr[i].org = make_float4(o.x, o.y, o.z, l0);
+    Wv.pr[i].dir = make_float4(d.x, d.y, d.z, 0.0f);
+    Wv.pr[i].T = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    Wv.pr[i].aux = make_uint4(key, 2u, 0u, rid);  // the camera ray counts as specular for emitter MIS
 }
 
 __global__ void __launch_bounds__(kBlk) k_ext_raygen(ExtSceneDev S, ExtFrame F, ExtWave Wv) {
@@ -1238,7 +1249,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
         int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = idx < n;
         int p = active ? Wv.q[cur][idx] : 0;
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.aux[p].w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.pr[p].aux.w : 0u);
         const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, active, s_wq[threadIdx.x >> 6]);
         uint32_t ref = kMiss;
         if (active) {
@@ -1382,7 +1393,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
         float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sd = so, sx = so;
         if (ext) {
             p = Wv.q[cur][idx];
-            if (MULTI) rid = Wv.aux[p].w;
+            if (MULTI) rid = Wv.pr[p].aux.w;
         } else if (shadow) {
             const int i = idx - ne;
             so = Wv.s_org[i];
@@ -1419,7 +1430,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
     unsigned long long flops = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.q[cur][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
         f3 o, d;
         ray_of(Wv, cur, rq, idx, p, o, d);
         float2 h = Wv.hit[p];
@@ -1479,7 +1490,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
             const int pe = e >= 0 ? q[e] : -1;
             if (pe >= 0 && ((Wv.smask[pe] >> KIND) & 1u)) {
                 p = pe;
-                const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+                const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
                 sdfs = S.sdf;
                 nsdf = S.nsdf;
                 smax = S.sdf_steps_max;
@@ -1581,7 +1592,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         if (idx < n) {
             const int p = Wv.q[cur][idx];
-            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
             const uint32_t ref = __float_as_uint(Wv.hit[p].y);
             const int cls = hit_class(S, ref);
             bad += ref_is_bad(S, ref) ? 1 : 0;
@@ -1608,7 +1619,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
         int p = 0, cls = -1;
         if (idx < n) {
             p = Wv.q[cur][idx];
-            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
             cls = hit_class(S, __float_as_uint(Wv.hit[p].y));
         }
         uint64_t m[N_MATQ];
@@ -1644,12 +1655,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_aov(ExtSceneDev S0, ExtBatch B, Ex
     const int n = *qlen(Wv, 0);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const int p = Wv.q[0][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
         const float2 h = Wv.hit[p];
         const uint32_t ref = __float_as_uint(h.y);
         float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
         if (hit_class(S, ref) != Q_MISS) {
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            f3 o = xyz(Wv.pr[p].org), d = xyz(Wv.pr[p].dir);
             Surf sf = surface(S, ref, o, d, h.x);
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             f3 al = mk3(1.0f, 1.0f, 1.0f);
@@ -1716,19 +1727,19 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + which], (unsigned long long)n);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         int p = Wv.mq[which][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
-        f4 T = ld4(Wv.T, p);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
+        f4 T = f4of(Wv.pr[p].T);
         float l[4];
-        wavelengths(S, Wv.acc[p].w, l);
+        wavelengths(S, Wv.pr[p].org.w, l);
         if (which == Q_MISS) {
             if (S.has_env) add_xyz(Wv, p, to_xyz(S, l, T * S.env));
             continue;
         }
-        f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+        f3 o = xyz(Wv.pr[p].org), d = xyz(Wv.pr[p].dir);
         float2 h = Wv.hit[p];
         Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
         const ExtLight& L = S.light[sf.light];
-        uint4 aux = Wv.aux[p];
+        uint4 aux = Wv.pr[p].aux;
         float w = 1.0f;
         // emitters that NEE does not sample (lenses, cyclides) count with weight 1
         if (!(aux.y & 2u) && sf.light < S.nlight_nee) {
@@ -1774,21 +1785,22 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
         uint32_t rid = 0u;
         if (active) {
             p = Wv.mq[Q_LAMBERT][idx];
-            uint4 aux = Wv.aux[p];
+            uint4 aux = Wv.pr[p].aux;
             rid = aux.w;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
             const int maxDepth = MULTI ? B.frames[rid].maxDepth : F0.maxDepth;
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            const float4 org = Wv.pr[p].org;  // w: hero wavelength
+            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             const ExtMat& M = S.mat[sf.mat];
             float l[4];
-            wavelengths(S, Wv.acc[p].w, l);
+            wavelengths(S, org.w, l);
             const bool multi = S.n_lambda > 1;
             f4 rho = f4{reflectance(M, l[0]), 0.0f, 0.0f, 0.0f};
             if (multi) rho = f4{rho.x, reflectance(M, l[1]), reflectance(M, l[2]), reflectance(M, l[3])};
-            f4 T = ld4(Wv.T, p);
+            f4 T = f4of(Wv.pr[p].T);
             uint32_t key = aux.x;
             f3 P = sf.p + N * offset_scale(sf.p);
             if (S.nlight_nee > 0) {
@@ -1852,12 +1864,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
             T = T * rho;
             if (bounce + 1 < maxDepth && z > 0.0f && roulette(S, key, bounce, T)) {
                 cont = true;
-                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, org.w);
                 rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
-                Wv.org[p] = ro;
-                Wv.dir[p] = rd;
-                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), aux.w);
+                Wv.pr[p].org = ro;
+                Wv.pr[p].dir = rd;
+                Wv.pr[p].T = make_float4(T.x, T.y, T.z, T.w);
+                Wv.pr[p].aux = make_uint4(key, aux.y & ~2u, __float_as_uint(z * (1.0f / kPiF)), aux.w);
             }
         }
         int* const ctr[2] = {qlen(Wv, shadow_cnt(bounce)), qlen(Wv, nxt)};
@@ -1894,15 +1906,16 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
         float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
             p = Wv.mq[Q_DIELECTRIC][idx];
-            uint4 aux = Wv.aux[p];
+            uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            const float4 org = Wv.pr[p].org;  // w: hero wavelength
+            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
             const ExtMat& M = S.mat[sf.mat];
             uint32_t key = aux.x;
-            float l0 = Wv.acc[p].w;
+            float l0 = org.w;
             f3 N = sf.n;
             float ci = -dot(d, N);
             bool entering = ci > 0.0f;
@@ -1925,7 +1938,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
                 wo = normalize(d * er + N * (er * ci - ct));
                 P = sf.p - N * os;
             }
-            f4 T = ld4(Wv.T, p);
+            f4 T = f4of(Wv.pr[p].T);
             uint32_t fl = aux.y;
             if (M.dispersive && S.n_lambda > 1 && !(fl & 1u)) {
                 T = f4{T.x * (float)S.n_lambda, 0.0f, 0.0f, 0.0f};
@@ -1933,12 +1946,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
             }
             if (bounce + 1 < maxDepth && roulette(S, key, bounce, T)) {
                 cont = true;
-                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, l0);
                 rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
-                Wv.org[p] = ro;
-                Wv.dir[p] = rd;
-                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(key, fl | 2u, aux.z, aux.w);
+                Wv.pr[p].org = ro;
+                Wv.pr[p].dir = rd;
+                Wv.pr[p].T = make_float4(T.x, T.y, T.z, T.w);
+                Wv.pr[p].aux = make_uint4(key, fl | 2u, aux.z, aux.w);
             }
         }
         int* const ctr[1] = {qlen(Wv, nxt)};
@@ -1969,27 +1982,28 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
         float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
             p = Wv.mq[Q_CONDUCTOR][idx];
-            uint4 aux = Wv.aux[p];
+            uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
-            f3 o = ld3(Wv.org, p), d = ld3(Wv.dir, p);
+            const float4 org = Wv.pr[p].org;  // w: hero wavelength
+            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
             float2 h = Wv.hit[p];
             Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
             const ExtMat& M = S.mat[sf.mat];
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             float ci = -dot(d, N);
             float Fr = fresnel_conductor(ci, M.p[0], M.p[1]);
-            f4 T = ld4(Wv.T, p) * Fr;
+            f4 T = f4of(Wv.pr[p].T) * Fr;
             f3 wo = d + N * (2.0f * ci);
             f3 P = sf.p + N * offset_scale(sf.p);
             if (bounce + 1 < maxDepth && roulette(S, aux.x, bounce, T)) {
                 cont = true;
-                ro = make_float4(P.x, P.y, P.z, 0.0f);
+                ro = make_float4(P.x, P.y, P.z, org.w);
                 rd = make_float4(wo.x, wo.y, wo.z, 0.0f);
-                Wv.org[p] = ro;
-                Wv.dir[p] = rd;
-                Wv.T[p] = make_float4(T.x, T.y, T.z, T.w);
-                Wv.aux[p] = make_uint4(aux.x, aux.y | 2u, aux.z, aux.w);
+                Wv.pr[p].org = ro;
+                Wv.pr[p].dir = rd;
+                Wv.pr[p].T = make_float4(T.x, T.y, T.z, T.w);
+                Wv.pr[p].aux = make_uint4(aux.x, aux.y | 2u, aux.z, aux.w);
             }
         }
         int* const ctr[1] = {qlen(Wv, nxt)};
@@ -2123,7 +2137,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_extras(ExtSceneDev S0, ExtBatch B,
     const int n = *qlen(Wv, cur);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const int p = Wv.q[cur][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.aux[p].w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
         if (S.nlens + S.ncyc == 0) continue;
         f3 o, d;
         ray_of(Wv, cur, rq, idx, p, o, d);
@@ -2675,18 +2689,18 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     drop_graphs(st);  // they hold the old wave's addresses
     dev_free(ctx, st->mem);
     size_t n = (size_t)cap;
-    // per path: org, dir, T, acc, aux (16 B each), hit 8, queues 4 * (2 + N_MATQ),
-    // shadow org/dir/xyz 48, sums 12
+    // per path: record (org, dir, T, aux) 64, acc 16, hit 8, queues 4 * (2 + N_MATQ),
+    // shadow org/dir/xyz 48, first-hit features 32, SDF mask 4, sums 12,
     // queue-ordered rays 64
-    size_t per_path = 5 * 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12 + 64;
+    size_t per_path = 64 + 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12 + 64;
     // + 256-byte rounding of 26 arrays + the counters and cursors, one line each
     int rc = dev_alloc(ctx, st->mem, n * per_path + 28 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
     char* p = static_cast<char*>(st->mem.p);
     ExtWave& W = st->wave;
-    W.org = carve_e<float4>(p, n); W.dir = carve_e<float4>(p, n);
-    W.T = carve_e<float4>(p, n); W.acc = carve_e<float4>(p, n);
-    W.aux = carve_e<uint4>(p, n); W.hit = carve_e<float2>(p, n);
+    W.pr = carve_e<PathRec>(p, n);
+    W.acc = carve_e<float4>(p, n);
+    W.hit = carve_e<float2>(p, n);
     W.q[0] = carve_e<int>(p, n); W.q[1] = carve_e<int>(p, n);
     for (int c = 0; c < 2; ++c) {
         W.rq_org[c] = carve_e<float4>(p, n);
