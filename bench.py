@@ -53,21 +53,22 @@ FP32_PEAK_TFLOPS = 157.3
 #     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8
 #     + primitive material/light lookup 8 + material-queue entry 4      = 48 B
 #   in k_ext_intersect (hit only; k_ext_classify appends the material queues):
-#     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8    = 36 B
-#     (the ray is read as two 16-byte records kept in queue order, 32 B)
+#     extension ray (path index 4, origin/direction 24) + hit (t, primitive) 8 = 36 B
+#     (the ray is read as two 16-byte records kept in queue order, 32 B, so the
+#     kernel needs no path index, and the hit is written in queue order)
 #   in k_ext_classify: queue index 4 + hit 8 + material/light lookup 8
-#     + material-queue entry 4                                          = 24 B
+#     + material-queue entry (path, t, primitive) 12                    = 32 B
 #   shadow ray, in k_ext_trace / k_ext_shadow:
 #     record (origin 12, direction 12, tmax 4, XYZ 12, path 4) 44 + path XYZ read+write 24 = 68 B
 #   Lambert vertex, in k_ext_shade_lambert:
-#     queue 4 + ray 24 + hit 8 + throughput 16 + wavelength 4 + key 4 + flags 4
+#     queue entry (path, hit) 12 + ray 24 + throughput 16 + wavelength 4 + key 4 + flags 4
 #     + writes: ray 24, throughput 16, pdf 4, flags 4, shadow record 44, next-queue 4 = 160 B
 #     (a continuing ray is also written into its next-queue slot, 32 B more, so the
 #     intersect kernel reads it in queue order)
 BYTES_PER_UNIT = {
     "k_ext_trace": [(48, "segments"), (68, "shadow_segments")],
     "k_ext_intersect": [(36, "segments")],
-    "k_ext_classify": [(24, "segments")],
+    "k_ext_classify": [(32, "segments")],
     "k_ext_shadow": [(68, "shadow_segments")],
     "k_ext_shade_lambert": [(160, "lambert_hits")],
     # reference mode (SoA scalar records):

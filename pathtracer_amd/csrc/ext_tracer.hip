@@ -158,24 +158,36 @@ struct PathRec {
                  // w: request slot
 };
 
+// A material-queue entry: the path and its hit, so that the shading kernels
+// read the hit with the entry instead of gathering it.
+struct MatEntry {
+    int p;         // path index
+    float t;       // hit distance
+    uint32_t ref;  // primitive reference bits
+    int pad;
+};
+
 // State of one extended wave.
 struct ExtWave {
     PathRec* pr;   // by path index
     float4* acc;   // XYZ gathered so far (w unused)
-    float2* hit;   // (t, primitive reference bits)
+    // (t, primitive reference bits) by entry of the current extension queue:
+    // the traversal kernels write it in the order they take the entries and
+    // the classification reads it in that order
+    float2* hit;
     int* q[2];
     // the extension rays of queue c in queue order (entry e: the ray of path
     // q[c][e]), written where the shading kernels append, so the intersect
     // kernels read them coalesced; org / dir keep them by path for shading
     float4* rq_org[2];
     float4* rq_dir[2];
-    int* mq[N_MATQ];
+    MatEntry* mq[N_MATQ];
     float4* s_org; // shadow ray origin, w = tmax
     float4* s_dir; // direction, w = path index bits
     float4* s_xyz; // unoccluded contribution
     float4* aov0;  // first hit: albedo (xyz), distance (w)
     float4* aov1;  // first hit: shading normal (xyz)
-    uint32_t* smask;  // SDF scenes: bit k set when the ray enters the box of an SDF of kind k before its analytic hit
+    uint32_t* smask;  // SDF scenes, by entry like hit: bit k set when the ray enters the box of an SDF of kind k before its analytic hit
     float* sum;
     int* counts;   // N_CNT queue lengths, kCntStride apart
     int* cursor;   // N_CUR x kSeg persistent-kernel cursors, kCntStride apart
@@ -199,19 +211,13 @@ __device__ __forceinline__ const ExtSceneDev& scene_of(const ExtSceneDev& S0, co
     return MULTI ? B.scenes[rid] : S0;
 }
 
-// The ray of entry e (path p) of extension queue `cur`: from the queue-ordered
-// copy (rq != 0, every bounce after the first), else by path (bounce 0, whose
-// queue is the identity, so both read the same addresses in order).
-__device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int rq, int e, int p, f3& o, f3& d) {
-    if (rq) {
-        const float4 a = Wv.rq_org[cur][e], b = Wv.rq_dir[cur][e];
-        o = mk3(a.x, a.y, a.z);
-        d = mk3(b.x, b.y, b.z);
-    } else {
-        const float4 a = Wv.pr[p].org, b = Wv.pr[p].dir;
-        o = mk3(a.x, a.y, a.z);
-        d = mk3(b.x, b.y, b.z);
-    }
+// The ray of entry e of extension queue `cur`: from the queue-ordered copy
+// (rq != 0, every bounce after the first), else from the path's record
+// (bounce 0, whose queue is the identity: entry e is path e).
+__device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int rq, int e, f3& o, f3& d) {
+    const float4 a = rq ? Wv.rq_org[cur][e] : Wv.pr[e].org, b = rq ? Wv.rq_dir[cur][e] : Wv.pr[e].dir;
+    o = mk3(a.x, a.y, a.z);
+    d = mk3(b.x, b.y, b.z);
 }
 
 static __constant__ float c_cie[PT_CIE_SIZE] = {
@@ -1223,14 +1229,15 @@ __global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const Ext
 }
 
 // Appends the path to its material queue; every thread of the block calls it.
-__device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& Wv, bool active, int p, uint32_t ref) {
+__device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& Wv, bool active, int p, float t,
+                                                uint32_t ref) {
     const int cls = active ? hit_class(S, ref) : -1;
     count_bad_refs(Wv.stats, active && ref_is_bad(S, ref));
     int* const ctr[N_MATQ] = {qlen(Wv, CNT_MAT + 0), qlen(Wv, CNT_MAT + 1), qlen(Wv, CNT_MAT + 2),
                               qlen(Wv, CNT_MAT + 3), qlen(Wv, CNT_MAT + 4)};
     int slot[N_MATQ];
     block_append<kBlk, N_MATQ>(ctr, cls >= 0 ? 1u << cls : 0u, slot);
-    if (cls >= 0) Wv.mq[cls][slot[cls]] = p;
+    if (cls >= 0) Wv.mq[cls][slot[cls]] = MatEntry{p, t, ref, 0};
 }
 
 // Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
@@ -1248,19 +1255,20 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
     for (int r = 0; r < rounds; ++r) {
         int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool active = idx < n;
-        int p = active ? Wv.q[cur][idx] : 0;
+        // the path index: its request slot, its material-queue entry
+        int p = active && (MULTI || CLASSIFY) ? Wv.q[cur][idx] : 0;
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI && active ? Wv.pr[p].aux.w : 0u);
         const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, active, s_wq[threadIdx.x >> 6]);
         uint32_t ref = kMiss;
+        float t = 1e30f;
         if (active) {
             f3 o, d;
-            ray_of(Wv, cur, rq, idx, p, o, d);
-            float t = 1e30f;
+            ray_of(Wv, cur, rq, idx, o, d);
             scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, visits);
-            Wv.hit[p] = make_float2(t, __uint_as_float(ref));
-            if (S.nsdf > 0) Wv.smask[p] = sdf_mask(S, o, d, t);
+            Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
+            if (S.nsdf > 0) Wv.smask[idx] = sdf_mask(S, o, d, t);
         }
-        if (CLASSIFY) classify_append(S, Wv, active, p, ref);
+        if (CLASSIFY) classify_append(S, Wv, active, p, t, ref);
     }
     wave_add_u64(stat_slot(Wv.stats, 11), visits);
 }
@@ -1308,8 +1316,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
     const int nq = stage_quads(S, s_quad);
     const int n = *qlen(Wv, cur);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[0], (unsigned long long)n);
-    const int* q = Wv.q[cur];
-    int p = -1, sp = 0, node = -1, best = -1;
+    int p = -1, sp = 0, node = -1, best = -1;  // p: the lane's queue entry (-1 idle)
     unsigned visits = 0;
     f3 o = mk3(0.0f, 0.0f, 0.0f), d = o, inv = o, oi = o;
     float t = 0.0f;
@@ -1320,8 +1327,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
         if (!chunk.drained() && (__popcll(idle) >= kRefill || ~idle == 0ull)) {
             int e = wave_refill(idle, chunk);
             if (e >= 0) {
-                p = q[e];
-                ray_of(Wv, cur, rq, e, p, o, d);
+                p = e;
+                ray_of(Wv, cur, rq, e, o, d);
                 t = 1e30f;
                 ref = kMiss;
                 best = -1;
@@ -1390,6 +1397,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
         const bool ext = idx < ne, shadow = !ext && idx < n;
         int p = 0;
         uint32_t ref = kMiss, rid = 0u, srid = 0u;
+        float t = 1e30f;
         float4 so = make_float4(0.0f, 0.0f, 0.0f, 0.0f), sd = so, sx = so;
         if (ext) {
             p = Wv.q[cur][idx];
@@ -1405,18 +1413,17 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
         const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, ext || shadow, s_wq[threadIdx.x >> 6]);
         if (ext) {
             f3 o, d;
-            ray_of(Wv, cur, rq, idx, p, o, d);
-            float t = 1e30f;
+            ray_of(Wv, cur, rq, idx, o, d);
             scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, ev);
-            Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+            Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
         } else if (shadow) {
             f3 o = mk3(so.x, so.y, so.z), d = mk3(sd.x, sd.y, sd.z);
-            float t = so.w;
+            float ts = so.w;
             uint32_t sref = kMiss;
-            if (!scene_trace<W, STACK, SE, true>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, sref, stack, sv))
+            if (!scene_trace<W, STACK, SE, true>(S, staged, s_wq[threadIdx.x >> 6], o, d, ts, sref, stack, sv))
                 add_xyz(Wv, __float_as_int(sd.w), mk3(sx.x, sx.y, sx.z));
         }
-        classify_append(scene_of<MULTI>(S0, B, rid), Wv, ext, p, ref);
+        classify_append(scene_of<MULTI>(S0, B, rid), Wv, ext, p, t, ref);
     }
     wave_add_u64(stat_slot(Wv.stats, 11), ev);
     wave_add_u64(stat_slot(Wv.stats, 12), sv);
@@ -1429,11 +1436,10 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
     unsigned steps = 0, exhausted = 0;
     unsigned long long flops = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        int p = Wv.q[cur][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][idx]].aux.w : 0u);
         f3 o, d;
-        ray_of(Wv, cur, rq, idx, p, o, d);
-        float2 h = Wv.hit[p];
+        ray_of(Wv, cur, rq, idx, o, d);
+        float2 h = Wv.hit[idx];
         float t = h.x;
         uint32_t ref = __float_as_uint(h.y);
         bool upd = false;
@@ -1446,7 +1452,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
                 upd = true;
             }
         }
-        if (upd) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+        if (upd) Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
     }
     wave_add_u64(stat_slot(Wv.stats, 2), steps);
     wave_add_u64(stat_slot(Wv.stats, 13), exhausted);
@@ -1467,10 +1473,9 @@ constexpr int kMarchRefill = 8;
 template <int KIND, bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     const int n = *qlen(Wv, cur);
-    const int* q = Wv.q[cur];
     unsigned steps = 0, exhausted = 0, it = 0;
     unsigned long long flops = 0;
-    // Lane state: path p (-1 idle), its ray, nearest hit (t, ref), index s of
+    // Lane state: queue entry p (-1 idle), its ray, nearest hit (t, ref), index s of
     // the SDF being marched, and that march's position tm, exit tout, step
     // count and SDF parameters. `seek`: start the next SDF's march.
     int p = -1, s = -1, nsdf = 0, smax = 0, nst = 0;
@@ -1487,14 +1492,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
             const int e = wave_refill(idle, chunk);
             // rays that enter no box of this kind (the intersect kernel's mask)
             // go back to idle without loading their ray
-            const int pe = e >= 0 ? q[e] : -1;
-            if (pe >= 0 && ((Wv.smask[pe] >> KIND) & 1u)) {
-                p = pe;
-                const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
+            if (e >= 0 && ((Wv.smask[e] >> KIND) & 1u)) {
+                p = e;
+                const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][e]].aux.w : 0u);
                 sdfs = S.sdf;
                 nsdf = S.nsdf;
                 smax = S.sdf_steps_max;
-                ray_of(Wv, cur, rq, e, p, o, d);
+                ray_of(Wv, cur, rq, e, o, d);
                 const float2 h = Wv.hit[p];
                 t = h.x;
                 ref = __float_as_uint(h.y);
@@ -1591,9 +1595,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
     for (int r = 0; r < rounds; ++r) {
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         if (idx < n) {
-            const int p = Wv.q[cur][idx];
-            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
-            const uint32_t ref = __float_as_uint(Wv.hit[p].y);
+            const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][idx]].aux.w : 0u);
+            const uint32_t ref = __float_as_uint(Wv.hit[idx].y);
             const int cls = hit_class(S, ref);
             bad += ref_is_bad(S, ref) ? 1 : 0;
 #pragma unroll
@@ -1617,10 +1620,12 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
     for (int r = 0; r < rounds; ++r) {
         const int idx = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
         int p = 0, cls = -1;
+        float2 h = make_float2(0.0f, 0.0f);
         if (idx < n) {
             p = Wv.q[cur][idx];
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
-            cls = hit_class(S, __float_as_uint(Wv.hit[p].y));
+            h = Wv.hit[idx];
+            cls = hit_class(S, __float_as_uint(h.y));
         }
         uint64_t m[N_MATQ];
 #pragma unroll
@@ -1632,7 +1637,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_classify(ExtSceneDev S0, ExtBatch 
         if (cls >= 0) {
             int off = s_base[cls];
             for (int w = 0; w < wv; ++w) off += s_n[cls][w];
-            Wv.mq[cls][off + (int)lanes_below(m[cls])] = p;
+            Wv.mq[cls][off + (int)lanes_below(m[cls])] = MatEntry{p, h.x, __float_as_uint(h.y), 0};
         }
         __syncthreads();  // every thread has read s_base before it advances
         if ((int)threadIdx.x < N_MATQ) {
@@ -1656,7 +1661,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_aov(ExtSceneDev S0, ExtBatch B, Ex
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const int p = Wv.q[0][idx];
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
-        const float2 h = Wv.hit[p];
+        const float2 h = Wv.hit[idx];
         const uint32_t ref = __float_as_uint(h.y);
         float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
         if (hit_class(S, ref) != Q_MISS) {
@@ -1726,7 +1731,8 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
     const int n = *qlen(Wv, CNT_MAT + which);
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[6 + which], (unsigned long long)n);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        int p = Wv.mq[which][idx];
+        const MatEntry me = Wv.mq[which][idx];
+        const int p = me.p;
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
         f4 T = f4of(Wv.pr[p].T);
         float l[4];
@@ -1736,8 +1742,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
             continue;
         }
         f3 o = xyz(Wv.pr[p].org), d = xyz(Wv.pr[p].dir);
-        float2 h = Wv.hit[p];
-        Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+        Surf sf = surface(S, me.ref, o, d, me.t);
         const ExtLight& L = S.light[sf.light];
         uint4 aux = Wv.pr[p].aux;
         float w = 1.0f;
@@ -1747,7 +1752,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
             float pl;
             if (L.type == REF_QUAD) {
                 float cl = fabsf(dot(L.n, d));
-                pl = (h.x * h.x) / fmaxf(cl * L.area, 1e-30f);
+                pl = (me.t * me.t) / fmaxf(cl * L.area, 1e-30f);
             } else {
                 f3 dc = L.p - o;
                 float sin2 = (L.radius * L.radius) / dot(dc, dc);
@@ -1784,15 +1789,15 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
         float smax = 0.0f;
         uint32_t rid = 0u;
         if (active) {
-            p = Wv.mq[Q_LAMBERT][idx];
+            const MatEntry me = Wv.mq[Q_LAMBERT][idx];
+            p = me.p;
             uint4 aux = Wv.pr[p].aux;
             rid = aux.w;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
             const int maxDepth = MULTI ? B.frames[rid].maxDepth : F0.maxDepth;
             const float4 org = Wv.pr[p].org;  // w: hero wavelength
             f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
-            float2 h = Wv.hit[p];
-            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            Surf sf = surface(S, me.ref, o, d, me.t);
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             const ExtMat& M = S.mat[sf.mat];
             float l[4];
@@ -1905,14 +1910,14 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
         bool cont = false;
         float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
-            p = Wv.mq[Q_DIELECTRIC][idx];
+            const MatEntry me = Wv.mq[Q_DIELECTRIC][idx];
+            p = me.p;
             uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
             const float4 org = Wv.pr[p].org;  // w: hero wavelength
             f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
-            float2 h = Wv.hit[p];
-            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            Surf sf = surface(S, me.ref, o, d, me.t);
             const ExtMat& M = S.mat[sf.mat];
             uint32_t key = aux.x;
             float l0 = org.w;
@@ -1981,14 +1986,14 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
         bool cont = false;
         float4 ro = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rd = ro;  // the next ray, for its queue entry
         if (active) {
-            p = Wv.mq[Q_CONDUCTOR][idx];
+            const MatEntry me = Wv.mq[Q_CONDUCTOR][idx];
+            p = me.p;
             uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
             const float4 org = Wv.pr[p].org;  // w: hero wavelength
             f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
-            float2 h = Wv.hit[p];
-            Surf sf = surface(S, __float_as_uint(h.y), o, d, h.x);
+            Surf sf = surface(S, me.ref, o, d, me.t);
             const ExtMat& M = S.mat[sf.mat];
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             float ci = -dot(d, N);
@@ -2136,15 +2141,14 @@ template <bool MULTI>
 __global__ void __launch_bounds__(kBlk) k_ext_extras(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
     const int n = *qlen(Wv, cur);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
-        const int p = Wv.q[cur][idx];
-        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
+        const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][idx]].aux.w : 0u);
         if (S.nlens + S.ncyc == 0) continue;
         f3 o, d;
-        ray_of(Wv, cur, rq, idx, p, o, d);
-        const float2 h = Wv.hit[p];
+        ray_of(Wv, cur, rq, idx, o, d);
+        const float2 h = Wv.hit[idx];
         float t = h.x;
         uint32_t ref = __float_as_uint(h.y);
-        if (trace_extras<false>(S, o, d, t, ref)) Wv.hit[p] = make_float2(t, __uint_as_float(ref));
+        if (trace_extras<false>(S, o, d, t, ref)) Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
     }
 }
 // Lenses and cyclides against the shadow rays of counter `shq`, before the
@@ -2689,10 +2693,10 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
     drop_graphs(st);  // they hold the old wave's addresses
     dev_free(ctx, st->mem);
     size_t n = (size_t)cap;
-    // per path: record (org, dir, T, aux) 64, acc 16, hit 8, queues 4 * (2 + N_MATQ),
-    // shadow org/dir/xyz 48, first-hit features 32, SDF mask 4, sums 12,
-    // queue-ordered rays 64
-    size_t per_path = 64 + 16 + 8 + 4 * (2 + N_MATQ) + 48 + 32 + 4 + 12 + 64;
+    // per path: record (org, dir, T, aux) 64, acc 16, hit 8, extension queues 4 * 2,
+    // material-queue entries 16 * N_MATQ, shadow org/dir/xyz 48, first-hit
+    // features 32, SDF mask 4, sums 12, queue-ordered rays 64
+    size_t per_path = 64 + 16 + 8 + 4 * 2 + 16 * N_MATQ + 48 + 32 + 4 + 12 + 64;
     // + 256-byte rounding of 26 arrays + the counters and cursors, one line each
     int rc = dev_alloc(ctx, st->mem, n * per_path + 28 * 256 + (size_t)(N_CNT + N_CUR * kSeg) * kCntStride * 4);
     if (rc != PT_OK) return rc;
@@ -2706,7 +2710,7 @@ static int ensure_ext_wave(pt_ctx* ctx, int cap) {
         W.rq_org[c] = carve_e<float4>(p, n);
         W.rq_dir[c] = carve_e<float4>(p, n);
     }
-    for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<int>(p, n);
+    for (int c = 0; c < N_MATQ; ++c) W.mq[c] = carve_e<MatEntry>(p, n);
     W.s_org = carve_e<float4>(p, n); W.s_dir = carve_e<float4>(p, n); W.s_xyz = carve_e<float4>(p, n);
     W.aov0 = carve_e<float4>(p, n); W.aov1 = carve_e<float4>(p, n);
     W.smask = carve_e<uint32_t>(p, n);
