@@ -211,13 +211,18 @@ __device__ __forceinline__ const ExtSceneDev& scene_of(const ExtSceneDev& S0, co
     return MULTI ? B.scenes[rid] : S0;
 }
 
-// The ray of entry e of extension queue `cur`: from the queue-ordered copy
-// (rq != 0, every bounce after the first), else from the path's record
-// (bounce 0, whose queue is the identity: entry e is path e).
-__device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int rq, int e, f3& o, f3& d) {
-    const float4 a = rq ? Wv.rq_org[cur][e] : Wv.pr[e].org, b = rq ? Wv.rq_dir[cur][e] : Wv.pr[e].dir;
+// The ray of entry e of extension queue `cur`, from the queue-ordered records.
+__device__ __forceinline__ void ray_of(const ExtWave& Wv, int cur, int e, f3& o, f3& d) {
+    const float4 a = Wv.rq_org[cur][e], b = Wv.rq_dir[cur][e];
     o = mk3(a.x, a.y, a.z);
     d = mk3(b.x, b.y, b.z);
+}
+// The ray path p traces (w of the origin: the hero wavelength), from the
+// path's record: raygen and the shading kernels write each ray both there and
+// into its queue's records.
+__device__ __forceinline__ void path_ray(const ExtWave& Wv, int p, float4& org, float4& dir) {
+    org = Wv.pr[p].org;
+    dir = Wv.pr[p].dir;
 }
 
 static __constant__ float c_cie[PT_CIE_SIZE] = {
@@ -1118,7 +1123,7 @@ __device__ __forceinline__ float bk7_index(float lnm) {
 // Camera ray of sample k of pixel `pix` of frame F (scene S) into wave slot i,
 // request slot rid.
 __device__ __forceinline__ void gen_ray(const ExtSceneDev& S, const ExtFrame& F, int pix, int k, ExtWave& Wv, int i,
-                                        uint32_t rid) {
+                                        uint32_t rid, PathRec& rec) {
     int gx = pix % F.W, gy = pix / F.W;
     uint32_t sample = (uint32_t)(F.frame - F.spf + k);
     uint32_t key = path_key(S.seed, (uint32_t)pix, sample);
@@ -1182,19 +1187,64 @@ __device__ __forceinline__ void gen_ray(const ExtSceneDev& S, const ExtFrame& F,
             d = refract(d, nb, n12);
         }
     }
-    Wv.pr[i].org = make_float4(o.x, o.y, o.z, l0);
-    Wv.pr[i].dir = make_float4(d.x, d.y, d.z, 0.0f);
-    Wv.pr[i].T = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    // the camera ray in queue 0's records for the traversal, and in the path's
+    // record (stored by the caller) for the shading
+    Wv.rq_org[0][i] = make_float4(o.x, o.y, o.z, l0);
+    Wv.rq_dir[0][i] = make_float4(d.x, d.y, d.z, 0.0f);
     Wv.acc[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    Wv.pr[i].aux = make_uint4(key, 2u, 0u, rid);  // the camera ray counts as specular for emitter MIS
+    rec.org = make_float4(o.x, o.y, o.z, l0);
+    rec.dir = make_float4(d.x, d.y, d.z, 0.0f);
+    rec.T = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    rec.aux = make_uint4(key, 2u, 0u, rid);  // the camera ray counts as specular for emitter MIS
+}
+
+// Stores the records of the calling wave's active lanes, which hold
+// consecutive paths from the first lane's i on (a prefix of the wave), as
+// contiguous 16-byte vectors through the wave's LDS slice (64 records, padded
+// to 80 bytes a lane): stored lane by lane, a 64-byte record per lane is four
+// stores 64 bytes apart, which the memory system writes at half the rate of
+// contiguous ones. Every lane of the wave calls it.
+constexpr int kRecPad = 5;  // float4 per lane in the slice
+__device__ __forceinline__ void store_recs(PathRec* pr, int i, bool active, const PathRec& r, float4* slice) {
+    const int nact = (int)__popcll(__ballot(active));
+    if (nact == 0) return;
+    const int lane = (int)__lane_id();
+    const int i0 = __shfl(i, 0);
+    float4* mine = slice + lane * kRecPad;
+    mine[0] = r.org;
+    mine[1] = r.dir;
+    mine[2] = r.T;
+    mine[3] = make_float4(__uint_as_float(r.aux.x), __uint_as_float(r.aux.y), __uint_as_float(r.aux.z),
+                          __uint_as_float(r.aux.w));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float4* dst = reinterpret_cast<float4*>(pr + i0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int v = lane + 64 * k;  // vector v of the wave's records: record v / 4, field v % 4
+        if (v < 4 * nact) dst[v] = slice[(v >> 2) * kRecPad + (v & 3)];
+    }
+    // every lane has read the slice before it is written again
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __global__ void __launch_bounds__(kBlk) k_ext_raygen(ExtSceneDev S, ExtFrame F, ExtWave Wv) {
-    int n = F.pc * F.m;
+    __shared__ float4 s_rec[kBlk * kRecPad];
+    const int n = F.pc * F.m;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)n);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        gen_ray(S, F, F.p0 + i % F.pc, F.k0 + i / F.pc, Wv, i, 0u);
-        Wv.q[0][i] = i;
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (n + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {  // uniform rounds: every lane reaches store_recs
+        const int i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        PathRec rec{};
+        if (i < n) {
+            gen_ray(S, F, F.p0 + i % F.pc, F.k0 + i / F.pc, Wv, i, 0u, rec);
+            Wv.q[0][i] = i;
+        }
+        store_recs(Wv.pr, i, i < n, rec, s_rec + (threadIdx.x >> 6) * 64 * kRecPad);
     }
 }
 
@@ -1218,13 +1268,22 @@ __device__ __forceinline__ int seg_find(int n, int i, Start start) {
 
 __global__ void __launch_bounds__(kBlk) k_ext_raygen_batch(ExtBatch B, const ExtSeg* segs, const ExtBatchHdr* hdr,
                                                             ExtWave Wv) {
+    __shared__ float4 s_rec[kBlk * kRecPad];
     const int npaths = hdr->npaths, nseg = hdr->nseg;
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&Wv.stats[3], (unsigned long long)npaths);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < npaths; i += gridDim.x * blockDim.x) {
-        const ExtSeg& g = segs[seg_find(nseg, i, [&](int k) { return segs[k].base; })];
-        const int local = i - g.base;
-        gen_ray(B.scenes[g.rid], B.frames[g.rid], g.p0 + local % g.pc, g.k0 + local / g.pc, Wv, i, (uint32_t)g.rid);
-        Wv.q[0][i] = i;
+    const int stride = gridDim.x * blockDim.x;
+    const int rounds = (npaths + stride - 1) / stride;
+    for (int r = 0; r < rounds; ++r) {  // uniform rounds: every lane reaches store_recs
+        const int i = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        PathRec rec{};
+        if (i < npaths) {
+            const ExtSeg& g = segs[seg_find(nseg, i, [&](int k) { return segs[k].base; })];
+            const int local = i - g.base;
+            gen_ray(B.scenes[g.rid], B.frames[g.rid], g.p0 + local % g.pc, g.k0 + local / g.pc, Wv, i,
+                    (uint32_t)g.rid, rec);
+            Wv.q[0][i] = i;
+        }
+        store_recs(Wv.pr, i, i < npaths, rec, s_rec + (threadIdx.x >> 6) * 64 * kRecPad);
     }
 }
 
@@ -1243,7 +1302,7 @@ __device__ __forceinline__ void classify_append(const ExtSceneDev& S, ExtWave& W
 // Closest hit over parallelograms, spheres and planes; with CLASSIFY, the path
 // is also appended to its material queue (scenes without SDFs).
 template <int W, int STACK, typename SE, bool CLASSIFY, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     __shared__ SE stack_mem[STACK * kBlk];
     __shared__ float4 s_wq[kBlk / 64][4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
@@ -1263,7 +1322,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect(ExtSceneDev S0, ExtBatch
         float t = 1e30f;
         if (active) {
             f3 o, d;
-            ray_of(Wv, cur, rq, idx, o, d);
+            ray_of(Wv, cur, idx, o, d);
             scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, visits);
             Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
             if (S.nsdf > 0) Wv.smask[idx] = sdf_mask(S, o, d, t);
@@ -1309,7 +1368,7 @@ __device__ __forceinline__ int stage_quads(const ExtSceneDev& S, float4* s_quad)
 // atomic per wave and class for 64 paths, where appending as lanes finish
 // would take one per finishing lane).
 template <int W, int STACK, typename SE>
-__global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur, int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWave Wv, int cur) {
     __shared__ SE stack_mem[STACK * kBlk];
     __shared__ float4 s_quad[4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
@@ -1328,7 +1387,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
             int e = wave_refill(idle, chunk);
             if (e >= 0) {
                 p = e;
-                ray_of(Wv, cur, rq, e, o, d);
+                ray_of(Wv, cur, e, o, d);
                 t = 1e30f;
                 ref = kMiss;
                 best = -1;
@@ -1378,8 +1437,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_intersect_pt(ExtSceneDev S, ExtWav
 // fetches; one launch instead of two pays that once per bounce. Scenes without
 // SDF objects only (their shadow rays also march the SDFs).
 template <int W, int STACK, typename SE, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq,
-                                                     int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int shq) {
     __shared__ SE stack_mem[STACK * kBlk];
     __shared__ float4 s_wq[kBlk / 64][4 * kQuadBrute];
     SE* stack = stack_mem + threadIdx.x;
@@ -1413,7 +1471,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
         const bool staged = wave_quads(S.quad, S.quad_bvh ? 0 : S.nquad, ext || shadow, s_wq[threadIdx.x >> 6]);
         if (ext) {
             f3 o, d;
-            ray_of(Wv, cur, rq, idx, o, d);
+            ray_of(Wv, cur, idx, o, d);
             scene_trace<W, STACK, SE, false>(S, staged, s_wq[threadIdx.x >> 6], o, d, t, ref, stack, ev);
             Wv.hit[idx] = make_float2(t, __uint_as_float(ref));
         } else if (shadow) {
@@ -1431,14 +1489,14 @@ __global__ void __launch_bounds__(kBlk) k_ext_trace(ExtSceneDev S0, ExtBatch B, 
 
 // Sphere tracing of every SDF object of one kind against the current hit.
 template <int KIND, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     unsigned steps = 0, exhausted = 0;
     unsigned long long flops = 0;
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][idx]].aux.w : 0u);
         f3 o, d;
-        ray_of(Wv, cur, rq, idx, o, d);
+        ray_of(Wv, cur, idx, o, d);
         float2 h = Wv.hit[idx];
         float t = h.x;
         uint32_t ref = __float_as_uint(h.y);
@@ -1471,7 +1529,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march(ExtSceneDev S0, ExtBatch B, 
 constexpr int kMarchRefill = 8;
 
 template <int KIND, bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     unsigned steps = 0, exhausted = 0, it = 0;
     unsigned long long flops = 0;
@@ -1498,7 +1556,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_march_pt(ExtSceneDev S0, ExtBatch 
                 sdfs = S.sdf;
                 nsdf = S.nsdf;
                 smax = S.sdf_steps_max;
-                ray_of(Wv, cur, rq, e, o, d);
+                ray_of(Wv, cur, e, o, d);
                 const float2 h = Wv.hit[p];
                 t = h.x;
                 ref = __float_as_uint(h.y);
@@ -1665,7 +1723,7 @@ __global__ void __launch_bounds__(kBlk) k_ext_aov(ExtSceneDev S0, ExtBatch B, Ex
         const uint32_t ref = __float_as_uint(h.y);
         float4 a0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), a1 = a0;
         if (hit_class(S, ref) != Q_MISS) {
-            f3 o = xyz(Wv.pr[p].org), d = xyz(Wv.pr[p].dir);
+            f3 o = xyz(Wv.rq_org[0][p]), d = xyz(Wv.rq_dir[0][p]);  // bounce 0: the camera ray
             Surf sf = surface(S, ref, o, d, h.x);
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             f3 al = mk3(1.0f, 1.0f, 1.0f);
@@ -1735,13 +1793,15 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_terminal(ExtSceneDev S0, Ext
         const int p = me.p;
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[p].aux.w : 0u);
         f4 T = f4of(Wv.pr[p].T);
+        float4 org, dir;  // org.w: hero wavelength
+        path_ray(Wv, p, org, dir);
         float l[4];
-        wavelengths(S, Wv.pr[p].org.w, l);
+        wavelengths(S, org.w, l);
         if (which == Q_MISS) {
             if (S.has_env) add_xyz(Wv, p, to_xyz(S, l, T * S.env));
             continue;
         }
-        f3 o = xyz(Wv.pr[p].org), d = xyz(Wv.pr[p].dir);
+        f3 o = xyz(org), d = xyz(dir);
         Surf sf = surface(S, me.ref, o, d, me.t);
         const ExtLight& L = S.light[sf.light];
         uint4 aux = Wv.pr[p].aux;
@@ -1795,8 +1855,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_lambert(ExtSceneDev S0, ExtF
             rid = aux.w;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, rid);
             const int maxDepth = MULTI ? B.frames[rid].maxDepth : F0.maxDepth;
-            const float4 org = Wv.pr[p].org;  // w: hero wavelength
-            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
+            float4 org, dir;  // org.w: hero wavelength
+            path_ray(Wv, p, org, dir);
+            f3 o = xyz(org), d = xyz(dir);
             Surf sf = surface(S, me.ref, o, d, me.t);
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
             const ExtMat& M = S.mat[sf.mat];
@@ -1915,8 +1976,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_dielectric(ExtSceneDev S0, E
             uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
-            const float4 org = Wv.pr[p].org;  // w: hero wavelength
-            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
+            float4 org, dir;  // org.w: hero wavelength
+            path_ray(Wv, p, org, dir);
+            f3 o = xyz(org), d = xyz(dir);
             Surf sf = surface(S, me.ref, o, d, me.t);
             const ExtMat& M = S.mat[sf.mat];
             uint32_t key = aux.x;
@@ -1991,8 +2053,9 @@ __global__ void __launch_bounds__(kBlk) k_ext_shade_conductor(ExtSceneDev S0, Ex
             uint4 aux = Wv.pr[p].aux;
             const ExtSceneDev& S = scene_of<MULTI>(S0, B, aux.w);
             const int maxDepth = MULTI ? B.frames[aux.w].maxDepth : F0.maxDepth;
-            const float4 org = Wv.pr[p].org;  // w: hero wavelength
-            f3 o = xyz(org), d = xyz(Wv.pr[p].dir);
+            float4 org, dir;  // org.w: hero wavelength
+            path_ray(Wv, p, org, dir);
+            f3 o = xyz(org), d = xyz(dir);
             Surf sf = surface(S, me.ref, o, d, me.t);
             const ExtMat& M = S.mat[sf.mat];
             f3 N = dot(sf.n, d) < 0.0f ? sf.n : -sf.n;
@@ -2138,13 +2201,13 @@ __global__ void __launch_bounds__(kBlk) k_ext_shadow_pt(ExtSceneDev S, ExtWave W
 // SDF box mask was taken with the farther hit, so it can only name boxes a
 // march then bounds by the nearer one.
 template <bool MULTI>
-__global__ void __launch_bounds__(kBlk) k_ext_extras(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur, int rq) {
+__global__ void __launch_bounds__(kBlk) k_ext_extras(ExtSceneDev S0, ExtBatch B, ExtWave Wv, int cur) {
     const int n = *qlen(Wv, cur);
     for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         const ExtSceneDev& S = scene_of<MULTI>(S0, B, MULTI ? Wv.pr[Wv.q[cur][idx]].aux.w : 0u);
         if (S.nlens + S.ncyc == 0) continue;
         f3 o, d;
-        ray_of(Wv, cur, rq, idx, o, d);
+        ray_of(Wv, cur, idx, o, d);
         const float2 h = Wv.hit[idx];
         float t = h.x;
         uint32_t ref = __float_as_uint(h.y);
@@ -2758,14 +2821,14 @@ struct ExtPlan {
 constexpr int kMarchBlocksPerCu = 8;
 
 template <int KIND, bool MULTI>
-static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int rq, int grid) {
+static void launch_march(pt_ctx* ctx, const char* name, const ExtPlan& P, ExtWave& Wv, int cur, int grid) {
     if (ctx->cfg.march == 1) {
-        launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur, rq);
+        launch(ctx, name, k_ext_march<KIND, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, cur);
         return;
     }
     const int bpc = ctx->cfg.march_blocks_per_cu > 0 ? ctx->cfg.march_blocks_per_cu : kMarchBlocksPerCu;
     const int mgrid = std::max(1, std::min(grid, P.cus * bpc));
-    launch(ctx, name, k_ext_march_pt<KIND, MULTI>, dim3(mgrid), dim3(kBlk), P.S0, P.B, Wv, cur, rq);
+    launch(ctx, name, k_ext_march_pt<KIND, MULTI>, dim3(mgrid), dim3(kBlk), P.S0, P.B, Wv, cur);
 }
 
 // One bounce b (extension queue `cur`). Scenes without SDF objects trace the
@@ -2784,38 +2847,35 @@ static void launch_bounce(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int cur, i
     // lenses and cyclides are tested by their own kernels between the traversal
     // and the classification, which the fused kernel does in one
     const bool fused = !persistent && !P.any_sdf && !P.extras;
-    // bounce 0's queue is the identity over the camera rays, which raygen
-    // writes by path; later queues have their rays in queue order
-    const int rq = b > 0 ? 1 : 0;
     auto extras = [&]() {
-        if (P.extras) launch(ctx, "k_ext_extras", k_ext_extras<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur, rq);
+        if (P.extras) launch(ctx, "k_ext_extras", k_ext_extras<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     };
     launch(ctx, "k_ext_reset", k_ext_reset, dim3(1), dim3(64), Wv.counts, Wv.cursor, cur ^ 1, shadow_cnt(b));
     if (fused) {
         launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur,
-               shadow_cnt(b - 1), rq);
+               shadow_cnt(b - 1));
     } else if (P.any_sdf) {
         if (persistent)
-            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
+            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         else
             launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
-                   Wv, cur, rq);
+                   Wv, cur);
         extras();
         for (int kind : P.sdf_kinds) {
             switch (kind) {
-                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, rq, grid); break;
-                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER, MULTI>(ctx, "k_ext_march_menger", P, Wv, cur, rq, grid); break;
-                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB, MULTI>(ctx, "k_ext_march_blob", P, Wv, cur, rq, grid); break;
-                default: launch_march<PT_SDF_TERRAIN, MULTI>(ctx, "k_ext_march_terrain", P, Wv, cur, rq, grid); break;
+                case PT_SDF_MANDELBULB: launch_march<PT_SDF_MANDELBULB, MULTI>(ctx, "k_ext_march_mandelbulb", P, Wv, cur, grid); break;
+                case PT_SDF_MENGER: launch_march<PT_SDF_MENGER, MULTI>(ctx, "k_ext_march_menger", P, Wv, cur, grid); break;
+                case PT_SDF_BLOB: launch_march<PT_SDF_BLOB, MULTI>(ctx, "k_ext_march_blob", P, Wv, cur, grid); break;
+                default: launch_march<PT_SDF_TERRAIN, MULTI>(ctx, "k_ext_march_terrain", P, Wv, cur, grid); break;
             }
         }
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     } else {
         if (persistent)
-            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur, rq);
+            launch(ctx, "k_ext_intersect", k_ext_intersect_pt<W, STACK, SE>, dim3(grid), dim3(kBlk), S, Wv, cur);
         else
             launch(ctx, "k_ext_intersect", k_ext_intersect<W, STACK, SE, false, MULTI>, dim3(grid), dim3(kBlk), S, P.B,
-                   Wv, cur, rq);
+                   Wv, cur);
         extras();
         launch(ctx, "k_ext_classify", k_ext_classify<MULTI>, dim3(grid), dim3(kBlk), S, P.B, Wv, cur);
     }
@@ -2846,7 +2906,7 @@ static void launch_tail(pt_ctx* ctx, const ExtPlan& P, ExtWave& Wv, int nb, int 
     const bool persistent = !MULTI && ctx->cfg.traversal != 2 && !P.quad_bvh;
     if (persistent || P.any_sdf || P.extras || nb <= 0) return;
     launch(ctx, "k_ext_trace", k_ext_trace<W, STACK, SE, MULTI>, dim3(grid), dim3(kBlk), P.S0, P.B, Wv, nb & 1,
-           shadow_cnt(nb - 1), 1);
+           shadow_cnt(nb - 1));
 }
 
 template <int W, int STACK, typename SE>
