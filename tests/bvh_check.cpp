@@ -101,6 +101,8 @@ static float hit_sphere(const Sphere& s, const float o[3], const float d[3]) {
     return t > 1e-6f ? t : -1.0f;
 }
 
+constexpr uint32_t kRefMask = 0x5A5A0000u;  // references are index ^ mask
+
 int main(int argc, char** argv) {
     if (argc != 6) {
         std::fprintf(stderr, "usage: %s n width max_leaf layout seed\n", argv[0]);
@@ -143,7 +145,7 @@ int main(int argc, char** argv) {
             boxes[(size_t)i].lo[k] = sph[(size_t)i].c[k] - sph[(size_t)i].r;
             boxes[(size_t)i].hi[k] = sph[(size_t)i].c[k] + sph[(size_t)i].r;
         }
-        refs[(size_t)i] = (uint32_t)(1000003u * (uint32_t)i + 7u);  // distinct, not the index itself
+        refs[(size_t)i] = (uint32_t)i ^ kRefMask;  // distinct, not the index itself
     }
     const pt::WideBvh t = pt::build_wide_bvh(boxes, refs, W, max_leaf);
     if (t.width != W) return fail("width", t.width, W);
@@ -154,8 +156,8 @@ int main(int argc, char** argv) {
     std::vector<int> of_ref((size_t)n, -1);
     for (int j = 0; j < n; ++j) {
         const uint32_t r = t.prim_refs[(size_t)j];
-        if ((r - 7u) % 1000003u != 0) return fail("unknown reference", j, (long)r);
-        const uint32_t i = (r - 7u) / 1000003u;
+        const uint32_t i = r ^ kRefMask;
+        if (i >= (uint32_t)n) return fail("unknown reference", j, (long)r);
         if (i >= (uint32_t)n || of_ref[(size_t)j] != -1) return fail("bad reference", j, (long)r);
         of_ref[(size_t)j] = (int)i;
     }
