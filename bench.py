@@ -51,7 +51,7 @@ FP32_PEAK_TFLOPS = 157.3
 # divided by its launches.
 #   extension ray (segment), in k_ext_trace (which also classifies):
 #     queue index 4 + ray origin/direction 24 + hit (t, primitive) 8
-#     + primitive material/light lookup 8 + material-queue entry 4      = 48 B
+#     + primitive material/light lookup 8 + material-queue entry (path, t, primitive) 12 = 56 B
 #   in k_ext_intersect (hit only; k_ext_classify appends the material queues):
 #     extension ray (path index 4, origin/direction 24) + hit (t, primitive) 8 = 36 B
 #     (the ray is read as two 16-byte records kept in queue order, 32 B, so the
@@ -66,7 +66,7 @@ FP32_PEAK_TFLOPS = 157.3
 #     (a continuing ray is also written into its next-queue slot, 32 B more, so the
 #     intersect kernel reads it in queue order)
 BYTES_PER_UNIT = {
-    "k_ext_trace": [(48, "segments"), (68, "shadow_segments")],
+    "k_ext_trace": [(56, "segments"), (68, "shadow_segments")],
     "k_ext_intersect": [(36, "segments")],
     "k_ext_classify": [(32, "segments")],
     "k_ext_shadow": [(68, "shadow_segments")],
