@@ -126,7 +126,7 @@ typedef enum pt_mode { PT_MODE_REF = 0, PT_MODE_EXT = 1 } pt_mode;
 /* Launch/tuning knobs (0 = library default). */
 typedef struct pt_config {
     int32_t wave_paths;    /* paths in flight per wave (queue capacity); 0 = 2^27 for pt_dispatch (allocated as a
-                              dispatch needs, about 208 B per path) and 2^22 for a server's batched waves */
+                              dispatch needs, about 330 B per path) and 2^22 for a server's batched waves */
     int32_t block_size;    /* threads per block of the queue kernels: 0 or 256 (they are compiled for 256
                               lanes, 4 wavefronts; other values are rejected) */
     int32_t bvh_width;     /* 2, 4 or 8 (0 = 2) */
