@@ -164,19 +164,22 @@ def parse_args():
                     help="1: batched waves replayed from captured hipGraphs; 0: kernels launched one by one")
     ap.add_argument("--serve-wave", type=int, default=0,
                     help="paths per batched wave of the server's context, both systems (0 = the library's 2^22)")
-    ap.add_argument("--serve-max-active", type=int, default=0, help="requests the SRPT server traces together (0 = 16)")
+    ap.add_argument("--serve-max-active", type=int, default=0, help="requests the batching server traces together (0 = 16)")
     ap.add_argument("--serve-profile", action="store_true",
                     help="GPU time by kernel family in both systems (HIP events per launch, graphs off)")
     ap.add_argument("--serve-denoise", action="store_true",
-                    help="the SRPT server may finish requests with the shipped denoiser's image")
+                    help="the batching server may finish requests with the shipped denoiser's image")
     ap.add_argument("--serve-no-observe", action="store_true",
                     help="one-at-a-time p99 from the service times only (no real-time one-at-a-time run)")
     ap.add_argument("--serve-slice", type=int, default=24, metavar="N",
                     help="the default (spheres) bench line also carries p50/p99 time-to-RMSE of requests 0..N-1 "
-                         "of the serving stream, one at a time and SRPT+denoiser at 1x and 3x arrivals (0 = off)")
+                         "of the serving stream, one at a time and the batching server + denoiser at 1x and 3x arrivals (0 = off)")
+    ap.add_argument("--serve-policy", choices=("srpt", "hrrn"), default="hrrn",
+                    help="the served runs' ordering: highest response ratio (time in system + estimated "
+                         "remaining GPU time) / remaining first, or shortest estimated remaining GPU time first")
     ap.add_argument("--serve-speedup", type=str, default="1",
-                    help="comma-separated factors: the SRPT server's arrivals come this many times faster than the "
-                         "one-at-a-time ones (one SRPT run per factor)")
+                    help="comma-separated factors: the batching server's arrivals come this many times faster than the "
+                         "one-at-a-time ones (one served run per factor)")
     ap.add_argument("--denoiser", action="store_true",
                     help="denoiser workload instead: forward passes at --width x --height and training steps")
     ap.add_argument("--denoiser-eval", action="store_true",
@@ -427,7 +430,7 @@ def serving_measure(args, n_requests: int, speedups, denoise: bool, observe_fifo
        arrivals: its observed response times (time from arrival to the
        relative-RMSE target or to the sample budget).
     4. The same arrival sequence, compressed `speedup` times, served in real
-       time by the SRPT server (continuous batching; with `denoise` requests
+       time by the batching server (--serve-policy; continuous batching; with `denoise` requests
        may finish with the shipped denoiser's image).
     Returns the measurement as a dict."""
     import numpy as np
@@ -552,16 +555,17 @@ def serving_measure(args, n_requests: int, speedups, denoise: bool, observe_fifo
     fifo_conv = np.array([fifo_resp[i] for i in range(len(reqs)) if service[reqs[i]["id"]]["converged"]])
     p99_fifo = fifo_obs["p99_response_s"] if fifo_obs else percentile(fifo_resp, 99)
 
-    # 4. SRPT in real time, once per arrival speedup
+    # 4. the batching server in real time, once per arrival speedup
     runs = []
+    policy = pt.SERVE_HRRN if args.serve_policy == "hrrn" else pt.SERVE_SRPT
     for k in speedups:
-        runs.append(realtime(pt.SERVE_SRPT, float(k), args.serve_max_active, dn is not None))
-        log(f"SRPT at {k}x arrivals: p99 response {runs[-1]['p99_response_s']:.2f} s, converged "
-            f"{runs[-1]['converged']} ({time.perf_counter() - t_start:.0f} s)")
+        runs.append(realtime(policy, float(k), args.serve_max_active, dn is not None))
+        log(f"{args.serve_policy.upper()} at {k}x arrivals: p99 response {runs[-1]['p99_response_s']:.2f} s, "
+            f"converged {runs[-1]['converged']} ({time.perf_counter() - t_start:.0f} s)")
     if dn is not None:
         dn.close()
     ctx.close()
-    # the largest arrival speedup at which the SRPT server's p99 response is no
+    # the largest arrival speedup at which the batching server's p99 response is no
     # worse than the one-at-a-time server's at 1x
     equal_p99 = max([r["arrival_speedup"] for r in runs if r["p99_response_s"] <= p99_fifo], default=None)
     one = {"p50_time_to_rmse_s": percentile(fifo_conv, 50), "p99_time_to_rmse_s": percentile(fifo_conv, 99),
@@ -575,6 +579,7 @@ def serving_measure(args, n_requests: int, speedups, denoise: bool, observe_fifo
     return {"requests": len(reqs), "target_rel": args.serve_target, "load": args.serve_load,
             "arrival_rate_per_s": rate, "srpt_denoise": dn is not None,
             "wave_paths": args.serve_wave or (1 << 22), "srpt_max_active": args.serve_max_active or 16,
+            "served_policy": args.serve_policy,
             "p99_one_at_a_time_s": p99_fifo, "p99_basis": "observed" if fifo_obs else "from service times",
             "srpt": runs[-1], "srpt_runs": runs, "throughput_ratio_at_equal_p99": equal_p99,
             "one_at_a_time": one, "bench_seconds": time.perf_counter() - t_start}
@@ -587,21 +592,21 @@ SERVE_WORKLOAD = ("serving stream requests 0..{last}: 50% Cornell+N random spher
 
 def serve_bench(args):
     """Serving workload (requests 0..--serve - 1): prints one JSON line whose
-    value is the SRPT server's p50 time to the relative-RMSE target."""
+    value is the batching server's p50 time to the relative-RMSE target."""
     m = serving_measure(args, args.serve, args.serve_speedup.split(","), args.serve_denoise,
                         observe_fifo=not args.serve_no_observe)
     srpt = m["srpt"]
     line = {
         "metric": METRIC,
         "value": srpt["p50_time_to_rmse_s"],
-        "unit": "s (p50 time-to-RMSE<=1%, SRPT server)",
+        "unit": f"s (p50 time-to-RMSE<=1%, {args.serve_policy.upper()} server)",
         "n_gpus": 1,
         "higher_is_better": False,
         "data": "synthetic",
         "dtype": "fp32",
         "config": {"workload": SERVE_WORKLOAD.format(last=m["requests"] - 1),
                    **{k: m[k] for k in ("requests", "target_rel", "load", "arrival_rate_per_s", "srpt_denoise",
-                                        "wave_paths", "srpt_max_active")}},
+                                        "wave_paths", "srpt_max_active", "served_policy")}},
         **{k: m[k] for k in ("srpt", "srpt_runs", "throughput_ratio_at_equal_p99", "p99_one_at_a_time_s",
                              "p99_basis", "one_at_a_time", "bench_seconds")},
     }
@@ -958,7 +963,7 @@ def serving_slice(args):
     """The metric's second half on a bounded slice of the serving stream: p50 /
     p99 time to relative RMSE <= 1 % (over the requests that reach it) and
     response times, one request at a time (observed in real time) and with the
-    SRPT server plus denoiser at 1x and 3x the arrival rate."""
+    batching server (--serve-policy) plus denoiser at 1x and 3x the arrival rate."""
     m = serving_measure(args, args.serve_slice, ["1", "3"], denoise=True, observe_fifo=True)
     one = m["one_at_a_time"]["observed"]
     pick = ("p50_time_to_rmse_s", "p99_time_to_rmse_s", "converged", "p50_response_s", "p99_response_s",
@@ -968,8 +973,9 @@ def serving_slice(args):
             "p50_time_to_rmse_s": m["srpt_runs"][0]["p50_time_to_rmse_s"],
             "p99_time_to_rmse_s": m["srpt_runs"][0]["p99_time_to_rmse_s"],
             "one_at_a_time": {k: one[k] for k in pick if k in one},
-            "srpt_denoise_1x": {k: m["srpt_runs"][0][k] for k in pick},
-            "srpt_denoise_3x": {k: m["srpt_runs"][1][k] for k in pick},
+            "served_policy": args.serve_policy,
+            "served_denoise_1x": {k: m["srpt_runs"][0][k] for k in pick},
+            "served_denoise_3x": {k: m["srpt_runs"][1][k] for k in pick},
             "throughput_ratio_at_equal_p99": m["throughput_ratio_at_equal_p99"],
             "seconds": round(m["bench_seconds"], 1)}
 

@@ -340,6 +340,9 @@ typedef struct pt_server pt_server;
 typedef struct pt_denoiser pt_denoiser; /* the denoiser section below */
 #define PT_SERVE_SRPT 0 /* shortest estimated remaining GPU time first */
 #define PT_SERVE_FIFO 1 /* arrival order, one request to completion at a time */
+#define PT_SERVE_HRRN 2 /* highest response ratio first: (time since arrival + estimated remaining GPU
+                           time) / estimated remaining GPU time, so a long request's priority grows
+                           while it waits (shortest remaining first, with aging) */
 #define PT_SERVE_IDLE 1 /* pt_server_step status: no request to trace */
 #define PT_REQ_WAITING 0
 #define PT_REQ_ACTIVE 1
@@ -349,7 +352,7 @@ typedef struct pt_denoiser pt_denoiser; /* the denoiser section below */
 typedef struct pt_server_config {
     int64_t hbm_budget;    /* bytes of admitted scenes + moment buffers (0 = 64 GiB) */
     int32_t max_active;    /* requests admitted at once (0 = 16) */
-    int32_t policy;        /* PT_SERVE_SRPT or PT_SERVE_FIFO */
+    int32_t policy;        /* PT_SERVE_SRPT, PT_SERVE_FIFO or PT_SERVE_HRRN */
     int32_t min_spp;       /* samples per pixel before a request may be declared converged (0 = 8) */
     int32_t reserved;
 } pt_server_config;

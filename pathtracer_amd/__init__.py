@@ -188,7 +188,7 @@ class RequestStatus(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
-SERVE_SRPT, SERVE_FIFO, SERVE_IDLE = 0, 1, 1
+SERVE_SRPT, SERVE_FIFO, SERVE_HRRN, SERVE_IDLE = 0, 1, 2, 1
 REQ_WAITING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 # Every symbol include/pathtracer.h declares, with (restype, argtypes).

@@ -253,7 +253,7 @@ pt_status pt_server_create(pt_ctx* ctx, const pt_server_config* cfg, pt_server**
     if (s->cfg.hbm_budget <= 0) s->cfg.hbm_budget = (int64_t)64 << 30;
     if (s->cfg.max_active <= 0) s->cfg.max_active = 16;
     if (s->cfg.min_spp <= 0) s->cfg.min_spp = 8;
-    if (s->cfg.policy != PT_SERVE_FIFO) s->cfg.policy = PT_SERVE_SRPT;
+    if (s->cfg.policy != PT_SERVE_FIFO && s->cfg.policy != PT_SERVE_HRRN) s->cfg.policy = PT_SERVE_SRPT;
     s->t0 = Clock::now();
     (void)hipSetDevice(ctx->device);
     if (hipMalloc(&s->d_rel, (size_t)ptk::kMaxBatch * kRelLines * kRelStride * sizeof(double)) != hipSuccess) {
@@ -336,10 +336,24 @@ pt_status pt_server_step(pt_server* s, pt_request_status* stepped, int32_t cap, 
     // first, then the estimated remaining ones), while paths fit the wave;
     // FIFO traces only the oldest request
     std::vector<int> order = s->active;
-    if (s->cfg.policy == PT_SERVE_SRPT)
+    if (s->cfg.policy == PT_SERVE_SRPT) {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
             return remaining_cost(s, *s->reqs[a]) < remaining_cost(s, *s->reqs[b]);
         });
+    } else if (s->cfg.policy == PT_SERVE_HRRN) {
+        // response ratio (age + remaining) / remaining, largest first (Brinch
+        // Hansen's highest-response-ratio-next); equal ratios keep arrival order
+        const double now = now_s(s);
+        std::vector<std::pair<double, int>> key;
+        for (int id : order) {
+            const Request& r = *s->reqs[id];
+            const double rem = std::max(remaining_cost(s, r), 1e-9);
+            key.emplace_back(-(std::max(now - r.t_arrive, 0.0) + rem) / rem, id);
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
+        for (size_t i = 0; i < key.size(); ++i) order[i] = key[i].second;
+    }
     const int64_t wave = server_wave_paths(ctx);
     int64_t left = wave;
     std::vector<ptk::ExtBatchItem> items;

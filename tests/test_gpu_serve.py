@@ -98,19 +98,28 @@ def test_relative_error_estimate_tracks_the_converged_render(gpu_ctx):
     assert 0.6 * st.rel_error <= actual <= 1.5 * st.rel_error
 
 
-def test_srpt_finishes_small_requests_first(gpu_ctx):
+def test_policies_order_requests(gpu_ctx):
     big = scenes.random_spheres(n=1024, scene_seed=2, wavelengths=4, seed=1)
     small = no_caustic_cornell(seed=2)
     order = {}
-    for policy in (pt.SERVE_FIFO, pt.SERVE_SRPT):
+    # "aged": the big request arrived long before the small ones
+    for policy, aged in ((pt.SERVE_FIFO, False), (pt.SERVE_SRPT, False), (pt.SERVE_HRRN, False),
+                         (pt.SERVE_HRRN, True), (pt.SERVE_SRPT, True)):
         with pt.Server(gpu_ctx, policy=policy) as srv:
-            ids = [srv.submit(big, 256, 256, spp_max=256, path_length=8, target=1e-6)]
-            ids += [srv.submit(small, 32, 32, spp_max=64, path_length=8, target=1e-6) for _ in range(3)]
+            t = srv.now()
+            ids = [srv.submit(big, 256, 256, spp_max=256, path_length=8, target=1e-6,
+                              t_arrive=t - 1000.0 if aged else t)]
+            ids += [srv.submit(small, 32, 32, spp_max=64, path_length=8, target=1e-6, t_arrive=t) for _ in range(3)]
             st = run_until_done(srv, ids)
-            order[policy] = sorted(ids, key=lambda i: st[i].t_done)
+            order[policy, aged] = sorted(ids, key=lambda i: st[i].t_done)
             big_id = ids[0]
-    assert order[pt.SERVE_FIFO][0] == big_id  # one request at a time in arrival order
-    assert order[pt.SERVE_SRPT][-1] == big_id  # the small ones overtake it
+    assert order[pt.SERVE_FIFO, False][0] == big_id  # one request at a time in arrival order
+    assert order[pt.SERVE_SRPT, False][-1] == big_id  # the small ones overtake it
+    assert order[pt.SERVE_SRPT, True][-1] == big_id  # whatever its wait
+    # highest response ratio first: with equal waits the shorter remaining time
+    # leads; a request that has waited long enough leads whatever its size
+    assert order[pt.SERVE_HRRN, False][-1] == big_id
+    assert order[pt.SERVE_HRRN, True][0] == big_id
 
 
 def test_hbm_budget_limits_admission_and_release_frees(gpu_ctx):
